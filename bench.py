@@ -1,0 +1,182 @@
+"""Benchmark: one CLBM-IBM time step of the 600 x 200 x 50 moving NACA 0012 case.
+
+A step is what main_moving.py:116-122 does per iteration: host kinematics,
+``DroneMovingIBM.compute_force`` (marker update, interpolate + force + spread,
+clamp into fields) and ``CLBMSolver.step`` (fused collide + stream, inlet and
+outlet planes).  Populations, markers and forces are resident in HBM before
+the timed region starts.  Prints one JSON line.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--storage f32|f16|bf16|f64]
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import importlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+PKG = "clbm_ibm-moving-airfoil_amd"
+METRIC = "MLUPS + % HBM roofline, 600×200×50 CLBM-IBM; FP16-store vs FP32 @1e-3 CL err"
+HBM_PEAK_GBS = 8000.0
+ELEM = {"f32": 4, "f16": 2, "bf16": 2, "f64": 8}
+
+
+def parse(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--storage", default="f32", choices=sorted(ELEM))
+    ap.add_argument("--case", default="c1", choices=["c0", "c1", "c2"])
+    ap.add_argument("--kernel-launches", type=int, default=400,
+                    help="launches of the fused step kernel timed on their own")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="target duration of the CPU baseline sample (0 = skip)")
+    ap.add_argument("--cpu-threads", type=int, default=8)
+    ap.add_argument("--profile-only", action="store_true",
+                    help="run warmup + timed steps only (for rocprofv3)")
+    return ap.parse_args(argv)
+
+
+def cpu_baseline(cfg, seconds: float, threads: int):
+    """The reference-structured C restatement (oracle/, AoS double buffer,
+    12 kernels per step, float32) timed on the host on a bounded sample of the
+    same workload: the same grid and markers for as many steps as fit in
+    ``seconds``."""
+    if seconds <= 0:
+        return None
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    try:
+        import clbm_ref_c
+    except Exception as e:  # noqa: BLE001
+        return {"value": None, "unit": "MLUPS", "cores": threads, "kind": "port",
+                "sample": f"unavailable: {e}"}
+    if not clbm_ref_c.available():
+        return {"value": None, "unit": "MLUPS", "cores": threads, "kind": "port",
+                "sample": "oracle/libclbm_ref.so not built"}
+    sim = clbm_ref_c.RefSim(cfg, threads=threads)
+    t0 = time.perf_counter()
+    sim.run(1, first_step=1)
+    t1 = time.perf_counter() - t0
+    n = int(max(2, min(500, seconds / max(t1, 1e-6))))
+    t0 = time.perf_counter()
+    sim.run(n, first_step=2)
+    dt = time.perf_counter() - t0
+    ncell = cfg.Nx * cfg.Ny * cfg.Nz
+    return {"value": round(n * ncell / dt / 1e6, 2), "unit": "MLUPS", "cores": threads,
+            "kind": "port",
+            "sample": (f"{n} steps of the {cfg.Nx}x{cfg.Ny}x{cfg.Nz} case with its "
+                       f"{cfg.N_MARKERS} markers ({dt:.1f} s), reference-structured C/OpenMP "
+                       f"restatement of the 12 kernels, float32 AoS, {threads} threads")}
+
+
+def traffic_record(case: str, storage: str):
+    """HBM bytes per launch of the fused step kernel, from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs summarised in profiles/ (null when absent)."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        rec = json.load(fh)
+    return rec.get(f"{case}/{storage}")
+
+
+def main(argv=None) -> int:
+    a = parse(argv)
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {a.gpus}); the multi-GPU x-slab "
+              f"decomposition is a design, not a measured path.", file=sys.stderr)
+        return 2
+    import torch
+    pkg = importlib.import_module(PKG)
+    cfgmod = pkg.config
+    cfg = {"c0": cfgmod.case_c0, "c1": cfgmod.case_c1, "c2": cfgmod.case_c2_large}[a.case]()
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    lbm = pkg.CLBMSolver(cfg, storage=a.storage, device=dev)
+    ibm = pkg.DroneMovingIBM(cfg, device=dev)
+    ncell = cfg.Nx * cfg.Ny * cfg.Nz
+
+    def one(step):
+        xc, yc, th, vx, vy, om = cfg.motion_state(step)
+        fx, fy, fz = ibm.compute_force(lbm.ux, lbm.uy, xc, yc, th, vx, vy, om)
+        lbm.step(fx, fy, fz)
+
+    step = 1
+    for _ in range(a.warmup):
+        one(step)
+        step += 1
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        one(step)
+        step += 1
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    rho_min, finite = lbm.health()
+    if a.profile_only:
+        print(json.dumps({"steps": a.steps, "seconds": dt, "finite": finite}))
+        return 0
+
+    # fused step kernel alone: same state and force, the x planes left periodic
+    # so that each clbm_step call is exactly one launch of the fused kernel
+    lib = importlib.import_module(PKG + "._lib")
+    params = lib.ClbmParams(lbm.params.s_v, lbm.params.s_e, lbm.params.ux_in, lbm.params.uy_in, 0, 0)
+    stream = torch.cuda.current_stream(dev)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    fxp, fyp, boxp = ibm.fx._t.data_ptr(), ibm.fy._t.data_ptr(), ibm.box.data_ptr()
+    state = C.c_void_p(lbm.state.data_ptr())
+    sh = C.c_void_p(stream.cuda_stream)
+    par = lbm.parity
+    for i in range(20):
+        lib.call("clbm_step", C.byref(lbm.grid), C.byref(params), state, par, C.c_void_p(fxp),
+                 C.c_void_p(fyp), None, C.c_void_p(boxp), sh)
+        par ^= 1
+    e0.record(stream)
+    for i in range(a.kernel_launches):
+        lib.call("clbm_step", C.byref(lbm.grid), C.byref(params), state, par, C.c_void_p(fxp),
+                 C.c_void_p(fyp), None, C.c_void_p(boxp), sh)
+        par ^= 1
+    e1.record(stream)
+    e1.synchronize()
+    k_ms = e0.elapsed_time(e1) / a.kernel_launches
+    b_alg = 2 * 19 * ELEM[a.storage] * ncell
+    achieved = b_alg / (k_ms * 1e-3) / 1e9
+
+    mlups = a.steps * ncell / dt / 1e6
+    cpu = cpu_baseline(cfg, a.cpu_seconds, a.cpu_threads)
+    out = {
+        "metric": METRIC, "value": round(mlups, 1), "unit": "MLUPS", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(dt / a.steps * 1e3, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": {"f32": "f32", "f16": "f16-store/f32", "bf16": "bf16-store/f32",
+                  "f64": "f64"}[a.storage],
+        "data": "synthetic (feq initial state, prescribed kinematics; no dataset)",
+        "config": {"workload": f"{cfg.name}: D3Q19 cascaded LBM + Peskin IBM step, "
+                               f"{cfg.N_MARKERS} markers, Re {cfg.Re:g}, tau {cfg.tau:.3f}",
+                   "grid": [cfg.Nx, cfg.Ny, cfg.Nz], "markers": cfg.N_MARKERS,
+                   "storage": a.storage, "spread": "fixed-point", "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": "k_step (fused collide+force+stream)",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_record(a.case, a.storage),
+                     "kernel_ms": round(k_ms, 5),
+                     "bytes_per_launch": b_alg,
+                     "step_frac": round(mlups * 1e6 * 2 * 19 * ELEM[a.storage] / 1e9 / HBM_PEAK_GBS, 4)},
+        "cpu_baseline": cpu,
+        "health": {"rho_min": rho_min, "finite": finite},
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

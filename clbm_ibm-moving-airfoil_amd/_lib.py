@@ -1,0 +1,107 @@
+"""ctypes binding of libclbm_amd.so (the C ABI declared in include/clbm_amd.h).
+
+The library is loaded from this package's directory.  There is no fallback:
+if the shared object is missing or does not load, every entry point raises.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libclbm_amd.so")
+
+STORE_F32, STORE_F16, STORE_BF16, STORE_F64 = 0, 1, 2, 3
+STORAGE = {"f32": STORE_F32, "f16": STORE_F16, "bf16": STORE_BF16, "f64": STORE_F64}
+ELEM_BYTES = {STORE_F32: 4, STORE_F16: 2, STORE_BF16: 2, STORE_F64: 8}
+SPREAD_FIXED, SPREAD_ATOMIC_F32 = 0, 1
+SPREAD_MODES = {"fixed": SPREAD_FIXED, "atomic": SPREAD_ATOMIC_F32}
+BOX_HEADER = 16
+BOX_PARTIALS = 64
+BOX_INTS = BOX_HEADER + 6 * BOX_PARTIALS
+ABI_VERSION = 1
+
+
+class ClbmGrid(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("storage", C.c_int32)]
+
+
+class ClbmParams(C.Structure):
+    _fields_ = [("s_v", C.c_double), ("s_e", C.c_double), ("ux_in", C.c_double),
+                ("uy_in", C.c_double), ("inlet", C.c_int32), ("reserved", C.c_int32)]
+
+
+class IbmParams(C.Structure):
+    _fields_ = [("f_max", C.c_float), ("wrap_weight_quirk", C.c_int32), ("spread_mode", C.c_int32),
+                ("fixed_shift", C.c_int32)]
+
+
+_P = C.c_void_p
+_F = C.c_float
+_I = C.c_int
+_G = C.POINTER(ClbmGrid)
+_SIGNATURES = {
+    "clbm_abi_version": (C.c_int, []),
+    "clbm_last_error": (C.c_char_p, []),
+    "clbm_state_bytes": (C.c_size_t, [_G]),
+    "clbm_init_feq": (_I, [_G, _P, C.c_double, C.c_double, C.c_double, C.c_double, _P]),
+    "clbm_step": (_I, [_G, C.POINTER(ClbmParams), _P, _I, _P, _P, _P, _P, _P]),
+    "clbm_macro": (_I, [_G, _P, _I, _P, _P, _P, _P, _P]),
+    "clbm_export_state": (_I, [_G, _P, _I, _P, _P]),
+    "clbm_import_state": (_I, [_G, _P, _P, _P]),
+    "clbm_export_state_f64": (_I, [_G, _P, _I, _P, _P]),
+    "clbm_import_state_f64": (_I, [_G, _P, _P, _P]),
+    "clbm_health": (_I, [_G, _P, _I, _P, _P]),
+    "ibm_update_markers": (_I, [_G, _I, _P, _P, _P, _F, _F, _F, _F, _F, _F, _P, _P, _P, _P, _P, _P,
+                                _P]),
+    "ibm_band_velocity": (_I, [_G, _P, _I, _P, _P, _P, _P]),
+    "ibm_interp_force": (_I, [_G, C.POINTER(IbmParams), _P, _I, _P, _P, _I, _P, _P, _P, _P, _P,
+                              _P, _P, _P, _P, _P, _P]),
+    "ibm_spread": (_I, [_G, C.POINTER(IbmParams), _I, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "ibm_interp_spread": (_I, [_G, C.POINTER(IbmParams), _P, _I, _P, _P, _I, _P, _P, _P, _P, _P,
+                               _P, _P, _P, _P, _P, _P, _P, _P]),
+    "ibm_clip_forces": (_I, [_G, C.POINTER(IbmParams), _P, _P, _P, _P, _P, _P]),
+    "ibm_force_sum_f64": (_I, [_I, _P, _P, _P, _P, _P]),
+}
+
+_lock = threading.Lock()
+_lib = None
+
+
+class ClbmError(RuntimeError):
+    pass
+
+
+def load():
+    """Load the shared library once; raise if it is absent (no fallback path)."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise ClbmError(f"{LIB_PATH} is missing: build it with "
+                            f"`python {os.path.join(HERE, 'build.py')}` (hipcc, gfx950)")
+        lib = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        if lib.clbm_abi_version() != ABI_VERSION:
+            raise ClbmError("libclbm_amd.so ABI version mismatch")
+        _lib = lib
+        return lib
+
+
+def call(name: str, *args) -> None:
+    """Call an entry point and raise ClbmError with the library's message on failure."""
+    lib = load()
+    rc = getattr(lib, name)(*args)
+    if rc != 0:
+        msg = lib.clbm_last_error().decode(errors="replace")
+        raise ClbmError(f"{name} failed ({rc}): {msg}")
+
+
+def exported_symbols() -> list:
+    return list(_SIGNATURES)

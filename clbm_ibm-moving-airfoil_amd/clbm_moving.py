@@ -1,0 +1,181 @@
+"""CLBMSolver: the reference's solver object over the HIP kernels.
+
+Mirrors ``CLBMSolver`` of clbm_moving.py:41-343: constructed without
+arguments from the ``config_moving`` convention (or given a CaseConfig),
+advanced by ``step(fx, fy, fz)``, read through ``f, rho, ux, uy, uz`` and
+``M_inv`` with ``.to_numpy()``.
+
+Differences of mechanism (not of results):
+* one population buffer updated in place (AA pattern) instead of f/f_post;
+  ``parity`` records which of the two in-place layouts it currently holds;
+* ``rho, ux, uy, uz`` are computed on demand (K7 of clbm_moving.py:305-321)
+  when read, instead of after every step; the immersed-boundary object reads
+  the force-free velocity straight from the populations.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .config import CaseConfig, resolve
+from .fields import Field, HostConstant, as_device_ptr
+
+Q = 19
+
+
+def _m_raw_inv_f32() -> np.ndarray:
+    """M_raw^-1 in float32 (config_moving.py:62-73); exact dyadic entries."""
+    ex = [0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0]
+    ey = [0, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, 1, -1]
+    ez = [0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1]
+    pqr = [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), (1, 1, 0), (1, 0, 1), (0, 1, 1), (2, 0, 0),
+           (0, 2, 0), (0, 0, 2), (2, 1, 0), (1, 2, 0), (2, 0, 1), (0, 2, 1), (1, 0, 2), (0, 1, 2),
+           (2, 2, 0), (2, 0, 2), (0, 2, 2)]
+    m = np.array([[ex[c] ** p * ey[c] ** q * ez[c] ** r for c in range(Q)] for (p, q, r) in pqr],
+                 dtype=np.float64)
+    return np.linalg.inv(m).astype(np.float32)
+
+
+def _stream_handle(device) -> C.c_void_p:
+    import torch
+    return C.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+class PopulationField:
+    """``CLBMSolver.f``: (Nx, Ny, Nz, 19) post-stream populations on the host side."""
+
+    def __init__(self, solver: "CLBMSolver"):
+        self._s = solver
+        self.shape = (solver.Nx, solver.Ny, solver.Nz, Q)
+
+    def to_numpy(self) -> np.ndarray:
+        import torch
+        s = self._s
+        wide = s.storage == "f64"
+        out = torch.empty((Q, s.Nz, s.Ny, s.Nx), dtype=torch.float64 if wide else torch.float32,
+                          device=s.device)
+        _lib.call("clbm_export_state_f64" if wide else "clbm_export_state", C.byref(s.grid),
+                  C.c_void_p(s.state.data_ptr()), s.parity, C.c_void_p(out.data_ptr()),
+                  s._stream())
+        return np.ascontiguousarray(out.cpu().numpy().transpose(3, 2, 1, 0))
+
+    def from_numpy(self, arr) -> None:
+        import torch
+        s = self._s
+        wide = s.storage == "f64"
+        a = np.asarray(arr, dtype=np.float64 if wide else np.float32)
+        if a.shape != self.shape:
+            raise ValueError(f"shape {a.shape} != {self.shape}")
+        src = torch.from_numpy(np.ascontiguousarray(a.transpose(3, 2, 1, 0))).to(s.device)
+        _lib.call("clbm_import_state_f64" if wide else "clbm_import_state", C.byref(s.grid),
+                  C.c_void_p(s.state.data_ptr()), C.c_void_p(src.data_ptr()), s._stream())
+        s.parity = 0
+        s._macro_valid = False
+        if s.device.type == "cuda":
+            torch.cuda.current_stream(s.device).synchronize()
+
+
+class CLBMSolver:
+    """D3Q19 cascaded LBM with Guo forcing on an Nx x Ny x Nz lattice (HIP kernels).
+
+    Parameters mirror the reference's import-time constants; all are optional:
+    ``config`` (CaseConfig or config module; default: ``sys.modules
+    ['config_moving']`` or the reference airplane case), ``storage`` one of
+    f32 (drop-in), f16, bf16 (deviation storage) or f64 (error reference),
+    ``inlet`` False for an x-periodic box."""
+
+    def __init__(self, config=None, storage: str = "f32", device=None, inlet: bool = True):
+        import torch
+        cfg: CaseConfig = resolve(config)
+        self.cfg = cfg
+        self.Nx, self.Ny, self.Nz = int(cfg.Nx), int(cfg.Ny), int(cfg.Nz)
+        self.device = torch.device(device if device is not None else "cuda")
+        if storage not in _lib.STORAGE:
+            raise ValueError(f"storage must be one of {sorted(_lib.STORAGE)}")
+        self.storage = storage
+        _lib.load()
+        self.grid = _lib.ClbmGrid(self.Nx, self.Ny, self.Nz, _lib.STORAGE[storage])
+        self.params = _lib.ClbmParams(float(cfg.s_v), float(cfg.s_e), float(cfg.ux_in),
+                                      float(cfg.uy_in), 1 if inlet else 0, 0)
+        nbytes = Q * self.Nx * self.Ny * self.Nz * _lib.ELEM_BYTES[self.grid.storage]
+        self.state = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
+        self.parity = 0
+        self.steps = 0
+        shape = (self.Nx, self.Ny, self.Nz)
+        self._macro = None
+        self._macro_valid = False
+        self.rho = Field(shape, None, refresh=self._refresh_macro, owner=self)
+        self.ux = Field(shape, None, refresh=self._refresh_macro, owner=self)
+        self.uy = Field(shape, None, refresh=self._refresh_macro, owner=self)
+        self.uz = Field(shape, None, refresh=self._refresh_macro, owner=self)
+        self.f = PopulationField(self)
+        self.M_inv = HostConstant(_m_raw_inv_f32())
+        self._init_feq()
+
+    # -- reference surface ------------------------------------------------
+    def _init_feq(self) -> None:
+        """K1 (clbm_moving.py:75-91): equilibrium at the inlet velocity, rho = 1."""
+        _lib.call("clbm_init_feq", C.byref(self.grid), C.c_void_p(self.state.data_ptr()), 1.0,
+                  float(self.cfg.ux_in), float(self.cfg.uy_in), 0.0, self._stream())
+        self.parity = 0
+        self._macro_valid = False
+
+    def step(self, fx=None, fy=None, fz=None) -> None:
+        """Advance one step with body force (fx, fy, fz) (clbm_moving.py:326-343)."""
+        shape = (self.Nx, self.Ny, self.Nz)
+        box = None
+        for fld in (fx, fy):
+            if isinstance(fld, Field) and fld.box is not None:
+                box = fld.box
+        fzp = as_device_ptr(fz, shape)
+        if isinstance(fz, Field) and fz.box is not None:
+            fzp = 0     # the immersed-boundary fz is identically zero (airfoil_moving.py:225)
+        _lib.call("clbm_step", C.byref(self.grid), C.byref(self.params),
+                  C.c_void_p(self.state.data_ptr()), self.parity,
+                  C.c_void_p(as_device_ptr(fx, shape)), C.c_void_p(as_device_ptr(fy, shape)),
+                  C.c_void_p(fzp), C.c_void_p(box.data_ptr() if box is not None else 0),
+                  self._stream())
+        self.parity ^= 1
+        self.steps += 1
+        self._macro_valid = False
+
+    @property
+    def f_post(self):
+        raise AttributeError("the in-place (AA) population layout keeps no separate post-collision "
+                             "buffer; read CLBMSolver.f for the post-stream state")
+
+    # -- readout ------------------------------------------------------------
+    def _refresh_macro(self) -> None:
+        import torch
+        if self._macro_valid:
+            return
+        if self._macro is None:
+            self._macro = torch.empty((4, self.Nz, self.Ny, self.Nx), dtype=torch.float32,
+                                      device=self.device)
+            for i, fld in enumerate((self.rho, self.ux, self.uy, self.uz)):
+                fld._t = self._macro[i]
+        m = self._macro
+        _lib.call("clbm_macro", C.byref(self.grid), C.c_void_p(self.state.data_ptr()), self.parity,
+                  C.c_void_p(m[0].data_ptr()), C.c_void_p(m[1].data_ptr()),
+                  C.c_void_p(m[2].data_ptr()), C.c_void_p(m[3].data_ptr()), self._stream())
+        self._macro_valid = True
+
+    def health(self):
+        """(rho_min, all_finite) of the current state, computed on the device
+        (the divergence guard of main_moving.py:143-145)."""
+        import torch
+        out = torch.empty(2, dtype=torch.float32, device=self.device)
+        _lib.call("clbm_health", C.byref(self.grid), C.c_void_p(self.state.data_ptr()),
+                  self.parity, C.c_void_p(out.data_ptr()), self._stream())
+        h = out.cpu().numpy()
+        return float(h[0]), bool(h[1] > 0.5)
+
+    def _stream(self) -> C.c_void_p:
+        return _stream_handle(self.device)
+
+    @property
+    def ncell(self) -> int:
+        return self.Nx * self.Ny * self.Nz

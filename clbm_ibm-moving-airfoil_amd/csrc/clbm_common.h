@@ -1,0 +1,186 @@
+// Shared device/host definitions for the CLBM-IBM kernels (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_bf16.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/clbm_amd.h"
+
+namespace clbm {
+
+// ---------------------------------------------------------------------------
+// D3Q19 lattice in the reference's direction order (clbm_moving.py:21-30).
+// ---------------------------------------------------------------------------
+constexpr int Q = 19;
+__host__ __device__ constexpr int ex(int q) {
+  constexpr int v[Q] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
+  return v[q];
+}
+__host__ __device__ constexpr int ey(int q) {
+  constexpr int v[Q] = {0, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, 1, -1};
+  return v[q];
+}
+__host__ __device__ constexpr int ez(int q) {
+  constexpr int v[Q] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1};
+  return v[q];
+}
+// opposite direction: e_opp(q) = -e_q
+__host__ __device__ constexpr int opp(int q) {
+  constexpr int v[Q] = {0, 2, 1, 4, 3, 6, 5, 10, 9, 8, 7, 14, 13, 12, 11, 18, 17, 16, 15};
+  return v[q];
+}
+template <typename R>
+__host__ __device__ constexpr R wq(int q) {
+  return q == 0 ? R(1) / R(3) : (q < 7 ? R(1) / R(18) : R(1) / R(36));
+}
+
+// ---------------------------------------------------------------------------
+// Population storage.  Each format maps a stored element to the compute type
+// R and back.  16-bit formats hold the deviation f - w.
+// ---------------------------------------------------------------------------
+template <int S> struct Store;
+template <> struct Store<CLBM_STORE_F32> {
+  using T = float;
+  using R = float;
+  __device__ static R ld(const T* p, int) { return *p; }
+  __device__ static void st(T* p, int, R v) { *p = v; }
+};
+template <> struct Store<CLBM_STORE_F64> {
+  using T = double;
+  using R = double;
+  __device__ static R ld(const T* p, int) { return *p; }
+  __device__ static void st(T* p, int, R v) { *p = v; }
+};
+template <> struct Store<CLBM_STORE_F16> {
+  using T = __half;
+  using R = float;
+  __device__ static R ld(const T* p, int q) { return wq<float>(q) + __half2float(*p); }
+  __device__ static void st(T* p, int q, R v) { *p = __float2half_rn(v - wq<float>(q)); }
+};
+template <> struct Store<CLBM_STORE_BF16> {
+  using T = __hip_bfloat16;
+  using R = float;
+  __device__ static R ld(const T* p, int q) { return wq<float>(q) + __bfloat162float(*p); }
+  __device__ static void st(T* p, int q, R v) { *p = __float2bfloat16(v - wq<float>(q)); }
+};
+
+inline size_t elem_bytes(int storage) {
+  switch (storage) {
+    case CLBM_STORE_F32: return 4;
+    case CLBM_STORE_F64: return 8;
+    case CLBM_STORE_F16:
+    case CLBM_STORE_BF16: return 2;
+    default: return 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Unsigned division by a run-time constant (round-up multiplier method):
+// q = (umulhi(n, m) + n) >> s, exact for every n < 2^31.
+// ---------------------------------------------------------------------------
+struct FastDiv {
+  uint32_t d, m, s;
+};
+inline FastDiv make_fastdiv(uint32_t d) {
+  FastDiv r{d, 0, 0};
+  while ((1ull << r.s) < d) r.s++;
+  r.m = (uint32_t)((((1ull << 32) * ((1ull << r.s) - d)) / d) + 1);
+  return r;
+}
+__device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv& f) {
+  return (__umulhi(n, f.m) + n) >> f.s;
+}
+
+// Lattice geometry passed by value to every kernel.
+struct Geo {
+  int nx, ny, nz;
+  int nxy;        // nx*ny
+  int ncell;      // nx*ny*nz
+  FastDiv dx;     // divide by nx
+  FastDiv dy;     // divide by ny
+};
+inline Geo make_geo(const clbm_grid_t& g) {
+  Geo o;
+  o.nx = g.nx;
+  o.ny = g.ny;
+  o.nz = g.nz;
+  o.nxy = g.nx * g.ny;
+  o.ncell = g.nx * g.ny * g.nz;
+  o.dx = make_fastdiv((uint32_t)g.nx);
+  o.dy = make_fastdiv((uint32_t)g.ny);
+  return o;
+}
+
+// Linear offsets from a cell to its neighbours along +-x/y/z with periodic wrap.
+struct Nbr {
+  int xm, xp, ym, yp, zm, zp;
+  __device__ __forceinline__ int off(int dxs, int dys, int dzs) const {
+    int o = 0;
+    o += dxs > 0 ? xp : (dxs < 0 ? xm : 0);
+    o += dys > 0 ? yp : (dys < 0 ? ym : 0);
+    o += dzs > 0 ? zp : (dzs < 0 ? zm : 0);
+    return o;
+  }
+};
+__device__ __forceinline__ Nbr make_nbr(const Geo& g, int x, int y, int z) {
+  Nbr n;
+  n.xm = (x == 0) ? (g.nx - 1) : -1;
+  n.xp = (x == g.nx - 1) ? -(g.nx - 1) : 1;
+  n.ym = (y == 0) ? (g.ny - 1) * g.nx : -g.nx;
+  n.yp = (y == g.ny - 1) ? -(g.ny - 1) * g.nx : g.nx;
+  n.zm = (z == 0) ? (g.nz - 1) * g.nxy : -g.nxy;
+  n.zp = (z == g.nz - 1) ? -(g.nz - 1) * g.nxy : g.nxy;
+  return n;
+}
+__device__ __forceinline__ void cell_xyz(const Geo& g, int c, int& x, int& y, int& z) {
+  uint32_t row = fdiv((uint32_t)c, g.dx);     // z*ny + y
+  x = c - (int)row * g.nx;
+  z = (int)fdiv(row, g.dy);
+  y = (int)row - z * g.ny;
+}
+
+// Block index remap so that consecutive logical tiles run on the same XCD
+// (the dispatcher hands workgroups to the 8 XCDs round robin).
+__device__ __forceinline__ int xcd_block(int bid, int nblk) {
+  constexpr int NX = 8;
+  int per = nblk / NX, rem = nblk % NX;
+  int xcd = bid % NX, loc = bid / NX;
+  return (xcd < rem) ? xcd * (per + 1) + loc : rem * (per + 1) + (xcd - rem) * per + loc;
+}
+
+// Address (element index) of post-stream population q of cell c in an AA
+// state of parity PAR: parity 0 keeps it at [q][c], parity 1 at
+// [opp(q)][c - e_q].
+template <int PAR>
+__device__ __forceinline__ size_t slot_in(int q, size_t N, int c, const Nbr& nb) {
+  if (PAR == 0) return (size_t)q * N + c;
+  return (size_t)opp(q) * N + (c + nb.off(-ex(q), -ey(q), -ez(q)));
+}
+// Address where the post-collision population q of cell c is written by a
+// step that starts from parity PAR: [opp(q)][c] for PAR 0, [q][c + e_q] for PAR 1.
+template <int PAR>
+__device__ __forceinline__ size_t slot_out(int q, size_t N, int c, const Nbr& nb) {
+  if (PAR == 0) return (size_t)opp(q) * N + c;
+  return (size_t)q * N + (c + nb.off(ex(q), ey(q), ez(q)));
+}
+
+template <int S, int PAR>
+__device__ __forceinline__ void load_state(const typename Store<S>::T* __restrict__ f, size_t N,
+                                           int c, const Nbr& nb, typename Store<S>::R* fq) {
+#pragma unroll
+  for (int q = 0; q < Q; ++q) fq[q] = Store<S>::ld(f + slot_in<PAR>(q, N, c, nb), q);
+}
+
+// ---------------------------------------------------------------------------
+// Errors
+// ---------------------------------------------------------------------------
+void set_error(const std::string& s);
+int fail(int code, const std::string& s);
+int check_launch(const char* what);
+int check_grid(const clbm_grid_t* g, bool need_inlet_width);
+
+}  // namespace clbm

@@ -1,0 +1,474 @@
+// Lattice-Boltzmann kernels of the CLBM-IBM step for gfx950 (MI355X).
+//
+// One fused pass per step replaces K2 _macro_with_force, K3 _collide and
+// K4 _stream of CLBMSolver.step (clbm_moving.py:326-343): each thread loads
+// the 19 post-stream populations of its cell, applies the force, collides
+// and writes the post-collision values where the next step will read them as
+// post-stream values (AA pattern, in place, one population buffer).  A small
+// second launch applies K5 _bc_inlet and K6 _bc_outlet (clbm_moving.py:
+// 266-300) on the two x planes, so the stored state between steps is always
+// the reference's post-BC state.
+#include "clbm_collide.h"
+
+#include <cmath>
+#include <cstdio>
+#include <mutex>
+
+namespace clbm {
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_last_error;
+void set_error(const std::string& s) { g_last_error = s; }
+int fail(int code, const std::string& s) {
+  g_last_error = s;
+  return code;
+}
+int check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail((int)e, std::string(what) + ": " + hipGetErrorString(e));
+  return 0;
+}
+int check_grid(const clbm_grid_t* g, bool need_inlet_width) {
+  if (!g) return fail(-1, "grid is NULL");
+  if (g->nx < 1 || g->ny < 1 || g->nz < 1) return fail(-1, "grid sizes must be positive");
+  if (need_inlet_width && g->nx < 3) return fail(-1, "the inlet/outlet rule needs nx >= 3");
+  long long nc = (long long)g->nx * g->ny * g->nz;
+  if (nc >= (1ll << 31)) return fail(-1, "grid has 2^31 cells or more");
+  if (elem_bytes(g->storage) == 0) return fail(-1, "unknown storage format");
+  return 0;
+}
+
+constexpr int BLK = 256;
+
+// ---------------------------------------------------------------------------
+// K1: equilibrium initialisation (clbm_moving.py:75-91)
+// ---------------------------------------------------------------------------
+template <int S>
+__global__ __launch_bounds__(BLK) void k_init_feq(Geo g, typename Store<S>::T* __restrict__ f,
+                                                  double rho0, double ux0, double uy0, double uz0) {
+  using R = typename Store<S>::R;
+  const int c = blockIdx.x * BLK + threadIdx.x;
+  if (c >= g.ncell) return;
+  const size_t N = (size_t)g.ncell;
+  const R cs2 = R(1) / R(3);
+  const R u = R(ux0), v = R(uy0), w = R(uz0), r = R(rho0);
+  const R u2 = u * u + v * v + w * w;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const R eu = R(ex(q)) * u + R(ey(q)) * v + R(ez(q)) * w;
+    const R fe = wq<R>(q) * r * (R(1) + eu / cs2 + R(0.5) * eu * eu / (cs2 * cs2) - R(0.5) * u2 / cs2);
+    Store<S>::st(f + (size_t)q * N + c, q, fe);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fused macro + collide + Guo + stream (AA pattern).
+// PAR = parity of the incoming state.  BOX: force reads limited to a box.
+// ---------------------------------------------------------------------------
+template <int S, int PAR, bool FORCE, bool BOX>
+__global__ __launch_bounds__(BLK) void k_step(Geo g, double s_v, double s_e,
+                                              typename Store<S>::T* __restrict__ f,
+                                              const float* __restrict__ fx,
+                                              const float* __restrict__ fy,
+                                              const float* __restrict__ fz,
+                                              const int32_t* __restrict__ box) {
+  using R = typename Store<S>::R;
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int c = lb * BLK + threadIdx.x;
+  if (c >= g.ncell) return;
+  int x, y, z;
+  cell_xyz(g, c, x, y, z);
+  const Nbr nb = make_nbr(g, x, y, z);
+  const size_t N = (size_t)g.ncell;
+
+  R fq[Q];
+  load_state<S, PAR>(f, N, c, nb, fq);
+
+  R Fx = R(0), Fy = R(0), Fz = R(0);
+  if (FORCE) {
+    bool in = true;
+    if (BOX) {
+      in = (x >= box[0]) & (y >= box[1]) & (z >= box[2]) & (x <= box[3]) & (y <= box[4]) &
+           (z <= box[5]);
+    }
+    if (in) {
+      if (fx) Fx = R(fx[c]);
+      if (fy) Fy = R(fy[c]);
+      if (fz) Fz = R(fz[c]);
+    }
+  }
+  collide_cell<R>(fq, Fx, Fy, Fz, R(s_v), R(s_e));
+
+#pragma unroll
+  for (int q = 0; q < Q; ++q) Store<S>::st(f + slot_out<PAR>(q, N, c, nb), q, fq[q]);
+}
+
+// ---------------------------------------------------------------------------
+// K5 inlet rule + K6 outlet copy on the x planes of a state of parity PAR.
+// Threads [0, ny*nz) apply the inlet rule to cell (0,y,z); threads
+// [ny*nz, 2*ny*nz) copy cell (nx-2,y,z) to (nx-1,y,z).  The inlet touches
+// only cell (0,y,z)'s own slots, the outlet reads cell (nx-2,y,z)'s and
+// writes cell (nx-1,y,z)'s, so no slot is shared between threads.  Every
+// thread loads all it needs before it stores anything.
+// ---------------------------------------------------------------------------
+constexpr int BCBLK = 64;
+template <int S, int PAR>
+__global__ __launch_bounds__(BCBLK) void k_bc(Geo g, double ux_in_f, double uy_in_f,
+                                              typename Store<S>::T* __restrict__ f) {
+  using R = typename Store<S>::R;
+  using T = typename Store<S>::T;
+  const int nyz = g.ny * g.nz;
+  int t = blockIdx.x * BCBLK + threadIdx.x;
+  if (t >= 2 * nyz) return;
+  const bool inlet = t < nyz;
+  if (!inlet) t -= nyz;
+  const int z = t / g.ny, y = t - z * g.ny;
+  const size_t N = (size_t)g.ncell;
+  if (inlet) {  // clbm_moving.py:272-294
+    const R ux_in = R(ux_in_f), uy_in = R(uy_in_f);
+    const int c = (z * g.ny + y) * g.nx;
+    const Nbr nb = make_nbr(g, 0, y, z);
+    R a[Q];
+    load_state<S, PAR>(f, N, c, nb, a);
+    const R s_known = a[0] + a[3] + a[4] + a[5] + a[6] + a[15] + a[16] + a[17] + a[18];
+    const R s_out = a[2] + a[8] + a[10] + a[12] + a[14];
+    const R rho_in = (s_known + R(2) * s_out) / (R(1) - ux_in);
+    const R dy = a[3] - a[4], dz = a[5] - a[6];
+    const R e = (R(1) / R(6)) * rho_in * ux_in;
+    Store<S>::st(f + slot_in<PAR>(1, N, c, nb), 1, a[2] + (R(2) / R(3)) * rho_in * ux_in);
+    Store<S>::st(f + slot_in<PAR>(7, N, c, nb), 7, a[10] + e + R(0.5) * rho_in * uy_in - R(0.5) * dy);
+    Store<S>::st(f + slot_in<PAR>(9, N, c, nb), 9, a[8] + e - R(0.5) * rho_in * uy_in + R(0.5) * dy);
+    Store<S>::st(f + slot_in<PAR>(11, N, c, nb), 11, a[14] + e - R(0.5) * dz);
+    Store<S>::st(f + slot_in<PAR>(13, N, c, nb), 13, a[12] + e + R(0.5) * dz);
+  } else {  // clbm_moving.py:296-300
+    const int cs = (z * g.ny + y) * g.nx + (g.nx - 2);
+    const int cd = cs + 1;
+    const Nbr ns = make_nbr(g, g.nx - 2, y, z);
+    const Nbr nd = make_nbr(g, g.nx - 1, y, z);
+    T v[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) v[q] = f[slot_in<PAR>(q, N, cs, ns)];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) f[slot_in<PAR>(q, N, cd, nd)] = v[q];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K7: force-free macroscopic fields (clbm_moving.py:305-321)
+// ---------------------------------------------------------------------------
+template <typename R>
+__device__ __forceinline__ void moments01(const R* a, R& r, R& mx, R& my, R& mz) {
+  r = R(0);
+  mx = R(0);
+  my = R(0);
+  mz = R(0);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    r += a[q];
+    if (ex(q)) mx += R(ex(q)) * a[q];
+    if (ey(q)) my += R(ey(q)) * a[q];
+    if (ez(q)) mz += R(ez(q)) * a[q];
+  }
+}
+
+template <int S, int PAR>
+__global__ __launch_bounds__(BLK) void k_macro(Geo g, const typename Store<S>::T* __restrict__ f,
+                                               float* __restrict__ rho, float* __restrict__ ux,
+                                               float* __restrict__ uy, float* __restrict__ uz) {
+  using R = typename Store<S>::R;
+  const int c = blockIdx.x * BLK + threadIdx.x;
+  if (c >= g.ncell) return;
+  int x, y, z;
+  cell_xyz(g, c, x, y, z);
+  const Nbr nb = make_nbr(g, x, y, z);
+  R a[Q];
+  load_state<S, PAR>(f, (size_t)g.ncell, c, nb, a);
+  R r, mx, my, mz;
+  moments01(a, r, mx, my, mz);
+  if (rho) rho[c] = (float)r;
+  if (ux) ux[c] = (float)(mx / r);
+  if (uy) uy[c] = (float)(my / r);
+  if (uz) uz[c] = (float)(mz / r);
+}
+
+template <int S, int PAR, typename OutT>
+__global__ __launch_bounds__(BLK) void k_export(Geo g, const typename Store<S>::T* __restrict__ f,
+                                                OutT* __restrict__ out) {
+  using R = typename Store<S>::R;
+  const int c = blockIdx.x * BLK + threadIdx.x;
+  if (c >= g.ncell) return;
+  int x, y, z;
+  cell_xyz(g, c, x, y, z);
+  const Nbr nb = make_nbr(g, x, y, z);
+  R a[Q];
+  load_state<S, PAR>(f, (size_t)g.ncell, c, nb, a);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) out[(size_t)q * g.ncell + c] = (OutT)a[q];
+}
+
+template <int S, typename InT>
+__global__ __launch_bounds__(BLK) void k_import(Geo g, typename Store<S>::T* __restrict__ f,
+                                                const InT* __restrict__ in) {
+  const int c = blockIdx.x * BLK + threadIdx.x;
+  if (c >= g.ncell) return;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const size_t i = (size_t)q * g.ncell + c;
+    Store<S>::st(f + i, q, (typename Store<S>::R)in[i]);
+  }
+}
+
+// Order-preserving map of a float onto a signed int (for atomicMin).
+__device__ __forceinline__ int fkey(float v) {
+  int b = __float_as_int(v);
+  return b >= 0 ? b : (b ^ 0x7fffffff);
+}
+__device__ __forceinline__ float fkey_inv(int k) {
+  return __int_as_float(k >= 0 ? k : (k ^ 0x7fffffff));
+}
+
+__global__ void k_health_init(int* acc) {
+  acc[0] = 0x7f800000;  // key of +inf
+  acc[1] = 1;
+}
+
+template <int S, int PAR>
+__global__ __launch_bounds__(BLK) void k_health(Geo g, const typename Store<S>::T* __restrict__ f,
+                                                int* __restrict__ acc) {
+  using R = typename Store<S>::R;
+  __shared__ int smin[BLK / 64], sfin[BLK / 64];
+  const int c = blockIdx.x * BLK + threadIdx.x;
+  int key = 0x7f800000, fin = 1;
+  if (c < g.ncell) {
+    int x, y, z;
+    cell_xyz(g, c, x, y, z);
+    const Nbr nb = make_nbr(g, x, y, z);
+    R a[Q];
+    load_state<S, PAR>(f, (size_t)g.ncell, c, nb, a);
+    R r = R(0);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      r += a[q];
+      fin &= (int)isfinite(a[q]);
+    }
+    if (fin) key = fkey((float)r);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    key = min(key, __shfl_xor(key, o));
+    fin = min(fin, __shfl_xor(fin, o));
+  }
+  const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+  if (ln == 0) {
+    smin[wv] = key;
+    sfin[wv] = fin;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int k = smin[0], fl = sfin[0];
+    for (int i = 1; i < BLK / 64; ++i) {
+      k = min(k, smin[i]);
+      fl = min(fl, sfin[i]);
+    }
+    atomicMin(acc, k);
+    atomicMin(acc + 1, fl);
+  }
+}
+
+__global__ void k_health_fin(const int* acc, float* out2) {
+  out2[0] = fkey_inv(acc[0]);
+  out2[1] = acc[1] ? 1.0f : 0.0f;
+}
+
+// ---------------------------------------------------------------------------
+// host dispatch helpers
+// ---------------------------------------------------------------------------
+static inline int nblocks(long long n) { return (int)((n + BLK - 1) / BLK); }
+
+template <template <int> class Fn, typename... A>
+static int dispatch_storage(int storage, A&&... a) {
+  switch (storage) {
+    case CLBM_STORE_F32: return Fn<CLBM_STORE_F32>::run(a...);
+    case CLBM_STORE_F16: return Fn<CLBM_STORE_F16>::run(a...);
+    case CLBM_STORE_BF16: return Fn<CLBM_STORE_BF16>::run(a...);
+    case CLBM_STORE_F64: return Fn<CLBM_STORE_F64>::run(a...);
+  }
+  return fail(-1, "unknown storage format");
+}
+
+template <int S> struct InitRun {
+  static int run(const Geo& g, void* f, double r, double u, double v, double w, hipStream_t st) {
+    hipLaunchKernelGGL(k_init_feq<S>, dim3(nblocks(g.ncell)), dim3(BLK), 0, st, g,
+                       (typename Store<S>::T*)f, r, u, v, w);
+    return check_launch("clbm_init_feq");
+  }
+};
+
+template <int S> struct StepRun {
+  template <int PAR, bool FORCE, bool BOX>
+  static void launch(const Geo& g, const clbm_params_t* p, void* f, const float* fx,
+                     const float* fy, const float* fz, const int32_t* box, hipStream_t st) {
+    hipLaunchKernelGGL((k_step<S, PAR, FORCE, BOX>), dim3(nblocks(g.ncell)), dim3(BLK), 0, st, g,
+                       p->s_v, p->s_e, (typename Store<S>::T*)f, fx, fy, fz, box);
+  }
+  template <int PAR>
+  static void launch_par(const Geo& g, const clbm_params_t* p, void* f, const float* fx,
+                         const float* fy, const float* fz, const int32_t* box, hipStream_t st) {
+    const bool force = fx || fy || fz;
+    if (!force)
+      launch<PAR, false, false>(g, p, f, fx, fy, fz, box, st);
+    else if (box)
+      launch<PAR, true, true>(g, p, f, fx, fy, fz, box, st);
+    else
+      launch<PAR, true, false>(g, p, f, fx, fy, fz, box, st);
+  }
+  static int run(const Geo& g, const clbm_params_t* p, void* f, int parity, const float* fx,
+                 const float* fy, const float* fz, const int32_t* box, hipStream_t st) {
+    if (parity == 0)
+      launch_par<0>(g, p, f, fx, fy, fz, box, st);
+    else
+      launch_par<1>(g, p, f, fx, fy, fz, box, st);
+    int e = check_launch("clbm_step");
+    if (e) return e;
+    if (p->inlet) {
+      const int nbc = (int)((2ll * g.ny * g.nz + BCBLK - 1) / BCBLK);
+      // the state after the step has the opposite parity
+      if (parity == 0)
+        hipLaunchKernelGGL((k_bc<S, 1>), dim3(nbc), dim3(BCBLK), 0, st, g, p->ux_in, p->uy_in,
+                           (typename Store<S>::T*)f);
+      else
+        hipLaunchKernelGGL((k_bc<S, 0>), dim3(nbc), dim3(BCBLK), 0, st, g, p->ux_in, p->uy_in,
+                           (typename Store<S>::T*)f);
+      return check_launch("clbm_step (boundary planes)");
+    }
+    return 0;
+  }
+};
+
+template <int S> struct MacroRun {
+  static int run(const Geo& g, const void* f, int parity, float* r, float* u, float* v, float* w,
+                 hipStream_t st) {
+    auto fp = (const typename Store<S>::T*)f;
+    if (parity == 0)
+      hipLaunchKernelGGL((k_macro<S, 0>), dim3(nblocks(g.ncell)), dim3(BLK), 0, st, g, fp, r, u, v, w);
+    else
+      hipLaunchKernelGGL((k_macro<S, 1>), dim3(nblocks(g.ncell)), dim3(BLK), 0, st, g, fp, r, u, v, w);
+    return check_launch("clbm_macro");
+  }
+};
+
+template <int S> struct ExportRun {
+  template <typename OutT>
+  static int run(const Geo& g, const void* f, int parity, OutT* out, hipStream_t st) {
+    auto fp = (const typename Store<S>::T*)f;
+    if (parity == 0)
+      hipLaunchKernelGGL((k_export<S, 0, OutT>), dim3(nblocks(g.ncell)), dim3(BLK), 0, st, g, fp, out);
+    else
+      hipLaunchKernelGGL((k_export<S, 1, OutT>), dim3(nblocks(g.ncell)), dim3(BLK), 0, st, g, fp, out);
+    return check_launch("clbm_export_state");
+  }
+};
+
+template <int S> struct ImportRun {
+  template <typename InT>
+  static int run(const Geo& g, void* f, const InT* in, hipStream_t st) {
+    hipLaunchKernelGGL((k_import<S, InT>), dim3(nblocks(g.ncell)), dim3(BLK), 0, st, g,
+                       (typename Store<S>::T*)f, in);
+    return check_launch("clbm_import_state");
+  }
+};
+
+template <int S> struct HealthRun {
+  static int run(const Geo& g, const void* f, int parity, float* out2, hipStream_t st) {
+    // out2 is used as int scratch before the final conversion
+    int* acc = (int*)out2;
+    hipLaunchKernelGGL(k_health_init, dim3(1), dim3(1), 0, st, acc);
+    auto fp = (const typename Store<S>::T*)f;
+    if (parity == 0)
+      hipLaunchKernelGGL((k_health<S, 0>), dim3(nblocks(g.ncell)), dim3(BLK), 0, st, g, fp, acc);
+    else
+      hipLaunchKernelGGL((k_health<S, 1>), dim3(nblocks(g.ncell)), dim3(BLK), 0, st, g, fp, acc);
+    hipLaunchKernelGGL(k_health_fin, dim3(1), dim3(1), 0, st, acc, out2);
+    return check_launch("clbm_health");
+  }
+};
+
+}  // namespace clbm
+
+using namespace clbm;
+
+extern "C" {
+
+int clbm_abi_version(void) { return CLBM_ABI_VERSION; }
+
+const char* clbm_last_error(void) { return g_last_error.c_str(); }
+
+size_t clbm_state_bytes(const clbm_grid_t* g) {
+  if (check_grid(g, false)) return 0;
+  return (size_t)Q * (size_t)g->nx * g->ny * g->nz * elem_bytes(g->storage);
+}
+
+int clbm_init_feq(const clbm_grid_t* g, void* f, double rho0, double ux0, double uy0, double uz0,
+                  void* stream) {
+  if (int e = check_grid(g, false)) return e;
+  if (!f) return fail(-1, "clbm_init_feq: f is NULL");
+  return dispatch_storage<InitRun>(g->storage, make_geo(*g), f, rho0, ux0, uy0, uz0,
+                                   (hipStream_t)stream);
+}
+
+int clbm_step(const clbm_grid_t* g, const clbm_params_t* p, void* f, int parity, const float* fx,
+              const float* fy, const float* fz, const int32_t* force_box, void* stream) {
+  if (!p) return fail(-1, "clbm_step: params is NULL");
+  if (int e = check_grid(g, p->inlet != 0)) return e;
+  if (!f) return fail(-1, "clbm_step: f is NULL");
+  if (parity != 0 && parity != 1) return fail(-1, "clbm_step: parity must be 0 or 1");
+  if (p->inlet && !(p->ux_in < 1.0)) return fail(-1, "clbm_step: inlet needs ux_in < 1");
+  return dispatch_storage<StepRun>(g->storage, make_geo(*g), p, f, parity, fx, fy, fz, force_box,
+                                   (hipStream_t)stream);
+}
+
+int clbm_macro(const clbm_grid_t* g, const void* f, int parity, float* rho, float* ux, float* uy,
+               float* uz, void* stream) {
+  if (int e = check_grid(g, false)) return e;
+  if (!f) return fail(-1, "clbm_macro: f is NULL");
+  if (parity != 0 && parity != 1) return fail(-1, "clbm_macro: parity must be 0 or 1");
+  return dispatch_storage<MacroRun>(g->storage, make_geo(*g), f, parity, rho, ux, uy, uz,
+                                    (hipStream_t)stream);
+}
+
+int clbm_export_state(const clbm_grid_t* g, const void* f, int parity, float* out, void* stream) {
+  if (int e = check_grid(g, false)) return e;
+  if (!f || !out) return fail(-1, "clbm_export_state: NULL buffer");
+  if (parity != 0 && parity != 1) return fail(-1, "clbm_export_state: parity must be 0 or 1");
+  return dispatch_storage<ExportRun>(g->storage, make_geo(*g), f, parity, out, (hipStream_t)stream);
+}
+
+int clbm_export_state_f64(const clbm_grid_t* g, const void* f, int parity, double* out,
+                          void* stream) {
+  if (int e = check_grid(g, false)) return e;
+  if (!f || !out) return fail(-1, "clbm_export_state_f64: NULL buffer");
+  if (parity != 0 && parity != 1) return fail(-1, "clbm_export_state_f64: parity must be 0 or 1");
+  return dispatch_storage<ExportRun>(g->storage, make_geo(*g), f, parity, out, (hipStream_t)stream);
+}
+
+int clbm_import_state(const clbm_grid_t* g, void* f, const float* in, void* stream) {
+  if (int e = check_grid(g, false)) return e;
+  if (!f || !in) return fail(-1, "clbm_import_state: NULL buffer");
+  return dispatch_storage<ImportRun>(g->storage, make_geo(*g), f, in, (hipStream_t)stream);
+}
+
+int clbm_import_state_f64(const clbm_grid_t* g, void* f, const double* in, void* stream) {
+  if (int e = check_grid(g, false)) return e;
+  if (!f || !in) return fail(-1, "clbm_import_state_f64: NULL buffer");
+  return dispatch_storage<ImportRun>(g->storage, make_geo(*g), f, in, (hipStream_t)stream);
+}
+
+int clbm_health(const clbm_grid_t* g, const void* f, int parity, float* out2, void* stream) {
+  if (int e = check_grid(g, false)) return e;
+  if (!f || !out2) return fail(-1, "clbm_health: NULL buffer");
+  if (parity != 0 && parity != 1) return fail(-1, "clbm_health: parity must be 0 or 1");
+  return dispatch_storage<HealthRun>(g->storage, make_geo(*g), f, parity, out2, (hipStream_t)stream);
+}
+
+}  // extern "C"

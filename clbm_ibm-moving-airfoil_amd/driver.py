@@ -1,0 +1,128 @@
+"""Driver loop: the counterpart of main_moving.py / main_highRe.py / fish/main_fish.py.
+
+Per step (main_moving.py:116-122): motion state on the host in float64,
+``ibm.compute_force(lbm.ux, lbm.uy, ...)``, ``lbm.step(fx, fy, fz)``.  Every
+``vis_interval`` steps (main_moving.py:124-161): divergence guard, a frame
+``frame_XXXXXX.npz`` with float32 ``ux, uy, uz, rho`` of shape (Nx, Ny, Nz),
+and at the end ``meta.npz`` with the reference's keys (main_moving.py:173-200),
+so the reference's renderers can read the output.  Added: the FP64 force
+history ``forces.npz`` (sums, C_D, C_L per step).
+
+Usage: python -m <package>.driver --case c0|c1|airplane|highre|fish [--steps N]
+"""
+
+from __future__ import annotations
+
+import argparse
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import config as cfgmod
+from .airfoil_moving import DroneMovingIBM
+from .clbm_moving import CLBMSolver
+
+CASES = {"c0": cfgmod.case_c0, "c1": cfgmod.case_c1, "c2": cfgmod.case_c2_large,
+         "airplane": cfgmod.reference_airplane, "highre": cfgmod.reference_highre,
+         "fish": cfgmod.reference_fish}
+
+
+def grid_meta(cfg) -> np.ndarray:
+    """The 24-entry float64 record of main_moving.py:185-199."""
+    e = cfg.extra
+    fuse = e.get("fuse", (0.0, 0.0, 0.0))
+    z0 = e.get("z0", cfg.Nz / 2.0)
+    return np.array([
+        cfg.Nx, cfg.Ny, cfg.Nz, cfg.U_inf, cfg.Re, cfg.x0, cfg.y0, z0,
+        fuse[0], fuse[1], e.get("b_half", 0.0), e.get("z_wing_start", 0.0), e.get("z_wing_end", 0.0),
+        float(int(round(z0))), float(int(round(cfg.x0))), cfg.vis_interval, fuse[2],
+        e.get("x_tail_arm", 0.0), e.get("c_htail", 0.0), e.get("b_half_htail", 0.0),
+        e.get("z_htail_start", 0.0), e.get("z_htail_end", 0.0), e.get("c_vtail", 0.0),
+        e.get("h_vtail", 0.0)], dtype=np.float64)
+
+
+def run(cfg, steps=None, out_dir=None, storage="f32", write_frames=True, record_forces=True,
+        verbose=True, spread_mode="fixed"):
+    """Run the coupled loop; returns a dict with the force history and timing."""
+    import torch
+    steps = cfg.N_steps if steps is None else steps
+    out_dir = cfg.DATA_DIR if out_dir is None else out_dir
+    if write_frames:
+        os.makedirs(out_dir, exist_ok=True)
+    lbm = CLBMSolver(cfg, storage=storage)
+    ibm = DroneMovingIBM(cfg, spread_mode=spread_mode)
+    snaps = dict(xc=[], yc=[], theta=[], step=[])
+    frame = 0
+    diverged = False
+    torch.cuda.synchronize()
+    t0 = last = time.perf_counter()
+    step = 0
+    for step in range(1, steps + 1):
+        xc, yc, th, vx, vy, om = cfg.motion_state(step)
+        fx, fy, fz = ibm.compute_force(lbm.ux, lbm.uy, xc, yc, th, vx, vy, om)
+        lbm.step(fx, fy, fz)
+        if record_forces:
+            ibm.record_force()
+        if step % cfg.vis_interval == 0 or step == steps:
+            rho_min, finite = lbm.health()
+            now = time.perf_counter()
+            if verbose:
+                rate = cfg.vis_interval / max(now - last, 1e-9)
+                print(f"  step {step:6d} | rho_min={rho_min:.4f} yc={yc:.2f} "
+                      f"theta={math.degrees(th):.2f} deg | {rate:.0f} steps/s", flush=True)
+            last = now
+            if not finite or rho_min < 0.3:
+                print(f"*** Diverged at step {step}! rho_min={rho_min:.4f} ***", flush=True)
+                diverged = True
+                break
+            if write_frames and step % cfg.vis_interval == 0:
+                frame += 1
+                np.savez_compressed(os.path.join(out_dir, f"frame_{frame:06d}.npz"),
+                                    ux=lbm.ux.to_numpy(), uy=lbm.uy.to_numpy(),
+                                    uz=lbm.uz.to_numpy(), rho=lbm.rho.to_numpy())
+                for k, v in zip(("xc", "yc", "theta", "step"), (xc, yc, th, step)):
+                    snaps[k].append(v)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    hist = ibm.force_history()
+    cd, cl = ibm.coefficients(hist) if len(hist) else (np.zeros(0), np.zeros(0))
+    if write_frames:
+        e = cfg.extra
+        if snaps["step"]:
+            np.savez_compressed(
+                os.path.join(out_dir, "meta.npz"),
+                xc=np.array(snaps["xc"], np.float32), yc=np.array(snaps["yc"], np.float32),
+                theta=np.array(snaps["theta"], np.float32), step=np.array(snaps["step"], np.int32),
+                X_template=e.get("X_template", cfg.X0), Y_template=e.get("Y_template", cfg.Y0),
+                X_htail_tmpl=e.get("X_htail_tmpl", np.zeros(0, np.float32)),
+                Y_htail_tmpl=e.get("Y_htail_tmpl", np.zeros(0, np.float32)),
+                grid_meta=grid_meta(cfg))
+        if record_forces:
+            np.savez_compressed(os.path.join(out_dir, "forces.npz"), sums=hist, cd=cd, cl=cl)
+    ncell = cfg.Nx * cfg.Ny * cfg.Nz
+    res = dict(steps=step, elapsed=elapsed, mlups=step * ncell / elapsed / 1e6, history=hist,
+               cd=cd, cl=cl, diverged=diverged)
+    if verbose:
+        print(f"Done. {step} steps in {elapsed:.1f} s ({step / elapsed:.0f} steps/s, "
+              f"{res['mlups']:.0f} MLUPS)", flush=True)
+    return res
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--case", default="c0", choices=sorted(CASES))
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--out", default=None)
+    ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16", "f64"])
+    ap.add_argument("--no-frames", action="store_true")
+    a = ap.parse_args(argv)
+    cfg = CASES[a.case]()
+    res = run(cfg, a.steps, a.out, a.storage, write_frames=not a.no_frames)
+    return 1 if res["diverged"] else 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

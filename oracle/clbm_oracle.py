@@ -1,0 +1,487 @@
+"""CPU oracle: a NumPy restatement of the reference CLBM-IBM time step.
+
+TEST INFRASTRUCTURE ONLY.  Nothing in the product package imports this
+module; only ``tests/``, ``__graft_entry__.smoke()`` and the CPU-baseline leg
+of ``bench.py`` use it, and only as the checker.
+
+Every function follows one reference kernel, formula for formula, in the
+reference's array layout: populations are ``(Nx, Ny, Nz, 19)`` (q fastest)
+and scalar fields ``(Nx, Ny, Nz)``.  The arithmetic runs in the dtype of the
+inputs (``float64`` for the exact oracle, ``float32`` to measure the FP32
+rounding spread).  Citations are to the reference repository
+(kioxli/CLBM_IBM-moving-airfoil).
+
+Pinning (what checks each function, in ``tests/test_oracle_*.py``):
+
+* lattice constants, ``M_raw``/``M_inv``        -- exact identities (M M^-1 = I, dyadic entries)
+* ``init_feq``                                  -- closed-form moments
+* ``macro_with_force`` / ``macro_post``         -- closed form
+* ``collide``                                   -- invariants computed by brute-force moment sums:
+                                                   mass, momentum gain (1.5 - s_v/2) F, relaxed
+                                                   central moments, rest fixed point, Guo moments
+* ``stream``                                    -- permutation / brute force on tiny grids
+* ``bc_inlet`` / ``bc_outlet``                  -- closed-form moment identities of the inlet rule
+* ``phi1d``                                     -- Peskin identities (partition of unity, sum phi^2 = 3/8)
+* ``update_markers``                            -- closed form (rotation, rigid velocity)
+* ``interpolate`` / ``spread``                  -- adjointness, partition of unity, seam weight 0.75,
+                                                   brute-force stencil enumeration
+* ``marker_forces`` / ``clip_forces``           -- closed form
+* ``force_sum_f64``                             -- exact FP64 sum (math.fsum)
+* ``motion_state``                              -- closed form derivatives
+
+Taichi itself is not installed here, so bit-level agreement with a Taichi
+run is unpinned; the only deliberate choice below is the FP32 marker
+rotation (see ``update_markers``), whose cos/sin are rounded from FP64.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+# --------------------------------------------------------------------------
+# Lattice constants -- clbm_moving.py:21-30, config_moving.py:37-43,48-57
+# --------------------------------------------------------------------------
+EX = np.array([0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0], dtype=np.int64)
+EY = np.array([0, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, 1, -1], dtype=np.int64)
+EZ = np.array([0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1], dtype=np.int64)
+W = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12, dtype=np.float64)
+CS2 = 1.0 / 3.0
+Q = 19
+
+# raw-moment ordering, config_moving.py:62-68
+PQR = [
+    (0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1),
+    (1, 1, 0), (1, 0, 1), (0, 1, 1),
+    (2, 0, 0), (0, 2, 0), (0, 0, 2),
+    (2, 1, 0), (1, 2, 0), (2, 0, 1), (0, 2, 1), (1, 0, 2), (0, 1, 2),
+    (2, 2, 0), (2, 0, 2), (0, 2, 2),
+]
+
+
+def m_raw() -> np.ndarray:
+    """19x19 raw-moment matrix, row = (p,q,r), column = direction (config_moving.py:69-72)."""
+    return np.array([[int(EX[c]) ** p * int(EY[c]) ** q * int(EZ[c]) ** r for c in range(Q)]
+                     for (p, q, r) in PQR], dtype=np.float64)
+
+
+def m_raw_inv_f32() -> np.ndarray:
+    """Its inverse cast to float32, as stored in CLBMSolver.M_inv (config_moving.py:73,
+    clbm_moving.py:66-67)."""
+    return np.linalg.inv(m_raw()).astype(np.float32)
+
+
+def relaxation_rates(U_inf: float, chord: float, Re: float) -> dict:
+    """nu, tau, s_v, s_e, s_q, s_pi as config_moving.py:37-43 define them."""
+    nu = U_inf * chord / Re
+    tau = 0.5 + nu / CS2
+    return dict(nu=nu, tau=tau, s_v=1.0 / tau, s_e=1.0, s_q=1.0, s_pi=1.0)
+
+
+def inlet_velocity(U_inf: float, aoa_deg: float) -> tuple:
+    """(_UX_IN, _UY_IN) -- clbm_moving.py:33-35 (inflow tilted by the mean AoA)."""
+    a = math.radians(aoa_deg)
+    return float(U_inf * math.cos(a)), float(-U_inf * math.sin(a))
+
+
+# --------------------------------------------------------------------------
+# CLBMSolver kernels -- clbm_moving.py
+# --------------------------------------------------------------------------
+def init_feq(nx: int, ny: int, nz: int, ux0: float, uy0: float, dtype=np.float64):
+    """K1 ``_init_feq`` (clbm_moving.py:75-91): second-order equilibrium at (ux0, uy0, 0),
+    rho = 1.  Returns (f, rho, ux, uy, uz)."""
+    ux0 = dtype(ux0)
+    uy0 = dtype(uy0)
+    cs2 = dtype(CS2)
+    u2 = ux0 * ux0 + uy0 * uy0
+    fq = np.empty(Q, dtype=dtype)
+    for q in range(Q):
+        eu = dtype(EX[q]) * ux0 + dtype(EY[q]) * uy0
+        fq[q] = dtype(W[q]) * (dtype(1.0) + eu / cs2 + dtype(0.5) * eu * eu / (cs2 * cs2)
+                               - dtype(0.5) * u2 / cs2)
+    f = np.broadcast_to(fq, (nx, ny, nz, Q)).copy()
+    rho = np.ones((nx, ny, nz), dtype=dtype)
+    ux = np.full((nx, ny, nz), ux0, dtype=dtype)
+    uy = np.full((nx, ny, nz), uy0, dtype=dtype)
+    uz = np.zeros((nx, ny, nz), dtype=dtype)
+    return f, rho, ux, uy, uz
+
+
+def _moments_0_1(f):
+    dt = f.dtype.type
+    r = np.zeros(f.shape[:3], dtype=f.dtype)
+    mx = np.zeros_like(r)
+    my = np.zeros_like(r)
+    mz = np.zeros_like(r)
+    for q in range(Q):          # sequential q order, as the unrolled loop does
+        fq = f[..., q]
+        r += fq
+        mx += dt(EX[q]) * fq
+        my += dt(EY[q]) * fq
+        mz += dt(EZ[q]) * fq
+    return r, mx, my, mz
+
+
+def macro_with_force(f, fx, fy, fz):
+    """K2 ``_macro_with_force`` (clbm_moving.py:96-115): rho, and u = m/rho + F/(2 rho)."""
+    dt = f.dtype.type
+    r, mx, my, mz = _moments_0_1(f)
+    half = dt(0.5)
+    return r, mx / r + half * fx / r, my / r + half * fy / r, mz / r + half * fz / r
+
+
+def macro_post(f):
+    """K7 ``_macro_post`` (clbm_moving.py:305-321): force-free rho and u = m/rho."""
+    r, mx, my, mz = _moments_0_1(f)
+    return r, mx / r, my / r, mz / r
+
+
+def collide(f, rho, ux, uy, uz, fx, fy, fz, s_v: float, s_e: float = 1.0):
+    """K3 ``_collide`` (clbm_moving.py:120-249).
+
+    Step 1 (:138-168): 15 central moments about u.  Step 2 (:170-196): shear
+    moments x(1-s_v); trace relaxed by s_e toward rho, deviators x(1-s_v),
+    recombined with the fixed-sign formula; third order -> 0 and fourth order
+    -> rho/9 (s_q = s_pi = 1 are hard-coded there).  Step 3 (:198-231): central
+    -> raw moments T.  Step 4 (:233-238): f* = M_inv T with the float32 M_inv.
+    Step 5 (:240-249): Guo forcing with coefficient (1 - s_v/2).
+    """
+    dt = f.dtype.type
+    u, v, w, r = ux, uy, uz, rho
+    one, two, three, four, nine = dt(1), dt(2), dt(3), dt(4), dt(9)
+    sv = dt(s_v)
+    se = dt(s_e)
+    z = np.zeros_like(r)
+    k110 = z.copy(); k101 = z.copy(); k011 = z.copy()
+    k200 = z.copy(); k020 = z.copy(); k002 = z.copy()
+    k210 = z.copy(); k120 = z.copy(); k201 = z.copy(); k021 = z.copy(); k102 = z.copy(); k012 = z.copy()
+    k220 = z.copy(); k202 = z.copy(); k022 = z.copy()
+    for q in range(Q):
+        fq = f[..., q]
+        xix = dt(EX[q]) - u
+        xiy = dt(EY[q]) - v
+        xiz = dt(EZ[q]) - w
+        xix2 = xix * xix
+        xiy2 = xiy * xiy
+        xiz2 = xiz * xiz
+        k110 += fq * xix * xiy
+        k101 += fq * xix * xiz
+        k011 += fq * xiy * xiz
+        k200 += fq * xix2
+        k020 += fq * xiy2
+        k002 += fq * xiz2
+        k210 += fq * xix2 * xiy
+        k120 += fq * xix * xiy2
+        k201 += fq * xix2 * xiz
+        k021 += fq * xiy2 * xiz
+        k102 += fq * xix * xiz2
+        k012 += fq * xiy * xiz2
+        k220 += fq * xix2 * xiy2
+        k202 += fq * xix2 * xiz2
+        k022 += fq * xiy2 * xiz2
+    # cascade relaxation (clbm_moving.py:170-196)
+    k110s = k110 * (one - sv)
+    k101s = k101 * (one - sv)
+    k011s = k011 * (one - sv)
+    kA = k200 + k020 + k002
+    kA_eq = r
+    kD = k200 - k020
+    kE = k200 - k002
+    kAs = kA - se * (kA - kA_eq)
+    kDs = kD * (one - sv)
+    kEs = kE * (one - sv)
+    k200s = (kAs + kDs + kEs) / three
+    k020s = (kAs - two * kDs + kEs) / three
+    k002s = (kAs + kDs - two * kEs) / three
+    k210s = k120s = k201s = k021s = k102s = k012s = z
+    k220s = r / nine
+    k202s = r / nine
+    k022s = r / nine
+    # central -> raw (clbm_moving.py:198-223)
+    T = [
+        r, u * r, v * r, w * r,
+        k110s + u * v * r, k101s + u * w * r, k011s + v * w * r,
+        k200s + u * u * r, k020s + v * v * r, k002s + w * w * r,
+        k210s + two * u * k110s + v * k200s + u * u * v * r,
+        k120s + two * v * k110s + u * k020s + u * v * v * r,
+        k201s + two * u * k101s + w * k200s + u * u * w * r,
+        k021s + two * v * k011s + w * k020s + v * v * w * r,
+        k102s + two * w * k101s + u * k002s + u * w * w * r,
+        k012s + two * w * k011s + v * k002s + v * w * w * r,
+        (k220s + two * u * k120s + two * v * k210s + four * u * v * k110s
+         + v * v * k200s + u * u * k020s + u * u * v * v * r),
+        (k202s + two * u * k102s + two * w * k201s + four * u * w * k101s
+         + w * w * k200s + u * u * k002s + u * u * w * w * r),
+        (k022s + two * v * k012s + two * w * k021s + four * v * w * k011s
+         + w * w * k020s + v * v * k002s + v * v * w * w * r),
+    ]
+    minv = m_raw_inv_f32().astype(f.dtype)
+    f_post = np.empty_like(f)
+    for q in range(Q):          # dense 19x19 matvec, r_idx order (clbm_moving.py:234-238)
+        val = np.zeros_like(r)
+        for ridx in range(Q):
+            val += minv[q, ridx] * T[ridx]
+        f_post[..., q] = val
+    # Guo forcing (clbm_moving.py:240-249)
+    cs2 = dt(CS2)
+    coeff = one - sv * dt(0.5)
+    for q in range(Q):
+        ex, ey, ez = dt(EX[q]), dt(EY[q]), dt(EZ[q])
+        eu = ex * u + ey * v + ez * w
+        cx = (ex - u) / cs2 + eu * ex / (cs2 * cs2)
+        cy = (ey - v) / cs2 + eu * ey / (cs2 * cs2)
+        cz = (ez - w) / cs2 + eu * ez / (cs2 * cs2)
+        f_post[..., q] += dt(W[q]) * coeff * (cx * fx + cy * fy + cz * fz)
+    return f_post
+
+
+def stream(f_post):
+    """K4 ``_stream`` (clbm_moving.py:254-261): pull with modular wrap on all three axes,
+    f[i,j,k,q] = f_post[(i-ex)%Nx, (j-ey)%Ny, (k-ez)%Nz, q]."""
+    f = np.empty_like(f_post)
+    for q in range(Q):
+        f[..., q] = np.roll(f_post[..., q], shift=(int(EX[q]), int(EY[q]), int(EZ[q])), axis=(0, 1, 2))
+    return f
+
+
+def bc_inlet(f, ux_in: float, uy_in: float):
+    """K5 ``_bc_inlet`` (clbm_moving.py:266-294): the reference's inlet rule at x = 0,
+    with its coefficients (2/3 on f1, 1/6 on the four edges), in place."""
+    dt = f.dtype.type
+    ux_in = dt(ux_in)
+    uy_in = dt(uy_in)
+    g = f[0]
+    s_known = g[..., 0] + g[..., 3] + g[..., 4] + g[..., 5] + g[..., 6] + g[..., 15] + g[..., 16] + g[..., 17] + g[..., 18]
+    s_out = g[..., 2] + g[..., 8] + g[..., 10] + g[..., 12] + g[..., 14]
+    rho_in = (s_known + dt(2.0) * s_out) / (dt(1.0) - ux_in)
+    diff_y = g[..., 3] - g[..., 4]
+    diff_z = g[..., 5] - g[..., 6]
+    sixth = dt(1.0) / dt(6.0)
+    half = dt(0.5)
+    g[..., 1] = g[..., 2] + (dt(2.0) / dt(3.0)) * rho_in * ux_in
+    g[..., 7] = g[..., 10] + sixth * rho_in * ux_in + half * rho_in * uy_in - half * diff_y
+    g[..., 9] = g[..., 8] + sixth * rho_in * ux_in - half * rho_in * uy_in + half * diff_y
+    g[..., 11] = g[..., 14] + sixth * rho_in * ux_in - half * diff_z
+    g[..., 13] = g[..., 12] + sixth * rho_in * ux_in + half * diff_z
+    return f
+
+
+def bc_outlet(f):
+    """K6 ``_bc_outlet`` (clbm_moving.py:296-300): f[Nx-1] = f[Nx-2], all q, in place."""
+    f[-1] = f[-2]
+    return f
+
+
+def lbm_step(f, fx, fy, fz, s_v: float, s_e: float, ux_in: float, uy_in: float):
+    """``CLBMSolver.step`` (clbm_moving.py:326-343): K2 -> K3 -> K4 -> K5 -> K6 -> K7.
+    Returns (f_new, rho, ux, uy, uz) with the K7 (force-free) macroscopic fields."""
+    rho, ux, uy, uz = macro_with_force(f, fx, fy, fz)
+    f_post = collide(f, rho, ux, uy, uz, fx, fy, fz, s_v, s_e)
+    f_new = stream(f_post)
+    bc_inlet(f_new, ux_in, uy_in)
+    bc_outlet(f_new)
+    rho, ux, uy, uz = macro_post(f_new)
+    return f_new, rho, ux, uy, uz
+
+
+# --------------------------------------------------------------------------
+# IBM kernels -- airfoil_moving.py
+# --------------------------------------------------------------------------
+def phi1d(r):
+    """Peskin 4-point kernel ``_phi1d`` (airfoil_moving.py:27-38), elementwise in r's dtype."""
+    r = np.asarray(r)
+    dt = r.dtype.type if r.dtype.kind == "f" else np.float64
+    r = r.astype(dt)
+    s = np.abs(r)
+    a = (dt(3.0) - dt(2.0) * s + np.sqrt(np.maximum(dt(1.0) + dt(4.0) * s - dt(4.0) * s * s, dt(0.0)))) / dt(8.0)
+    b = (dt(5.0) - dt(2.0) * s - np.sqrt(np.maximum(dt(-7.0) + dt(12.0) * s - dt(4.0) * s * s, dt(0.0)))) / dt(8.0)
+    return np.where(s < dt(1.0), a, np.where(s < dt(2.0), b, dt(0.0))).astype(dt)
+
+
+def trig_f32(theta):
+    """cos/sin of a float32 angle, evaluated in FP64 and rounded once to float32.
+
+    The reference evaluates ti.cos/ti.sin in float32 (airfoil_moving.py:114-115),
+    whose last-bit behaviour is Taichi's and unpinned; this build fixes the
+    rounding as 'FP64 then one rounding', which the GPU path reproduces."""
+    t = np.float64(np.float32(theta))
+    return np.float32(math.cos(t)), np.float32(math.sin(t))
+
+
+def update_markers(X0, Y0, Z0, xc, yc, theta, vx, vy, omega):
+    """K8 ``_update_markers`` (airfoil_moving.py:100-127) in float32.
+
+    x = xc + X0 cos - Y0 sin ; y = yc + X0 sin + Y0 cos ; z = Z0 ;
+    vx_b = vx - omega (y - yc) ; vy_b = vy + omega (x - xc).
+    Host scalars are truncated to float32 as the ti.f32 kernel arguments are
+    (airfoil_moving.py:102-103).  Each product and sum is rounded separately
+    (no fused multiply-add), left to right as written at :119-127."""
+    f32 = np.float32
+    X0 = np.asarray(X0, dtype=f32)
+    Y0 = np.asarray(Y0, dtype=f32)
+    Z0 = np.asarray(Z0, dtype=f32)
+    xc, yc, vx, vy, omega = (f32(a) for a in (xc, yc, vx, vy, omega))
+    c, s = trig_f32(theta)
+    X = (xc + X0 * c) - Y0 * s
+    Y = (yc + X0 * s) + Y0 * c
+    Z = Z0.copy()
+    VX = vx - omega * (Y - yc)
+    VY = vy + omega * (X - xc)
+    return X.astype(f32), Y.astype(f32), Z, VX.astype(f32), VY.astype(f32)
+
+
+def stencil(X, Y, Z, nx: int, ny: int, nz: int, wrap_weight_quirk: bool = True):
+    """Shared stencil of ``_interpolate``/``_spread`` (airfoil_moving.py:148-166,194-212).
+
+    base = floor(pos) - 1 per axis; x index clamped to [0, Nx-1]; y and z
+    indices wrapped (Python-style non-negative modulo).  The weight of each
+    axis is phi(float(index used) - pos): with the quirk on (the reference),
+    the *wrapped/clamped* index enters phi, so a stencil that crosses the
+    periodic seam loses weight.  Returns (base (N,3) int32, ii/jj/kk (N,4)
+    int64, phx/phy/phz (N,4) in the dtype of X)."""
+    X = np.asarray(X)
+    Y = np.asarray(Y)
+    Z = np.asarray(Z)
+    dt = X.dtype.type
+    ib = np.floor(X).astype(np.int64) - 1
+    jb = np.floor(Y).astype(np.int64) - 1
+    kb = np.floor(Z).astype(np.int64) - 1
+    d = np.arange(4)
+    iu = ib[:, None] + d
+    ju = jb[:, None] + d
+    ku = kb[:, None] + d
+    ii = np.clip(iu, 0, nx - 1)
+    jj = np.mod(ju + ny, ny)
+    kk = np.mod(ku + nz, nz)
+    if wrap_weight_quirk:
+        ax, ay, az = ii, jj, kk
+    else:
+        ax, ay, az = iu, ju, ku
+    phx = phi1d(ax.astype(dt) - X[:, None])
+    phy = phi1d(ay.astype(dt) - Y[:, None])
+    phz = phi1d(az.astype(dt) - Z[:, None])
+    base = np.stack([ib, jb, kb], axis=1).astype(np.int32)
+    return base, ii, jj, kk, phx, phy, phz
+
+
+def interpolate(ux, uy, X, Y, Z, wrap_weight_quirk: bool = True):
+    """K10 ``_interpolate`` (airfoil_moving.py:142-170): U_k, V_k by the 4x4x4
+    Peskin stencil, summed in (di, dj, dk) loop order with w = phx*phy*phz."""
+    nx, ny, nz = ux.shape
+    dt = np.asarray(X).dtype.type
+    _, ii, jj, kk, phx, phy, phz = stencil(X, Y, Z, nx, ny, nz, wrap_weight_quirk)
+    n = len(X)
+    U = np.zeros(n, dtype=dt)
+    V = np.zeros(n, dtype=dt)
+    for di in range(4):
+        for dj in range(4):
+            for dk in range(4):
+                w = phx[:, di] * phy[:, dj] * phz[:, dk]
+                a, b, c = ii[:, di], jj[:, dj], kk[:, dk]
+                U = U + ux[a, b, c].astype(dt) * w
+                V = V + uy[a, b, c].astype(dt) * w
+    return U, V
+
+
+def marker_forces(U, V, VX, VY, f_max: float = 0.05):
+    """K11 ``_marker_forces`` (airfoil_moving.py:175-183): F = clamp(-(U - V_body), +-f_max)."""
+    dt = np.asarray(U).dtype.type
+    fm = dt(f_max)
+    FX = np.maximum(-fm, np.minimum(fm, -(U - VX)))
+    FY = np.maximum(-fm, np.minimum(fm, -(V - VY)))
+    return FX.astype(dt), FY.astype(dt)
+
+
+def spread(X, Y, Z, FX, FY, shape, wrap_weight_quirk: bool = True, acc_dtype=np.float64):
+    """K9 + K12 ``_zero_forces``/``_spread`` (airfoil_moving.py:132-137,188-214):
+    fx[ii,jj,kk] += F_k * w over the same stencil as ``interpolate``.  The
+    product F_k*w is formed in the dtype of the markers; the sum is accumulated
+    in ``acc_dtype`` (the reference's atomic order is undefined, so only a
+    tolerance can compare sums)."""
+    nx, ny, nz = shape
+    dt = np.asarray(X).dtype.type
+    _, ii, jj, kk, phx, phy, phz = stencil(X, Y, Z, nx, ny, nz, wrap_weight_quirk)
+    fx = np.zeros(shape, dtype=acc_dtype)
+    fy = np.zeros(shape, dtype=acc_dtype)
+    for di in range(4):
+        for dj in range(4):
+            for dk in range(4):
+                w = phx[:, di] * phy[:, dj] * phz[:, dk]
+                a, b, c = ii[:, di], jj[:, dj], kk[:, dk]
+                np.add.at(fx, (a, b, c), (FX * w).astype(dt).astype(acc_dtype))
+                np.add.at(fy, (a, b, c), (FY * w).astype(dt).astype(acc_dtype))
+    return fx, fy
+
+
+def clip_forces(fx, fy, f_max: float = 0.05, dtype=np.float32):
+    """K13 ``_clip_forces`` (airfoil_moving.py:219-225): clamp fx, fy to +-f_max; fz = 0."""
+    fm = f_max
+    fxc = np.clip(fx, -fm, fm).astype(dtype)
+    fyc = np.clip(fy, -fm, fm).astype(dtype)
+    return fxc, fyc, np.zeros_like(fxc)
+
+
+def compute_force(ux, uy, X0, Y0, Z0, xc, yc, theta, vx, vy, omega,
+                  f_max: float = 0.05, wrap_weight_quirk: bool = True, acc_dtype=np.float64,
+                  weight_dtype=np.float32):
+    """``DroneMovingIBM.compute_force`` (airfoil_moving.py:230-255): K8 -> K9 -> K10 ->
+    K11 -> K12 -> K13.  Marker positions are always the float32 values of K8;
+    ``weight_dtype`` sets the arithmetic of the stencil weights, interpolation
+    and marker forces (float32 as in the reference, float64 for the exact
+    oracle).  Returns a dict with every intermediate."""
+    X, Y, Z, VX, VY = update_markers(X0, Y0, Z0, xc, yc, theta, vx, vy, omega)
+    wd = weight_dtype
+    X, Y, Z, VX, VY = (a.astype(wd) for a in (X, Y, Z, VX, VY))
+    U, V = interpolate(ux, uy, X, Y, Z, wrap_weight_quirk)
+    FX, FY = marker_forces(U, V, VX, VY, f_max)
+    fx, fy = spread(X, Y, Z, FX, FY, ux.shape, wrap_weight_quirk, acc_dtype)
+    fxc, fyc, fzc = clip_forces(fx, fy, f_max)
+    return dict(X=X, Y=Y, Z=Z, VX=VX, VY=VY, U=U, V=V, FX=FX, FY=FY,
+                fx_raw=fx, fy_raw=fy, fx=fxc, fy=fyc, fz=fzc)
+
+
+def force_sum_f64(FX, FY, dV=None):
+    """FP64 marker force sums (sum_k F_k dV_k) with exact (fsum) accumulation.
+    The reference never reduces forces; this is the build's own readout."""
+    FX = np.asarray(FX, dtype=np.float64)
+    FY = np.asarray(FY, dtype=np.float64)
+    dv = np.ones_like(FX) if dV is None else np.asarray(dV, dtype=np.float64)
+    return math.fsum(FX * dv), math.fsum(FY * dv)
+
+
+def motion_state(step: int, x0: float, y0: float, A_heave: float, A_pitch: float,
+                 phi_pitch: float, AoA0_rad: float, w_motion: float):
+    """``_motion_state`` (main_moving.py:65-74): heave + pitch with analytic rates."""
+    t = float(step)
+    w = w_motion
+    xc = x0
+    yc = y0 + A_heave * math.sin(w * t)
+    theta = AoA0_rad + A_pitch * math.sin(w * t + phi_pitch)
+    vx = 0.0
+    vy = A_heave * w * math.cos(w * t)
+    omega = A_pitch * w * math.cos(w * t + phi_pitch)
+    return xc, yc, theta, vx, vy, omega
+
+
+def run_coupled(f, X0, Y0, Z0, steps, motion, s_v, s_e, ux_in, uy_in,
+                f_max=0.05, wrap_weight_quirk=True, first_step=1, dV=None, dtype=None):
+    """The driver loop of main_moving.py:116-122 for ``steps`` steps: per step
+    motion -> compute_force(ux, uy, ...) -> lbm step.  ``motion(step)`` returns
+    the 6-tuple of ``motion_state``.  ``f`` holds the state the loop starts
+    from; the force-free u that the first compute_force reads is K7 of it (at
+    step 1 of a fresh run that equals K1's u).  Returns (f, history) with the
+    per-step FP64 force sums."""
+    dtype = dtype or f.dtype.type
+    rho, ux, uy, uz = macro_post(f)
+    hist = []
+    for n in range(first_step, first_step + steps):
+        xc, yc, th, vx, vy, om = motion(n)
+        out = compute_force(ux, uy, X0, Y0, Z0, xc, yc, th, vx, vy, om, f_max, wrap_weight_quirk,
+                            weight_dtype=dtype)
+        fx = out["fx"].astype(dtype)
+        fy = out["fy"].astype(dtype)
+        fz = out["fz"].astype(dtype)
+        f, rho, ux, uy, uz = lbm_step(f, fx, fy, fz, s_v, s_e, ux_in, uy_in)
+        hist.append(force_sum_f64(out["FX"], out["FY"], dV))
+    return f, np.array(hist, dtype=np.float64)

@@ -1,0 +1,176 @@
+"""GPU parity of the immersed-boundary kernels against the CPU oracle.
+
+Bit-exact: marker positions (float32 rotation without contraction, cos/sin
+rounded once from float64) and the stencil base indices floor(pos) - 1.
+Tolerances: interpolated velocities and marker forces 1e-6 absolute (FP32
+weights and a different summation order than the oracle's loop); spread
+forces 1e-7 absolute (each cell sums at most a few dozen products <= 6e-3).
+"""
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import load_sub
+
+pytestmark = pytest.mark.gpu
+
+
+def _cfg(pkg, nx=24, ny=20, nz=8, chord=8.0, z_all=True, spacing=0.7):
+    X, Y, ds = pkg.config.naca0012_arclength(chord, spacing, 0.25)
+    zl = np.arange(0, nz, 1.0, dtype=np.float32) if z_all else np.array([3.0, 4.0], np.float32)
+    return pkg.config.CaseConfig(
+        name="ibm", Nx=nx, Ny=ny, Nz=nz, U_inf=0.05, Re=100, chord=chord, s_v=1 / 0.6,
+        ux_in=0.05, uy_in=0.0, X0=np.tile(X, len(zl)), Y0=np.tile(Y, len(zl)),
+        Z0=np.repeat(zl, len(X)), dV=np.full(len(X) * len(zl), ds, np.float32),
+        x0=nx / 3.0, y0=ny / 2.0)
+
+
+def _fields(shape, seed):
+    rng = np.random.default_rng(seed)
+    ux = (0.05 + 0.01 * rng.standard_normal(shape)).astype(np.float32)
+    uy = (0.01 * rng.standard_normal(shape)).astype(np.float32)
+    return ux, uy
+
+
+def _dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(np.asarray(a, np.float32).transpose(2, 1, 0))).cuda()
+
+
+KIN = [(8.3, 10.2, 0.31, 0.0, 0.004, -0.002), (7.9, 9.6, -0.52, 0.01, -0.003, 0.001),
+       (8.0, 10.0, 0.0, 0.0, 0.0, 0.0), (8.25, 10.5, 2.9, 0.0, 0.0, 0.01)]
+
+
+@pytest.mark.parametrize("kin", KIN)
+def test_update_markers_bit_exact(pkg, oracle, gpu, kin):
+    cfg = _cfg(pkg)
+    ibm = pkg.DroneMovingIBM(cfg, record_stencil=True)
+    ux, uy = _fields((cfg.Nx, cfg.Ny, cfg.Nz), 0)
+    ibm.compute_force(_dev(ux), _dev(uy), *kin)
+    X, Y, Z, VX, VY = oracle.update_markers(cfg.X0, cfg.Y0, cfg.Z0, *kin)
+    for got, ref in ((ibm.X_marker, X), (ibm.Y_marker, Y), (ibm.Z_marker, Z), (ibm.VX_body, VX),
+                     (ibm.VY_body, VY)):
+        np.testing.assert_array_equal(got.to_numpy(), ref)
+    base, *_ = oracle.stencil(X, Y, Z, cfg.Nx, cfg.Ny, cfg.Nz)
+    np.testing.assert_array_equal(ibm.stencil_base.cpu().numpy(), base)
+
+
+def test_stencil_base_bit_exact_near_integers(pkg, oracle, gpu):
+    """Markers placed on and 1 ulp either side of integer coordinates, and on
+    the periodic seam layers z = 0 and z = Nz - 1."""
+    nx, ny, nz = 20, 16, 6
+    pts = []
+    for v in (5.0, 7.0, 9.0):
+        for d in (np.float32(v), np.nextafter(np.float32(v), np.float32(0)),
+                  np.nextafter(np.float32(v), np.float32(100))):
+            pts.append(d)
+    pts = np.array(pts, np.float32)
+    zz = np.array([0.0, np.nextafter(np.float32(0), np.float32(1)), nz - 1.0, 2.5], np.float32)
+    X0 = np.repeat(pts, len(zz)) - np.float32(10.0)
+    Y0 = np.tile(pts[::-1], len(zz)) - np.float32(8.0)
+    Z0 = np.repeat(zz, len(pts))
+    cfg = pkg.config.CaseConfig(name="pts", Nx=nx, Ny=ny, Nz=nz, U_inf=0.05, Re=100, chord=4,
+                                s_v=1.5, X0=X0.astype(np.float32), Y0=Y0.astype(np.float32),
+                                Z0=Z0, x0=10.0, y0=8.0)
+    ibm = pkg.DroneMovingIBM(cfg, record_stencil=True)
+    ux, uy = _fields((nx, ny, nz), 1)
+    ibm.compute_force(_dev(ux), _dev(uy), 10.0, 8.0, 0.0, 0.0, 0.0, 0.0)
+    X, Y, Z, *_ = oracle.update_markers(cfg.X0, cfg.Y0, cfg.Z0, 10.0, 8.0, 0.0, 0.0, 0.0, 0.0)
+    base, *_ = oracle.stencil(X, Y, Z, nx, ny, nz)
+    np.testing.assert_array_equal(ibm.stencil_base.cpu().numpy(), base)
+
+
+@pytest.mark.parametrize("quirk", [True, False])
+@pytest.mark.parametrize("kin", KIN[:2])
+def test_compute_force_from_fields_matches_oracle(pkg, oracle, gpu, quirk, kin):
+    cfg = _cfg(pkg)
+    ibm = pkg.DroneMovingIBM(cfg, wrap_weight_quirk=quirk)
+    ux, uy = _fields((cfg.Nx, cfg.Ny, cfg.Nz), 2)
+    fx, fy, fz = ibm.compute_force(_dev(ux), _dev(uy), *kin)
+    ref = oracle.compute_force(ux.astype(np.float64), uy.astype(np.float64), cfg.X0, cfg.Y0, cfg.Z0,
+                               *kin, wrap_weight_quirk=quirk, weight_dtype=np.float64)
+    np.testing.assert_allclose(ibm.U_k.to_numpy(), ref["U"], rtol=0, atol=1e-6)
+    np.testing.assert_allclose(ibm.V_k.to_numpy(), ref["V"], rtol=0, atol=1e-6)
+    np.testing.assert_allclose(ibm.FX_k.to_numpy(), ref["FX"], rtol=0, atol=1e-6)
+    np.testing.assert_allclose(ibm.FY_k.to_numpy(), ref["FY"], rtol=0, atol=1e-6)
+    np.testing.assert_allclose(fx.to_numpy(), ref["fx"], rtol=0, atol=1e-7)
+    np.testing.assert_allclose(fy.to_numpy(), ref["fy"], rtol=0, atol=1e-7)
+    assert not fz.to_numpy().any()
+
+
+def test_spread_modes_agree_and_fixed_is_reproducible(pkg, oracle, gpu):
+    cfg = _cfg(pkg)
+    ux, uy = _fields((cfg.Nx, cfg.Ny, cfg.Nz), 3)
+    outs = {}
+    for mode in ("fixed", "atomic"):
+        ibm = pkg.DroneMovingIBM(cfg, spread_mode=mode)
+        a = ibm.compute_force(_dev(ux), _dev(uy), *KIN[0])[0].to_numpy()
+        b = ibm.compute_force(_dev(ux), _dev(uy), *KIN[0])[0].to_numpy()
+        outs[mode] = (a, b)
+    np.testing.assert_array_equal(outs["fixed"][0], outs["fixed"][1])
+    np.testing.assert_allclose(outs["atomic"][0], outs["fixed"][0], rtol=0, atol=1e-8)
+
+
+def test_force_box_clears_previous_cells(pkg, oracle, gpu):
+    """After the body moves, cells touched only by the previous spread read zero."""
+    cfg = _cfg(pkg, nx=40)
+    ibm = pkg.DroneMovingIBM(cfg)
+    ux, uy = _fields((cfg.Nx, cfg.Ny, cfg.Nz), 4)
+    ibm.compute_force(_dev(ux), _dev(uy), 10.0, 10.0, 0.2, 0.0, 0.0, 0.0)
+    fx, fy, _ = ibm.compute_force(_dev(ux), _dev(uy), 26.0, 9.0, -0.3, 0.0, 0.0, 0.0)
+    ref = oracle.compute_force(ux.astype(np.float64), uy.astype(np.float64), cfg.X0, cfg.Y0,
+                               cfg.Z0, 26.0, 9.0, -0.3, 0.0, 0.0, 0.0, weight_dtype=np.float64)
+    np.testing.assert_allclose(fx.to_numpy(), ref["fx"], rtol=0, atol=1e-7)
+    np.testing.assert_allclose(fy.to_numpy(), ref["fy"], rtol=0, atol=1e-7)
+    box = ibm.box.cpu().numpy()
+    nz = np.argwhere(ref["fx_raw"] != 0)
+    assert (box[0:3] <= nz.min(0)).all() and (box[3:6] >= nz.max(0)).all()
+
+
+@pytest.mark.parametrize("storage", ["f32", "f64"])
+def test_compute_force_from_solver_state(pkg, oracle, gpu, storage):
+    """ux/uy passed as the solver's own fields: the kernel computes the
+    force-free velocity from the populations (K7) at each stencil cell, at
+    both AA parities."""
+    cfg = _cfg(pkg)
+    shape = (cfg.Nx, cfg.Ny, cfg.Nz)
+    lbm = pkg.CLBMSolver(cfg, storage=storage)
+    rng = np.random.default_rng(5)
+    f, *_ = oracle.init_feq(*shape, 0.05, 0.0)
+    f = f * (1 + 0.02 * rng.standard_normal(f.shape))
+    lbm.f.from_numpy(f)
+    ibm = pkg.DroneMovingIBM(cfg)
+    for n in range(2):
+        fx, fy, fz = ibm.compute_force(lbm.ux, lbm.uy, *KIN[n])
+        _, ux, uy, _ = oracle.macro_post(f)
+        ref = oracle.compute_force(ux, uy, cfg.X0, cfg.Y0, cfg.Z0, *KIN[n], weight_dtype=np.float64)
+        np.testing.assert_allclose(ibm.U_k.to_numpy(), ref["U"], rtol=0, atol=1e-6)
+        np.testing.assert_allclose(fx.to_numpy(), ref["fx"], rtol=0, atol=1e-7)
+        lbm.step(fx, fy, fz)
+        f, *_ = oracle.lbm_step(f, ref["fx"].astype(np.float64), ref["fy"].astype(np.float64),
+                                0 * ref["fx"].astype(np.float64), cfg.s_v, cfg.s_e, cfg.ux_in,
+                                cfg.uy_in)
+        np.testing.assert_allclose(lbm.f.to_numpy(), f, rtol=0, atol=2e-6)
+
+
+def test_force_sum_f64(pkg, oracle, gpu):
+    cfg = _cfg(pkg)
+    ibm = pkg.DroneMovingIBM(cfg)
+    ux, uy = _fields((cfg.Nx, cfg.Ny, cfg.Nz), 6)
+    ibm.compute_force(_dev(ux), _dev(uy), *KIN[0])
+    got = ibm.force_sums().cpu().numpy()
+    ref = oracle.force_sum_f64(ibm.FX_k.to_numpy(), ibm.FY_k.to_numpy(), cfg.dV)
+    np.testing.assert_allclose(got, ref, rtol=1e-14, atol=1e-15)
+
+
+def test_ibm_rejects_bad_arguments(pkg, gpu):
+    lib = load_sub("_lib")
+    g = lib.ClbmGrid(8, 8, 8, 0)
+    ip = lib.IbmParams(0.05, 1, 7, 0)
+    with pytest.raises(lib.ClbmError, match="spread mode"):
+        lib.call("ibm_spread", C.byref(g), C.byref(ip), 1, *([C.c_void_p(1)] * 7), None, None)
+    ip = lib.IbmParams(-1.0, 1, 0, 0)
+    with pytest.raises(lib.ClbmError, match="f_max"):
+        lib.call("ibm_clip_forces", C.byref(g), C.byref(ip), *([C.c_void_p(1)] * 5), None)

@@ -1,0 +1,150 @@
+"""GPU parity of the lattice-Boltzmann kernels against the CPU oracle.
+
+Tolerances (stated per test): FP32 storage is compared with the FP64 oracle
+at an absolute 2e-6 on populations (~1e-5 relative to the smallest lattice
+weight 1/36) for up to ten steps; the FP64 storage path must match the FP64
+oracle to 1e-12, which fixes the algebra of the closed-form collision.
+"""
+
+import numpy as np
+import pytest
+
+from conftest import load_sub
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(pkg, nx, ny, nz, s_v=1 / 0.56, ux=0.05, uy=-0.007):
+    cfg = pkg.config.CaseConfig(name="t", Nx=nx, Ny=ny, Nz=nz, U_inf=0.05, Re=200, chord=10,
+                                s_v=s_v, ux_in=ux, uy_in=uy)
+    return cfg
+
+
+def _rand_state(oracle, shape, seed, amp=0.02, ux=0.05, uy=-0.007):
+    rng = np.random.default_rng(seed)
+    f, *_ = oracle.init_feq(*shape, ux, uy)
+    return f * (1.0 + amp * rng.standard_normal(f.shape))
+
+
+def _rand_force(shape, seed, amp=1e-3):
+    rng = np.random.default_rng(seed + 1000)
+    return [amp * rng.standard_normal(shape) for _ in range(3)]
+
+
+def _set_force(solver, arrs):
+    import torch
+    out = []
+    for a in arrs:
+        t = torch.from_numpy(np.ascontiguousarray(a.astype(np.float32).transpose(2, 1, 0))).cuda()
+        out.append(t)
+    return out
+
+
+@pytest.mark.parametrize("storage,atol", [("f32", 2e-7), ("f64", 1e-15)])
+def test_init_feq(pkg, oracle, gpu, storage, atol):
+    cfg = _case(pkg, 7, 5, 4)
+    s = pkg.CLBMSolver(cfg, storage=storage)
+    f_ref, *_ = oracle.init_feq(7, 5, 4, cfg.ux_in, cfg.uy_in)
+    np.testing.assert_allclose(s.f.to_numpy(), f_ref, rtol=0, atol=atol)
+
+
+@pytest.mark.parametrize("storage,atol", [("f32", 2e-6), ("f64", 1e-12)])
+@pytest.mark.parametrize("shape", [(9, 7, 5), (16, 12, 8), (4, 3, 3)])
+def test_steps_with_force_match_oracle(pkg, oracle, gpu, storage, atol, shape):
+    """Random post-stream state, random dense force, inlet + outlet: 1..6 steps
+    (both AA parities) against clbm_oracle.lbm_step in FP64."""
+    cfg = _case(pkg, *shape)
+    s = pkg.CLBMSolver(cfg, storage=storage)
+    f = _rand_state(oracle, shape, 1)
+    F = _rand_force(shape, 1)
+    s.f.from_numpy(f)
+    tF = _set_force(s, F)
+    for n in range(6):
+        s.step(*tF)
+        f, rho, ux, uy, uz = oracle.lbm_step(f, *F, cfg.s_v, cfg.s_e, cfg.ux_in, cfg.uy_in)
+        assert s.parity == (n + 1) % 2
+        np.testing.assert_allclose(s.f.to_numpy(), f, rtol=0, atol=atol, err_msg=f"step {n + 1}")
+    np.testing.assert_allclose(s.rho.to_numpy(), rho, rtol=0, atol=10 * atol)
+    np.testing.assert_allclose(s.ux.to_numpy(), ux, rtol=0, atol=10 * atol)
+    np.testing.assert_allclose(s.uy.to_numpy(), uy, rtol=0, atol=10 * atol)
+    np.testing.assert_allclose(s.uz.to_numpy(), uz, rtol=0, atol=10 * atol)
+
+
+@pytest.mark.parametrize("storage,atol", [("f32", 2e-6), ("f64", 1e-12)])
+def test_periodic_box_matches_oracle_and_conserves_mass(pkg, oracle, gpu, storage, atol):
+    shape = (10, 6, 5)
+    cfg = _case(pkg, *shape)
+    s = pkg.CLBMSolver(cfg, storage=storage, inlet=False)
+    f = _rand_state(oracle, shape, 2)
+    s.f.from_numpy(f)
+    m0 = f.sum()
+    for _ in range(5):
+        s.step()
+        rho, ux, uy, uz = oracle.macro_with_force(f, 0 * f[..., 0], 0 * f[..., 0], 0 * f[..., 0])
+        z = np.zeros(shape)
+        f = oracle.stream(oracle.collide(f, rho, ux, uy, uz, z, z, z, cfg.s_v, cfg.s_e))
+    got = s.f.to_numpy()
+    np.testing.assert_allclose(got, f, rtol=0, atol=atol)
+    assert abs(got.astype(np.float64).sum() - m0) < (1e-3 if storage == "f32" else 1e-10)
+
+
+def test_init_state_is_fixed_point_of_rest(pkg, oracle, gpu):
+    """rho = 1, u = 0 with no force and a periodic box: f = w is a fixed point."""
+    cfg = _case(pkg, 8, 6, 4, ux=0.0, uy=0.0)
+    s = pkg.CLBMSolver(cfg, storage="f64", inlet=False)
+    f0 = s.f.to_numpy()
+    for _ in range(4):
+        s.step()
+    np.testing.assert_allclose(s.f.to_numpy(), f0, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(f0[0, 0, 0], oracle.W, rtol=0, atol=1e-16)
+
+
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+def test_16bit_storage_tracks_fp64(pkg, oracle, gpu, storage):
+    """Deviation storage: one rounding of f - w per step; after 6 steps the
+    populations stay within the format's resolution of the FP64 result."""
+    shape = (12, 8, 6)
+    cfg = _case(pkg, *shape)
+    s = pkg.CLBMSolver(cfg, storage=storage)
+    f = _rand_state(oracle, shape, 3, amp=0.005)
+    F = _rand_force(shape, 3, amp=2e-4)
+    s.f.from_numpy(f)
+    tF = _set_force(s, F)
+    for _ in range(6):
+        s.step(*tF)
+        f, *_ = oracle.lbm_step(f, *F, cfg.s_v, cfg.s_e, cfg.ux_in, cfg.uy_in)
+    dev = np.abs(f - oracle.W).max()
+    tol = dev * (2.0 ** -10 if storage == "f16" else 2.0 ** -7) * 4
+    np.testing.assert_allclose(s.f.to_numpy(), f, rtol=0, atol=tol)
+
+
+def test_health_reports_min_and_nonfinite(pkg, oracle, gpu):
+    shape = (8, 6, 4)
+    cfg = _case(pkg, *shape)
+    s = pkg.CLBMSolver(cfg)
+    f = _rand_state(oracle, shape, 4)
+    s.f.from_numpy(f)
+    rmin, fin = s.health()
+    assert fin
+    assert abs(rmin - f.sum(-1).min()) < 1e-6
+    f[3, 2, 1, 7] = np.nan
+    s.f.from_numpy(f)
+    _, fin = s.health()
+    assert not fin
+    f[3, 2, 1, 7] = 0.0
+    f[1, 1, 1, :] *= 0.1
+    s.f.from_numpy(f)
+    rmin, fin = s.health()
+    assert fin and abs(rmin - f.sum(-1).min()) < 1e-6
+
+
+def test_step_rejects_bad_arguments(pkg, gpu):
+    lib = load_sub("_lib")
+    import ctypes as C
+    g = lib.ClbmGrid(2, 4, 4, 0)
+    p = lib.ClbmParams(1.5, 1.0, 0.05, 0.0, 1, 0)
+    with pytest.raises(lib.ClbmError, match="nx >= 3"):
+        lib.call("clbm_step", C.byref(g), C.byref(p), C.c_void_p(1), 0, None, None, None, None, None)
+    g = lib.ClbmGrid(8, 4, 4, 0)
+    with pytest.raises(lib.ClbmError, match="parity"):
+        lib.call("clbm_step", C.byref(g), C.byref(p), C.c_void_p(1), 2, None, None, None, None, None)
