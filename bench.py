@@ -135,16 +135,16 @@ def main(argv=None) -> int:
     fxp, fyp, boxp = ibm.fx._t.data_ptr(), ibm.fy._t.data_ptr(), ibm.box.data_ptr()
     state = C.c_void_p(lbm.state.data_ptr())
     sh = C.c_void_p(stream.cuda_stream)
-    par = lbm.parity
+    word = lbm.state_word
     for i in range(20):
-        lib.call("clbm_step", C.byref(lbm.grid), C.byref(params), state, par, C.c_void_p(fxp),
+        lib.call("clbm_step", C.byref(lbm.grid), C.byref(params), state, word, C.c_void_p(fxp),
                  C.c_void_p(fyp), None, C.c_void_p(boxp), sh)
-        par ^= 1
+        word = (word & 1) ^ 1
     e0.record(stream)
     for i in range(a.kernel_launches):
-        lib.call("clbm_step", C.byref(lbm.grid), C.byref(params), state, par, C.c_void_p(fxp),
+        lib.call("clbm_step", C.byref(lbm.grid), C.byref(params), state, word, C.c_void_p(fxp),
                  C.c_void_p(fyp), None, C.c_void_p(boxp), sh)
-        par ^= 1
+        word = (word & 1) ^ 1
     e1.record(stream)
     e1.synchronize()
     k_ms = e0.elapsed_time(e1) / a.kernel_launches

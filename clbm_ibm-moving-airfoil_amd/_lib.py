@@ -18,9 +18,9 @@ STORAGE = {"f32": STORE_F32, "f16": STORE_F16, "bf16": STORE_BF16, "f64": STORE_
 ELEM_BYTES = {STORE_F32: 4, STORE_F16: 2, STORE_BF16: 2, STORE_F64: 8}
 SPREAD_FIXED, SPREAD_ATOMIC_F32 = 0, 1
 SPREAD_MODES = {"fixed": SPREAD_FIXED, "atomic": SPREAD_ATOMIC_F32}
-BOX_HEADER = 16
-BOX_PARTIALS = 64
-BOX_INTS = BOX_HEADER + 6 * BOX_PARTIALS
+BOX_SUBSLOTS = 64
+BOX_WORDS = 2 * 6 * BOX_SUBSLOTS + 4        # uint64 words of a force-box record
+FORCE_BOX_OFFSET = 2 * 6 * BOX_SUBSLOTS     # word offset of the decoded int32[6] box
 ABI_VERSION = 1
 
 
@@ -35,36 +35,41 @@ class ClbmParams(C.Structure):
 
 class IbmParams(C.Structure):
     _fields_ = [("f_max", C.c_float), ("wrap_weight_quirk", C.c_int32), ("spread_mode", C.c_int32),
-                ("fixed_shift", C.c_int32)]
+                ("fixed_shift", C.c_int32), ("group", C.c_int32), ("reserved", C.c_int32)]
 
 
 _P = C.c_void_p
 _F = C.c_float
 _I = C.c_int
 _G = C.POINTER(ClbmGrid)
+_LP = C.POINTER(ClbmParams)
+_IP = C.POINTER(IbmParams)
 _SIGNATURES = {
     "clbm_abi_version": (C.c_int, []),
     "clbm_last_error": (C.c_char_p, []),
     "clbm_state_bytes": (C.c_size_t, [_G]),
+    "clbm_state_after_step": (_I, [_I, _LP]),
     "clbm_init_feq": (_I, [_G, _P, C.c_double, C.c_double, C.c_double, C.c_double, _P]),
-    "clbm_step": (_I, [_G, C.POINTER(ClbmParams), _P, _I, _P, _P, _P, _P, _P]),
-    "clbm_macro": (_I, [_G, _P, _I, _P, _P, _P, _P, _P]),
-    "clbm_export_state": (_I, [_G, _P, _I, _P, _P]),
+    "clbm_step": (_I, [_G, _LP, _P, _I, _P, _P, _P, _P, _P]),
+    "clbm_macro": (_I, [_G, _LP, _P, _I, _P, _P, _P, _P, _P]),
+    "clbm_export_state": (_I, [_G, _LP, _P, _I, _P, _P]),
+    "clbm_export_state_f64": (_I, [_G, _LP, _P, _I, _P, _P]),
     "clbm_import_state": (_I, [_G, _P, _P, _P]),
-    "clbm_export_state_f64": (_I, [_G, _P, _I, _P, _P]),
     "clbm_import_state_f64": (_I, [_G, _P, _P, _P]),
-    "clbm_health": (_I, [_G, _P, _I, _P, _P]),
+    "clbm_health": (_I, [_G, _LP, _P, _I, _P, _P]),
+    "ibm_compute_force": (_I, [_G, _IP, _LP, _P, _I, _P, _P, _I, _P, _P, _P, _P, _F, _F, _F, _F,
+                               _F, _F, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                               C.c_uint32, _P]),
     "ibm_update_markers": (_I, [_G, _I, _P, _P, _P, _F, _F, _F, _F, _F, _F, _P, _P, _P, _P, _P, _P,
-                                _P]),
-    "ibm_band_velocity": (_I, [_G, _P, _I, _P, _P, _P, _P]),
-    "ibm_interp_force": (_I, [_G, C.POINTER(IbmParams), _P, _I, _P, _P, _I, _P, _P, _P, _P, _P,
-                              _P, _P, _P, _P, _P, _P]),
-    "ibm_spread": (_I, [_G, C.POINTER(IbmParams), _I, _P, _P, _P, _P, _P, _P, _P, _P]),
-    "ibm_interp_spread": (_I, [_G, C.POINTER(IbmParams), _P, _I, _P, _P, _I, _P, _P, _P, _P, _P,
-                               _P, _P, _P, _P, _P, _P, _P, _P]),
-    "ibm_clip_forces": (_I, [_G, C.POINTER(IbmParams), _P, _P, _P, _P, _P, _P]),
+                                C.c_uint32, _P]),
+    "ibm_band_velocity": (_I, [_G, _LP, _P, _I, _P, C.c_uint32, _P, _P, _P]),
+    "ibm_interp_force": (_I, [_G, _IP, _LP, _P, _I, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                              _P, _P]),
+    "ibm_spread": (_I, [_G, _IP, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "ibm_clip_forces": (_I, [_G, _IP, _P, _P, _P, _P, _P, C.c_uint32, _P]),
     "ibm_force_sum_f64": (_I, [_I, _P, _P, _P, _P, _P]),
 }
+STATE_PARITY, STATE_PLANES = 1, 2
 
 _lock = threading.Lock()
 _lib = None

@@ -4,10 +4,11 @@ Mirrors ``DroneMovingIBM`` of airfoil_moving.py:44-255 (and its token-identical
 clone ``FishIBM``, fish/ibm_fish.py): constructed without arguments from the
 ``config_moving`` convention (or a CaseConfig), ``compute_force(ux, uy, xc,
 yc, theta, vx, vy, omega)`` returns its own ``fx, fy, fz`` fields, ready for
-``CLBMSolver.step``.  Per call it launches three kernels instead of six:
-marker update (K8), interpolate + marker force + spread (K10-K12, one
-wavefront per marker), and clamp-into-fields (K9 + K13) on the bounding box
-of the touched cells.
+``CLBMSolver.step``.  Per call it launches two kernels instead of six: one
+for marker update, interpolation, marker force and spreading (K8, K10-K12;
+markers visited in a spatially sorted order so that each workgroup can
+pre-sum its stencils), and one that clamps the accumulated force into the
+fields (K9 + K13) on the bounding box of the touched cells.
 
 Added readout with no reference counterpart: FP64 sums of the marker forces
 (``force_sums``) and a device-side history of them for C_D / C_L.
@@ -21,7 +22,7 @@ import math
 import numpy as np
 
 from . import _lib
-from .config import CaseConfig, resolve
+from .config import CaseConfig, resolve, spatial_order
 from .fields import Field, MarkerField
 
 
@@ -32,10 +33,15 @@ class DroneMovingIBM:
     fixed-point sums, the default) or "atomic" (float atomics like
     ti.atomic_add); ``wrap_weight_quirk`` False to weight wrapped stencil
     points by their unwrapped offset; ``record_stencil`` to keep floor(pos)-1
-    per marker (for index checks)."""
+    per marker (for index checks); ``order`` "natural" (the template's own
+    order, neighbours along each section) or "morton" (a Z-order sort of the
+    template) for the marker groups of the fused kernel; ``group`` markers
+    per workgroup (4, 8 or 16).  None of these tuning options change results
+    in the fixed spread mode."""
 
     def __init__(self, config=None, device=None, spread_mode: str = "fixed",
-                 f_max=None, wrap_weight_quirk=None, record_stencil: bool = False):
+                 f_max=None, wrap_weight_quirk=None, record_stencil: bool = False,
+                 order: str = "natural", group: int = 8):
         import torch
         cfg: CaseConfig = resolve(config)
         self.cfg = cfg
@@ -57,24 +63,26 @@ class DroneMovingIBM:
         self.U_k, self.V_k = marr(), marr()
         self.FX_k, self.FY_k = marr(), marr()
         self.dV = marr(cfg.dV) if cfg.dV is not None else None
+        # visiting order: neighbours in the body frame stay neighbours after any rigid motion
+        self.order = (torch.from_numpy(spatial_order(cfg.X0, cfg.Y0, cfg.Z0)).to(dev)
+                      if order == "morton" else None)
         shape = (self.Nx, self.Ny, self.Nz)
         ncell = self.Nx * self.Ny * self.Nz
-        empty = [2**31 - 1] * 3 + [-2**31] * 3
-        box = torch.tensor(empty + empty + [0] * (_lib.BOX_INTS - 12), dtype=torch.int32, device=dev)
-        self.box = box
-        # force-free velocity of the band cells (written only inside the force box)
-        self._band = torch.zeros((2, ncell), dtype=f32, device=dev)
+        # force-box record (uint64 words, zero = no generation yet) and the generation counter
+        self.box = torch.zeros(_lib.BOX_WORDS, dtype=torch.int64, device=dev)
+        self.gen = 0
+        box_ptr = self.box.data_ptr() + 8 * _lib.FORCE_BOX_OFFSET
         self.fx = Field(shape, torch.zeros((self.Nz, self.Ny, self.Nx), dtype=f32, device=dev),
-                        owner=self, box=box)
+                        owner=self, box=box_ptr)
         self.fy = Field(shape, torch.zeros((self.Nz, self.Ny, self.Nx), dtype=f32, device=dev),
-                        owner=self, box=box)
+                        owner=self, box=box_ptr)
         self.fz = Field(shape, torch.zeros((self.Nz, self.Ny, self.Nx), dtype=f32, device=dev),
-                        owner=self, box=box)
+                        owner=self, box=box_ptr)
         self._acc = torch.zeros((2, ncell), dtype=torch.int64, device=dev)
         self.ip = _lib.IbmParams(float(cfg.f_max if f_max is None else f_max),
                                  int(cfg.wrap_weight_quirk if wrap_weight_quirk is None
                                      else wrap_weight_quirk),
-                                 _lib.SPREAD_MODES[spread_mode], 0)
+                                 _lib.SPREAD_MODES[spread_mode], 0, int(group), 0)
         self.stencil_base = (torch.zeros((self.N, 3), dtype=torch.int32, device=dev)
                              if record_stencil else None)
         self._hist = None
@@ -90,28 +98,29 @@ class DroneMovingIBM:
         device Field/tensor.  Returns (fx, fy, fz), this object's own fields."""
         st = self._stream()
         p = lambda m: C.c_void_p(m.data_ptr())
-        grid, f, parity, uxp, uyp = self._velocity_source(ux, uy)
+        grid, lp, f, state, uxp, uyp = self._velocity_source(ux, uy)
+        self.gen += 1
         boxp = C.c_void_p(self.box.data_ptr())
-        _lib.call("ibm_update_markers", C.byref(grid), self.N, p(self.X0), p(self.Y0), p(self.Z0),
-                  float(xc), float(yc), float(theta), float(vx), float(vy), float(omega),
-                  p(self.X_marker), p(self.Y_marker), p(self.Z_marker), p(self.VX_body),
-                  p(self.VY_body), boxp, st)
-        if f:
-            # velocity of the solver: computed once per band cell from its populations
-            uxp, uyp = self._band[0].data_ptr(), self._band[1].data_ptr()
-            _lib.call("ibm_band_velocity", C.byref(grid), C.c_void_p(f), parity, boxp,
-                      C.c_void_p(uxp), C.c_void_p(uyp), st)
-        _lib.call("ibm_interp_spread", C.byref(grid), C.byref(self.ip), C.c_void_p(f), parity,
-                  C.c_void_p(uxp), C.c_void_p(uyp), self.N, p(self.X_marker), p(self.Y_marker),
-                  p(self.Z_marker), p(self.VX_body), p(self.VY_body), p(self.U_k), p(self.V_k),
-                  p(self.FX_k), p(self.FY_k),
+        acc0, acc1 = C.c_void_p(self._acc[0].data_ptr()), C.c_void_p(self._acc[1].data_ptr())
+        _lib.call("ibm_compute_force", C.byref(grid), C.byref(self.ip), lp, C.c_void_p(f), state,
+                  C.c_void_p(uxp), C.c_void_p(uyp), self.N,
+                  C.c_void_p(self.order.data_ptr() if self.order is not None else 0),
+                  p(self.X0), p(self.Y0),
+                  p(self.Z0), float(xc), float(yc), float(theta), float(vx), float(vy),
+                  float(omega), p(self.X_marker), p(self.Y_marker), p(self.Z_marker),
+                  p(self.VX_body), p(self.VY_body), p(self.U_k), p(self.V_k), p(self.FX_k),
+                  p(self.FY_k),
                   C.c_void_p(self.stencil_base.data_ptr() if self.stencil_base is not None else 0),
-                  C.c_void_p(self._acc[0].data_ptr()), C.c_void_p(self._acc[1].data_ptr()), st)
-        _lib.call("ibm_clip_forces", C.byref(grid), C.byref(self.ip),
-                  C.c_void_p(self._acc[0].data_ptr()), C.c_void_p(self._acc[1].data_ptr()),
-                  C.c_void_p(self.fx._t.data_ptr()), C.c_void_p(self.fy._t.data_ptr()),
-                  C.c_void_p(self.box.data_ptr()), st)
+                  acc0, acc1, boxp, self.gen, st)
+        _lib.call("ibm_clip_forces", C.byref(grid), C.byref(self.ip), acc0, acc1,
+                  C.c_void_p(self.fx._t.data_ptr()), C.c_void_p(self.fy._t.data_ptr()), boxp,
+                  self.gen, st)
         return self.fx, self.fy, self.fz
+
+    def force_box(self):
+        """The (x0, y0, z0, x1, y1, z1) box of the cells the last compute_force reached."""
+        return self.box[_lib.FORCE_BOX_OFFSET:_lib.FORCE_BOX_OFFSET + 3].cpu().numpy().view(
+            np.int32).copy()
 
     def _velocity_source(self, ux, uy):
         from .clbm_moving import CLBMSolver
@@ -119,11 +128,12 @@ class DroneMovingIBM:
         if isinstance(owner, CLBMSolver) and getattr(uy, "owner", None) is owner:
             if (owner.Nx, owner.Ny, owner.Nz) != (self.Nx, self.Ny, self.Nz):
                 raise ValueError("solver and IBM lattices differ")
-            return owner.grid, owner.state.data_ptr(), owner.parity, 0, 0
+            return (owner.grid, C.byref(owner.params), owner.state.data_ptr(), owner.state_word,
+                    0, 0)
         from .fields import as_device_ptr
         shape = (self.Nx, self.Ny, self.Nz)
         grid = _lib.ClbmGrid(self.Nx, self.Ny, self.Nz, _lib.STORE_F32)
-        return grid, 0, 0, as_device_ptr(ux, shape), as_device_ptr(uy, shape)
+        return grid, None, 0, 0, as_device_ptr(ux, shape), as_device_ptr(uy, shape)
 
     # -- force readout (no reference counterpart) ----------------------------
     def force_sums(self, out=None):

@@ -7,7 +7,10 @@ advanced by ``step(fx, fy, fz)``, read through ``f, rho, ux, uy, uz`` and
 
 Differences of mechanism (not of results):
 * one population buffer updated in place (AA pattern) instead of f/f_post;
-  ``parity`` records which of the two in-place layouts it currently holds;
+  ``state_word`` records which of the two in-place layouts it holds (bit 0)
+  and whether the inlet/outlet planes of the last step are still to be
+  applied by the next reader (bit 1), which is how the reference's K5/K6
+  run without a pass of their own;
 * ``rho, ux, uy, uz`` are computed on demand (K7 of clbm_moving.py:305-321)
   when read, instead of after every step; the immersed-boundary object reads
   the force-free velocity straight from the populations.
@@ -58,8 +61,8 @@ class PopulationField:
         out = torch.empty((Q, s.Nz, s.Ny, s.Nx), dtype=torch.float64 if wide else torch.float32,
                           device=s.device)
         _lib.call("clbm_export_state_f64" if wide else "clbm_export_state", C.byref(s.grid),
-                  C.c_void_p(s.state.data_ptr()), s.parity, C.c_void_p(out.data_ptr()),
-                  s._stream())
+                  C.byref(s.params), C.c_void_p(s.state.data_ptr()), s.state_word,
+                  C.c_void_p(out.data_ptr()), s._stream())
         return np.ascontiguousarray(out.cpu().numpy().transpose(3, 2, 1, 0))
 
     def from_numpy(self, arr) -> None:
@@ -72,7 +75,7 @@ class PopulationField:
         src = torch.from_numpy(np.ascontiguousarray(a.transpose(3, 2, 1, 0))).to(s.device)
         _lib.call("clbm_import_state_f64" if wide else "clbm_import_state", C.byref(s.grid),
                   C.c_void_p(s.state.data_ptr()), C.c_void_p(src.data_ptr()), s._stream())
-        s.parity = 0
+        s.state_word = 0
         s._macro_valid = False
         if s.device.type == "cuda":
             torch.cuda.current_stream(s.device).synchronize()
@@ -102,7 +105,7 @@ class CLBMSolver:
                                       float(cfg.uy_in), 1 if inlet else 0, 0)
         nbytes = Q * self.Nx * self.Ny * self.Nz * _lib.ELEM_BYTES[self.grid.storage]
         self.state = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
-        self.parity = 0
+        self.state_word = 0
         self.steps = 0
         shape = (self.Nx, self.Ny, self.Nz)
         self._macro = None
@@ -120,13 +123,13 @@ class CLBMSolver:
         """K1 (clbm_moving.py:75-91): equilibrium at the inlet velocity, rho = 1."""
         _lib.call("clbm_init_feq", C.byref(self.grid), C.c_void_p(self.state.data_ptr()), 1.0,
                   float(self.cfg.ux_in), float(self.cfg.uy_in), 0.0, self._stream())
-        self.parity = 0
+        self.state_word = 0
         self._macro_valid = False
 
     def step(self, fx=None, fy=None, fz=None) -> None:
         """Advance one step with body force (fx, fy, fz) (clbm_moving.py:326-343)."""
         shape = (self.Nx, self.Ny, self.Nz)
-        box = None
+        box = 0
         for fld in (fx, fy):
             if isinstance(fld, Field) and fld.box is not None:
                 box = fld.box
@@ -134,13 +137,18 @@ class CLBMSolver:
         if isinstance(fz, Field) and fz.box is not None:
             fzp = 0     # the immersed-boundary fz is identically zero (airfoil_moving.py:225)
         _lib.call("clbm_step", C.byref(self.grid), C.byref(self.params),
-                  C.c_void_p(self.state.data_ptr()), self.parity,
+                  C.c_void_p(self.state.data_ptr()), self.state_word,
                   C.c_void_p(as_device_ptr(fx, shape)), C.c_void_p(as_device_ptr(fy, shape)),
-                  C.c_void_p(fzp), C.c_void_p(box.data_ptr() if box is not None else 0),
+                  C.c_void_p(fzp), C.c_void_p(box),
                   self._stream())
-        self.parity ^= 1
+        self.state_word = _lib.load().clbm_state_after_step(self.state_word, C.byref(self.params))
         self.steps += 1
         self._macro_valid = False
+
+    @property
+    def parity(self) -> int:
+        """Number of steps taken mod 2 (which AA layout the buffer holds)."""
+        return self.state_word & _lib.STATE_PARITY
 
     @property
     def f_post(self):
@@ -158,7 +166,8 @@ class CLBMSolver:
             for i, fld in enumerate((self.rho, self.ux, self.uy, self.uz)):
                 fld._t = self._macro[i]
         m = self._macro
-        _lib.call("clbm_macro", C.byref(self.grid), C.c_void_p(self.state.data_ptr()), self.parity,
+        _lib.call("clbm_macro", C.byref(self.grid), C.byref(self.params),
+                  C.c_void_p(self.state.data_ptr()), self.state_word,
                   C.c_void_p(m[0].data_ptr()), C.c_void_p(m[1].data_ptr()),
                   C.c_void_p(m[2].data_ptr()), C.c_void_p(m[3].data_ptr()), self._stream())
         self._macro_valid = True
@@ -168,8 +177,9 @@ class CLBMSolver:
         (the divergence guard of main_moving.py:143-145)."""
         import torch
         out = torch.empty(2, dtype=torch.float32, device=self.device)
-        _lib.call("clbm_health", C.byref(self.grid), C.c_void_p(self.state.data_ptr()),
-                  self.parity, C.c_void_p(out.data_ptr()), self._stream())
+        _lib.call("clbm_health", C.byref(self.grid), C.byref(self.params),
+                  C.c_void_p(self.state.data_ptr()), self.state_word, C.c_void_p(out.data_ptr()),
+                  self._stream())
         h = out.cpu().numpy()
         return float(h[0]), bool(h[1] > 0.5)
 

@@ -122,6 +122,21 @@ def naca0012_arclength(chord: float, spacing: float = 1.0, pivot_frac: float = 0
     return X.astype(np.float32), Y.astype(np.float32), per / n
 
 
+def spatial_order(X0, Y0, Z0, cell: float = 1.0) -> np.ndarray:
+    """Marker visiting order along a Morton (Z-order) curve of the body-frame
+    positions quantised to ``cell``: consecutive markers are neighbours, so
+    the 4x4x4 stencils of a group of them overlap.  Returns int32 indices."""
+    pts = np.stack([np.asarray(a, dtype=np.float64) for a in (X0, Y0, Z0)], 1)
+    if len(pts) == 0:
+        return np.zeros(0, np.int32)
+    q = np.floor((pts - pts.min(0)) / cell).astype(np.uint64)
+    key = np.zeros(len(q), dtype=np.uint64)
+    for bit in range(21):
+        for a in range(3):
+            key |= ((q[:, a] >> np.uint64(bit)) & np.uint64(1)) << np.uint64(3 * bit + a)
+    return np.argsort(key, kind="stable").astype(np.int32)
+
+
 # ---------------------------------------------------------------------------
 # case configuration
 # ---------------------------------------------------------------------------

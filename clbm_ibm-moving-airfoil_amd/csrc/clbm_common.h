@@ -175,6 +175,47 @@ __device__ __forceinline__ void load_state(const typename Store<S>::T* __restric
   for (int q = 0; q < Q; ++q) fq[q] = Store<S>::ld(f + slot_in<PAR>(q, N, c, nb), q);
 }
 
+// The reference's inlet rule at x = 0 (CLBMSolver._bc_inlet, clbm_moving.py:
+// 266-294): the five east-going populations from the known ones, in place.
+template <typename R>
+__device__ __forceinline__ void inlet_rule(R* a, R ux_in, R uy_in) {
+  const R s_known = a[0] + a[3] + a[4] + a[5] + a[6] + a[15] + a[16] + a[17] + a[18];
+  const R s_out = a[2] + a[8] + a[10] + a[12] + a[14];
+  const R rho_in = (s_known + R(2) * s_out) / (R(1) - ux_in);
+  const R dy = a[3] - a[4], dz = a[5] - a[6];
+  const R e = (R(1) / R(6)) * rho_in * ux_in;
+  a[1] = a[2] + (R(2) / R(3)) * rho_in * ux_in;
+  a[7] = a[10] + e + R(0.5) * rho_in * uy_in - R(0.5) * dy;
+  a[9] = a[8] + e - R(0.5) * rho_in * uy_in + R(0.5) * dy;
+  a[11] = a[14] + e - R(0.5) * dz;
+  a[13] = a[12] + e + R(0.5) * dz;
+}
+
+// Boundary planes of a state: `on` when the x planes still hold the raw
+// streamed values, so that readers apply the inlet rule at x = 0 and take
+// cell nx-2's populations for x = nx-1 (CLBMSolver._bc_outlet,
+// clbm_moving.py:296-300) as they load.
+struct Planes {
+  int on;
+  double ux_in, uy_in;
+};
+
+// Post-stream, post-boundary populations of cell (x, y, z): what the
+// reference's f holds between steps.
+template <int S, int PAR>
+__device__ __forceinline__ void load_cell(const typename Store<S>::T* __restrict__ f, const Geo& g,
+                                          int c, int x, int y, int z, const Planes& pl,
+                                          typename Store<S>::R* a) {
+  using R = typename Store<S>::R;
+  if (pl.on && x == g.nx - 1) {
+    c -= 1;
+    x -= 1;
+  }
+  const Nbr nb = make_nbr(g, x, y, z);
+  load_state<S, PAR>(f, (size_t)g.ncell, c, nb, a);
+  if (pl.on && x == 0) inlet_rule<R>(a, R(pl.ux_in), R(pl.uy_in));
+}
+
 // ---------------------------------------------------------------------------
 // Errors
 // ---------------------------------------------------------------------------
@@ -182,5 +223,7 @@ void set_error(const std::string& s);
 int fail(int code, const std::string& s);
 int check_launch(const char* what);
 int check_grid(const clbm_grid_t* g, bool need_inlet_width);
+// Validates a state word and its boundary parameters; fills `pl`.
+int check_state(const clbm_grid_t* g, const clbm_params_t* p, int state, Planes& pl);
 
 }  // namespace clbm

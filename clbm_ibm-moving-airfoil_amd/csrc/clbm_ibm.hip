@@ -1,24 +1,23 @@
 // Immersed-boundary kernels of the CLBM-IBM step for gfx950 (MI355X).
 //
 // DroneMovingIBM.compute_force (airfoil_moving.py:230-255) runs six kernels,
-// three of them over the full grid.  Here every kernel works on markers or on
-// the band of cells the markers' 4x4x4 Peskin stencils reach:
-//   * ibm_update_markers   K8, one thread per marker; also reduces the
-//                          bounding box of all stencil cells (the "force
-//                          box") with a last-block reduction;
-//   * ibm_band_velocity    force-free velocity (K7's formula) of the box
-//                          cells only, straight from the population state;
-//   * ibm_interp_spread    K10 + K11 + K12: one 64-lane wavefront per marker,
-//                          one lane per stencil point (x fastest across
-//                          lanes); the wave reduces U_k, V_k, forms the
-//                          clamped marker force and every lane adds F_k w to
-//                          its cell with the same weight;
-//   * ibm_clip_forces      K9 + K13 on the union of this and the previous
-//                          force box (cells outside both are already zero).
-// Spreading is order-independent by default: each product F_k w is converted
-// to a 64-bit fixed-point integer and summed with integer atomics, so the
-// grid force has the same bit pattern on every run.  The reference's float
-// atomics (airfoil_moving.py:213-214) are kept as IBM_SPREAD_ATOMIC_F32.
+// three of them over the full grid.  Here the whole of it is two launches:
+//   * ibm_compute_force  K8 + K10 + K11 + K12.  One 64-lane wavefront per
+//                        marker, one lane per point of its 4x4x4 Peskin
+//                        stencil (x fastest across lanes).  The wave places
+//                        its marker, each lane reads the force-free velocity
+//                        of its cell straight from the populations, the wave
+//                        reduces U_k, V_k and the clamped marker force, and
+//                        each lane spreads F_k w with the same weight.  A
+//                        workgroup holds 16 markers that are neighbours in
+//                        the body (the caller passes a spatially sorted
+//                        order), pre-sums their overlapping stencils in LDS
+//                        and sends one global atomic per touched cell.
+//   * ibm_clip_forces    K9 + K13 on the bounding box of the touched cells.
+// Spreading is order-independent by default: each product F_k w becomes a
+// 64-bit fixed-point integer and all sums (LDS and global) are integer sums,
+// so the grid force is the same bit pattern on every run.  The reference's
+// float atomics (airfoil_moving.py:213-214) are kept as IBM_SPREAD_ATOMIC_F32.
 #include "clbm_common.h"
 
 #include <algorithm>
@@ -27,9 +26,8 @@
 
 namespace clbm {
 
-constexpr int IBLK = 256;       // threads per block of the marker kernels
-constexpr int MPB = IBLK / 64;  // markers per block of the stencil kernel
-constexpr int UBLK_MAX = IBM_BOX_PARTIALS;  // blocks of the marker-update kernel
+constexpr int IBLK = 256;        // threads per block of the small marker kernels
+constexpr int MPB = IBLK / 64;   // markers per block of the separate stencil kernel
 
 // Peskin 4-point regularised delta (airfoil_moving.py:27-38).
 __device__ __forceinline__ float phi1d(float r) {
@@ -48,6 +46,55 @@ __device__ __forceinline__ int pmod(int a, int n) {
   return r < 0 ? r + n : r;
 }
 
+// ---------------------------------------------------------------------------
+// force box: atomic maxima of (gen << 32 | key); lower corners are stored as
+// INT32_MAX - coordinate so that both corners are maxima.  Each generation
+// has IBM_BOX_SUBSLOTS copies, picked by block index: atomics on one address
+// serialise, so spreading the workgroups' maxima over separate words keeps
+// them off the critical path; readers take the maximum over the copies.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void box_grow(uint64_t* box, uint32_t gen, const int* lo, const int* hi) {
+  uint64_t* slot =
+      box + 6 * ((gen & 1u) * IBM_BOX_SUBSLOTS + (blockIdx.x & (IBM_BOX_SUBSLOTS - 1)));
+  const uint64_t g = (uint64_t)gen << 32;
+  for (int a = 0; a < 3; ++a) {
+    if (lo[a] > hi[a]) continue;
+    atomicMax((unsigned long long*)&slot[a], (unsigned long long)(g | (uint32_t)(INT_MAX - lo[a])));
+    atomicMax((unsigned long long*)&slot[3 + a], (unsigned long long)(g | (uint32_t)hi[a]));
+  }
+}
+// Box of generation gen, combined by the workgroup's first wavefront (one
+// lane per copy) and shared through LDS; every thread of the block must call
+// it.  False when no stencil of that generation touched a cell.
+__device__ bool box_get_block(const uint64_t* box, uint32_t gen, int* lo, int* hi) {
+  static_assert(IBM_BOX_SUBSLOTS == 64, "one lane per box copy");
+  __shared__ uint64_t sk[6];
+  if (threadIdx.x < 64) {
+    uint64_t k[6] = {0, 0, 0, 0, 0, 0};
+    if (gen != 0) {
+      const uint64_t* slot = box + 6 * ((gen & 1u) * IBM_BOX_SUBSLOTS + threadIdx.x);
+      for (int a = 0; a < 6; ++a) k[a] = slot[a];
+    }
+    for (int o = 32; o > 0; o >>= 1)
+      for (int a = 0; a < 6; ++a) {
+        const uint64_t t = __shfl_xor(k[a], o);
+        k[a] = t > k[a] ? t : k[a];
+      }
+    if (threadIdx.x == 0)
+      for (int a = 0; a < 6; ++a) sk[a] = k[a];
+  }
+  __syncthreads();
+  bool ok = gen != 0;
+  for (int a = 0; a < 3; ++a) {
+    const uint64_t kl = sk[a], kh = sk[3 + a];
+    ok = ok && (uint32_t)(kl >> 32) == gen && (uint32_t)(kh >> 32) == gen;
+    lo[a] = INT_MAX - (int)(uint32_t)kl;
+    hi[a] = (int)(uint32_t)kh;
+  }
+  __syncthreads();  // sk is reused by the next call
+  return ok;
+}
+
 // Inclusive index range of the 4 stencil cells along one axis.  x is
 // clamped; a periodic axis whose 4 cells wrap takes the whole axis.
 __device__ __forceinline__ void axis_range_clamped(int b, int n, int& lo, int& hi) {
@@ -63,53 +110,52 @@ __device__ __forceinline__ void axis_range_periodic(int b, int n, int& lo, int& 
     hi = n - 1;
   }
 }
+__device__ __forceinline__ void stencil_ranges(const Geo& g, int ib, int jb, int kb, int* lo, int* hi) {
+  axis_range_clamped(ib, g.nx, lo[0], hi[0]);
+  axis_range_periodic(jb, g.ny, lo[1], hi[1]);
+  axis_range_periodic(kb, g.nz, lo[2], hi[2]);
+}
 
 // ---------------------------------------------------------------------------
-// K8 _update_markers (airfoil_moving.py:100-127) + force box of the stencils
+// K8 arithmetic (airfoil_moving.py:114-127), float32 without contraction
 // ---------------------------------------------------------------------------
+struct Kin {
+  float xc, yc, ct, st, vx, vy, omega;
+};
+__device__ __forceinline__ void place_marker(const Kin& k, float x0m, float y0m, float& xk,
+                                             float& yk, float& vxk, float& vyk) {
+#pragma clang fp contract(off)
+  xk = (k.xc + x0m * k.ct) - y0m * k.st;
+  yk = (k.yc + x0m * k.st) + y0m * k.ct;
+  vxk = k.vx - k.omega * (yk - k.yc);
+  vyk = k.vy + k.omega * (xk - k.xc);
+}
+
 __global__ __launch_bounds__(IBLK) void k_update_markers(
     Geo g, int n, const float* __restrict__ X0, const float* __restrict__ Y0,
-    const float* __restrict__ Z0, float xc, float yc, float theta, float vx, float vy, float omega,
-    float* __restrict__ X, float* __restrict__ Y, float* __restrict__ Z, float* __restrict__ VX,
-    float* __restrict__ VY, int32_t* __restrict__ box) {
-#pragma clang fp contract(off)
+    const float* __restrict__ Z0, Kin kin, float* __restrict__ X, float* __restrict__ Y,
+    float* __restrict__ Z, float* __restrict__ VX, float* __restrict__ VY, uint64_t* box,
+    uint32_t gen) {
   __shared__ int sb[IBLK / 64][6];
-  __shared__ int last;
-  const double td = (double)theta;
-  const float ct = (float)cos(td);
-  const float st = (float)sin(td);
   int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
-  for (int m = blockIdx.x * IBLK + threadIdx.x; m < n; m += gridDim.x * IBLK) {
-    const float x0m = X0[m], y0m = Y0[m];
-    const float xk = (xc + x0m * ct) - y0m * st;
-    const float yk = (yc + x0m * st) + y0m * ct;
+  const int m = blockIdx.x * IBLK + threadIdx.x;
+  if (m < n) {
+    float xk, yk, vxk, vyk;
+    place_marker(kin, X0[m], Y0[m], xk, yk, vxk, vyk);
     const float zk = Z0[m];
     X[m] = xk;
     Y[m] = yk;
     Z[m] = zk;
-    VX[m] = vx - omega * (yk - yc);
-    VY[m] = vy + omega * (xk - xc);
-    if (box) {
-      int l, h;
-      axis_range_clamped((int)floorf(xk) - 1, g.nx, l, h);
-      lo[0] = min(lo[0], l);
-      hi[0] = max(hi[0], h);
-      axis_range_periodic((int)floorf(yk) - 1, g.ny, l, h);
-      lo[1] = min(lo[1], l);
-      hi[1] = max(hi[1], h);
-      axis_range_periodic((int)floorf(zk) - 1, g.nz, l, h);
-      lo[2] = min(lo[2], l);
-      hi[2] = max(hi[2], h);
-    }
+    VX[m] = vxk;
+    VY[m] = vyk;
+    stencil_ranges(g, (int)floorf(xk) - 1, (int)floorf(yk) - 1, (int)floorf(zk) - 1, lo, hi);
   }
   if (!box) return;
-  // block reduction of the box
-  for (int a = 0; a < 3; ++a) {
+  for (int a = 0; a < 3; ++a)
     for (int o = 32; o > 0; o >>= 1) {
       lo[a] = min(lo[a], __shfl_xor(lo[a], o));
       hi[a] = max(hi[a], __shfl_xor(hi[a], o));
     }
-  }
   const int wv = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0)
     for (int a = 0; a < 3; ++a) {
@@ -117,62 +163,13 @@ __global__ __launch_bounds__(IBLK) void k_update_markers(
       sb[wv][3 + a] = hi[a];
     }
   __syncthreads();
-  int32_t* part = box + IBM_BOX_HEADER;
-  if (threadIdx.x < 6) {
-    const int a = threadIdx.x;
-    int v = sb[0][a];
-    for (int i = 1; i < IBLK / 64; ++i) v = a < 3 ? min(v, sb[i][a]) : max(v, sb[i][a]);
-    part[6 * blockIdx.x + a] = v;
-  }
-  // the last block to finish combines the partial boxes
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) last = (atomicAdd(&box[13], 1) == (int)gridDim.x - 1);
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  if (threadIdx.x < 6) {
-    const int a = threadIdx.x;
-    volatile int32_t* vp = part;
-    int v = vp[a];
-    for (int b = 1; b < (int)gridDim.x; ++b) v = a < 3 ? min(v, vp[6 * b + a]) : max(v, vp[6 * b + a]);
-    box[6 + a] = box[a];  // previous box
-    box[a] = v;
-  }
-  if (threadIdx.x == 0) box[13] = 0;
-}
-
-// ---------------------------------------------------------------------------
-// Force-free velocity on the box cells (K7's formula, clbm_moving.py:305-321)
-// ---------------------------------------------------------------------------
-template <int S, int PAR>
-__global__ __launch_bounds__(IBLK) void k_band_velocity(Geo g, const typename Store<S>::T* __restrict__ f,
-                                                        const int32_t* __restrict__ box,
-                                                        float* __restrict__ uo, float* __restrict__ vo) {
-  using R = typename Store<S>::R;
-  const int x0 = box[0], y0 = box[1], z0 = box[2];
-  const int bx = box[3] - x0 + 1, by = box[4] - y0 + 1, bz = box[5] - z0 + 1;
-  if (bx <= 0 || by <= 0 || bz <= 0) return;
-  const int vol = bx * by * bz;
-  for (int t = blockIdx.x * IBLK + threadIdx.x; t < vol; t += gridDim.x * IBLK) {
-    const int r = t / bx;
-    const int x = x0 + (t - r * bx);
-    const int k = r / by;
-    const int y = y0 + (r - k * by);
-    const int z = z0 + k;
-    const int c = (z * g.ny + y) * g.nx + x;
-    const Nbr nb = make_nbr(g, x, y, z);
-    R a[Q];
-    load_state<S, PAR>(f, (size_t)g.ncell, c, nb, a);
-    R rho = R(0), mx = R(0), my = R(0);
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      rho += a[q];
-      if (ex(q)) mx += R(ex(q)) * a[q];
-      if (ey(q)) my += R(ey(q)) * a[q];
-    }
-    uo[c] = (float)(mx / rho);
-    vo[c] = (float)(my / rho);
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < IBLK / 64; ++i)
+      for (int a = 0; a < 3; ++a) {
+        sb[0][a] = min(sb[0][a], sb[i][a]);
+        sb[0][3 + a] = max(sb[0][3 + a], sb[i][3 + a]);
+      }
+    box_grow(box, gen, &sb[0][0], &sb[0][3]);
   }
 }
 
@@ -182,25 +179,27 @@ __global__ __launch_bounds__(IBLK) void k_band_velocity(Geo g, const typename St
 // ---------------------------------------------------------------------------
 struct Lane {
   int ib, jb, kb;  // floor(pos) - 1
-  int ii, jj, kk;  // clamped / wrapped cell indices
+  int iu, ju, ku;  // unwrapped stencil coordinates
+  int cell;        // clamped / wrapped cell index
+  int ii, jj, kk;  // clamped / wrapped cell coordinates
   float w;         // (phx * phy) * phz
-  int cell;
 };
 
 __device__ __forceinline__ Lane lane_stencil(const Geo& g, int quirk, float xk, float yk, float zk,
                                              int lane) {
   Lane L;
-  const int di = lane & 3, dj = (lane >> 2) & 3, dk = lane >> 4;
   L.ib = (int)floorf(xk) - 1;
   L.jb = (int)floorf(yk) - 1;
   L.kb = (int)floorf(zk) - 1;
-  const int iu = L.ib + di, ju = L.jb + dj, ku = L.kb + dk;
-  L.ii = min(max(iu, 0), g.nx - 1);
-  L.jj = pmod(ju, g.ny);
-  L.kk = pmod(ku, g.nz);
-  const float phx = phi1d((float)(quirk ? L.ii : iu) - xk);
-  const float phy = phi1d((float)(quirk ? L.jj : ju) - yk);
-  const float phz = phi1d((float)(quirk ? L.kk : ku) - zk);
+  L.iu = L.ib + (lane & 3);
+  L.ju = L.jb + ((lane >> 2) & 3);
+  L.ku = L.kb + (lane >> 4);
+  L.ii = min(max(L.iu, 0), g.nx - 1);
+  L.jj = pmod(L.ju, g.ny);
+  L.kk = pmod(L.ku, g.nz);
+  const float phx = phi1d((float)(quirk ? L.ii : L.iu) - xk);
+  const float phy = phi1d((float)(quirk ? L.jj : L.ju) - yk);
+  const float phz = phi1d((float)(quirk ? L.kk : L.ku) - zk);
   L.w = (phx * phy) * phz;
   L.cell = (L.kk * g.ny + L.jj) * g.nx + L.ii;
   return L;
@@ -212,14 +211,15 @@ __device__ __forceinline__ float wave_sum(float v) {
   return v;
 }
 
-// Force-free velocity of one cell from the population state.
+// Force-free velocity of one cell from the population state (K7's formula,
+// clbm_moving.py:305-321).
 template <int S, int PAR>
-__device__ __forceinline__ void cell_velocity(const Geo& g, const typename Store<S>::T* f, int c,
-                                              int x, int y, int z, float& uo, float& vo) {
+__device__ __forceinline__ void cell_velocity(const Geo& g, const Planes& pl,
+                                              const typename Store<S>::T* f, int c, int x, int y,
+                                              int z, float& uo, float& vo) {
   using R = typename Store<S>::R;
-  const Nbr nb = make_nbr(g, x, y, z);
   R a[Q];
-  load_state<S, PAR>(f, (size_t)g.ncell, c, nb, a);
+  load_cell<S, PAR>(f, g, c, x, y, z, pl, a);
   R r = R(0), mx = R(0), my = R(0);
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
@@ -231,38 +231,229 @@ __device__ __forceinline__ void cell_velocity(const Geo& g, const typename Store
   vo = (float)(my / r);
 }
 
+template <int S, int UMODE>
+__device__ __forceinline__ void lane_velocity(const Geo& g, const Planes& pl,
+                                              const typename Store<S>::T* f, const float* uxf,
+                                              const float* uyf, const Lane& L, float& uc,
+                                              float& vc) {
+  if (UMODE == 0) {
+    uc = uxf[L.cell];
+    vc = uyf[L.cell];
+  } else if (UMODE == 1) {
+    cell_velocity<S, 0>(g, pl, f, L.cell, L.ii, L.jj, L.kk, uc, vc);
+  } else {
+    cell_velocity<S, 1>(g, pl, f, L.cell, L.ii, L.jj, L.kk, uc, vc);
+  }
+}
+
+// One contribution into the global accumulators.
+template <int SMODE>
+__device__ __forceinline__ void acc_add(void* accx, void* accy, int c, float px, float py,
+                                        float fscale) {
+  if (SMODE == IBM_SPREAD_FIXED) {
+    atomicAdd((unsigned long long*)accx + c, (unsigned long long)__float2ll_rn(px * fscale));
+    atomicAdd((unsigned long long*)accy + c, (unsigned long long)__float2ll_rn(py * fscale));
+  } else {
+    atomicAdd((float*)((unsigned long long*)accx + c), px);
+    atomicAdd((float*)((unsigned long long*)accy + c), py);
+  }
+}
+
 // ---------------------------------------------------------------------------
-// K10 + K11 (+ K12 when SPREAD): one wavefront per marker.
+// The fused kernel: K8 + K10 + K11 + K12 for G markers per workgroup.
 // UMODE 0: velocity from ux/uy fields; 1: from the state at parity 0; 2: parity 1.
+//
+// The workgroup's stencils are staged in an LDS tile spanning the bounding
+// box of their unwrapped coordinates: the force-free velocity of each tile
+// cell is computed once (instead of once per stencil point that reaches it),
+// and the spread products are summed per tile cell before one global atomic
+// per touched cell.  A group whose box exceeds the tile (markers far apart)
+// takes the direct path for both.
+// ---------------------------------------------------------------------------
+template <int G> struct Group {
+  static constexpr int threads = 64 * G;
+  static constexpr int tile = 128 * G;  // cells
+};
+
+template <int S, int UMODE, int SMODE, int G>
+__global__ __launch_bounds__(64 * G) void k_compute_force(
+    Geo g, Planes pl, int quirk, float f_max, float fscale, const typename Store<S>::T* __restrict__ f,
+    const float* __restrict__ uxf, const float* __restrict__ uyf, int n,
+    const int32_t* __restrict__ order, const float* __restrict__ X0, const float* __restrict__ Y0,
+    const float* __restrict__ Z0, Kin kin, float* __restrict__ X, float* __restrict__ Y,
+    float* __restrict__ Z, float* __restrict__ VX, float* __restrict__ VY, float* __restrict__ U,
+    float* __restrict__ V, float* __restrict__ FX, float* __restrict__ FY, int32_t* __restrict__ base,
+    void* __restrict__ accx, void* __restrict__ accy, uint64_t* __restrict__ box, uint32_t gen) {
+  constexpr int NT = Group<G>::threads, TC = Group<G>::tile;
+  __shared__ unsigned long long sacc[2 * TC];  // spread sums (x then y)
+  __shared__ float svel[2 * TC];               // staged velocity (u then v)
+  __shared__ int sbase[G][3];
+  __shared__ int srng[G][6];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int slot = blockIdx.x * G + wv;
+  const bool live = slot < n;  // uniform per wavefront
+  const int m = live ? (order ? order[slot] : slot) : 0;
+
+  // K8: place the marker (every lane of the wave computes the same values)
+  float xk = 0.f, yk = 0.f, zk = 0.f, vxk = 0.f, vyk = 0.f;
+  if (live) {
+    place_marker(kin, X0[m], Y0[m], xk, yk, vxk, vyk);
+    zk = Z0[m];
+    if (lane == 0) {
+      X[m] = xk;
+      Y[m] = yk;
+      Z[m] = zk;
+      VX[m] = vxk;
+      VY[m] = vyk;
+    }
+  }
+  const Lane L = lane_stencil(g, quirk, xk, yk, zk, lane);
+  if (lane == 0) {
+    sbase[wv][0] = live ? L.ib : INT_MAX;
+    sbase[wv][1] = live ? L.jb : INT_MAX;
+    sbase[wv][2] = live ? L.kb : INT_MAX;
+    int lo[3], hi[3];
+    stencil_ranges(g, L.ib, L.jb, L.kb, lo, hi);
+    for (int a = 0; a < 3; ++a) {
+      srng[wv][a] = live ? lo[a] : INT_MAX;
+      srng[wv][3 + a] = live ? hi[a] : INT_MIN;
+    }
+    if (live && base) {
+      base[3 * m + 0] = L.ib;
+      base[3 * m + 1] = L.jb;
+      base[3 * m + 2] = L.kb;
+    }
+  }
+  __syncthreads();
+  // the tile: bounding box of the workgroup's unwrapped stencil coordinates
+  int o[3] = {INT_MAX, INT_MAX, INT_MAX}, e[3] = {INT_MIN, INT_MIN, INT_MIN};
+  for (int i = 0; i < G; ++i) {
+    if (sbase[i][0] == INT_MAX) continue;
+    for (int a = 0; a < 3; ++a) {
+      o[a] = min(o[a], sbase[i][a]);
+      e[a] = max(e[a], sbase[i][a] + 3);
+    }
+  }
+  const bool any = o[0] != INT_MAX;
+  const long long lbx = (long long)e[0] - o[0] + 1, lby = (long long)e[1] - o[1] + 1,
+                  lbz = (long long)e[2] - o[2] + 1;
+  const bool use_tile = any && lbx * lby * lbz <= TC;
+  const int bx = use_tile ? (int)lbx : 0, by = use_tile ? (int)lby : 0;
+  const int vol = use_tile ? (int)(lbx * lby * lbz) : 0;
+  // stage the velocity of every tile cell; clear the spread sums
+  for (int j = threadIdx.x; j < vol; j += NT) {
+    const int ix = j % bx, r = j / bx;
+    const int iy = r % by, iz = r / by;
+    const int ii = min(max(o[0] + ix, 0), g.nx - 1);
+    const int jj = pmod(o[1] + iy, g.ny);
+    const int kk = pmod(o[2] + iz, g.nz);
+    Lane T;
+    T.ii = ii;
+    T.jj = jj;
+    T.kk = kk;
+    T.cell = (kk * g.ny + jj) * g.nx + ii;
+    float u, v;
+    lane_velocity<S, UMODE>(g, pl, f, uxf, uyf, T, u, v);
+    svel[j] = u;
+    svel[TC + j] = v;
+    if (SMODE == IBM_SPREAD_FIXED) {
+      sacc[j] = 0ull;
+      sacc[TC + j] = 0ull;
+    } else {
+      ((float*)sacc)[j] = 0.0f;
+      ((float*)sacc)[TC + j] = 0.0f;
+    }
+  }
+  __syncthreads();
+
+  if (live) {
+    const int j = use_tile ? ((L.ku - o[2]) * by + (L.ju - o[1])) * bx + (L.iu - o[0]) : 0;
+    // K10 _interpolate (airfoil_moving.py:142-170)
+    float uc, vc;
+    if (use_tile) {
+      uc = svel[j];
+      vc = svel[TC + j];
+    } else {
+      lane_velocity<S, UMODE>(g, pl, f, uxf, uyf, L, uc, vc);
+    }
+    const float su = wave_sum(uc * L.w);
+    const float sv = wave_sum(vc * L.w);
+    // K11 _marker_forces (airfoil_moving.py:175-183)
+    const float fxk = fmaxf(-f_max, fminf(f_max, -(su - vxk)));
+    const float fyk = fmaxf(-f_max, fminf(f_max, -(sv - vyk)));
+    if (lane == 0) {
+      U[m] = su;
+      V[m] = sv;
+      FX[m] = fxk;
+      FY[m] = fyk;
+    }
+    // K12 _spread (airfoil_moving.py:188-214)
+    const float px = fxk * L.w, py = fyk * L.w;
+    if (use_tile) {
+      if (SMODE == IBM_SPREAD_FIXED) {
+        atomicAdd(&sacc[j], (unsigned long long)__float2ll_rn(px * fscale));
+        atomicAdd(&sacc[TC + j], (unsigned long long)__float2ll_rn(py * fscale));
+      } else {
+        atomicAdd(&((float*)sacc)[j], px);
+        atomicAdd(&((float*)sacc)[TC + j], py);
+      }
+    } else {
+      acc_add<SMODE>(accx, accy, L.cell, px, py, fscale);
+    }
+  }
+  __syncthreads();
+  // one global atomic per touched cell of the tile
+  for (int j = threadIdx.x; j < vol; j += NT) {
+    const int ix = j % bx, r = j / bx;
+    const int iy = r % by, iz = r / by;
+    const int ii = min(max(o[0] + ix, 0), g.nx - 1);
+    const int jj = pmod(o[1] + iy, g.ny);
+    const int kk = pmod(o[2] + iz, g.nz);
+    const int c = (kk * g.ny + jj) * g.nx + ii;
+    if (SMODE == IBM_SPREAD_FIXED) {
+      const unsigned long long a = sacc[j], b = sacc[TC + j];
+      if (a) atomicAdd((unsigned long long*)accx + c, a);
+      if (b) atomicAdd((unsigned long long*)accy + c, b);
+    } else {
+      const float a = ((const float*)sacc)[j], b = ((const float*)sacc)[TC + j];
+      if (a != 0.0f) atomicAdd((float*)((unsigned long long*)accx + c), a);
+      if (b != 0.0f) atomicAdd((float*)((unsigned long long*)accy + c), b);
+    }
+  }
+  // the box of the cells this generation's stencils reach
+  if (box && threadIdx.x == 0) {
+    int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+    for (int i = 0; i < G; ++i)
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = min(lo[a], srng[i][a]);
+        hi[a] = max(hi[a], srng[i][3 + a]);
+      }
+    box_grow(box, gen, lo, hi);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The separate K10/K11 and K12 kernels (one wavefront per marker, no LDS).
 // ---------------------------------------------------------------------------
 template <int S, int UMODE, bool INTERP, bool SPREAD, int SMODE>
 __global__ __launch_bounds__(IBLK) void k_marker(
-    Geo g, int quirk, float f_max, float fscale, const typename Store<S>::T* __restrict__ f,
-    const float* __restrict__ uxf, const float* __restrict__ uyf, int n,
-    const float* __restrict__ X, const float* __restrict__ Y, const float* __restrict__ Z,
-    const float* __restrict__ VX, const float* __restrict__ VY, float* __restrict__ U,
-    float* __restrict__ V, float* __restrict__ FXo, float* __restrict__ FYo,
+    Geo g, Planes pl, int quirk, float f_max, float fscale,
+    const typename Store<S>::T* __restrict__ f, const float* __restrict__ uxf,
+    const float* __restrict__ uyf, int n, const float* __restrict__ X, const float* __restrict__ Y,
+    const float* __restrict__ Z, const float* __restrict__ VX, const float* __restrict__ VY,
+    float* __restrict__ U, float* __restrict__ V, float* __restrict__ FXo, float* __restrict__ FYo,
     const float* __restrict__ FXi, const float* __restrict__ FYi, int32_t* __restrict__ base,
     void* __restrict__ accx, void* __restrict__ accy) {
   const int lane = threadIdx.x & 63;
   const int m = blockIdx.x * MPB + (threadIdx.x >> 6);
   if (m >= n) return;  // uniform per wavefront
-  const float xk = X[m], yk = Y[m], zk = Z[m];
-  const Lane L = lane_stencil(g, quirk, xk, yk, zk, lane);
+  const Lane L = lane_stencil(g, quirk, X[m], Y[m], Z[m], lane);
   float fxk, fyk;
   if (INTERP) {
     float uc, vc;
-    if (UMODE == 0) {
-      uc = uxf[L.cell];
-      vc = uyf[L.cell];
-    } else if (UMODE == 1) {
-      cell_velocity<S, 0>(g, f, L.cell, L.ii, L.jj, L.kk, uc, vc);
-    } else {
-      cell_velocity<S, 1>(g, f, L.cell, L.ii, L.jj, L.kk, uc, vc);
-    }
+    lane_velocity<S, UMODE>(g, pl, f, uxf, uyf, L, uc, vc);
     const float su = wave_sum(uc * L.w);
     const float sv = wave_sum(vc * L.w);
-    // K11 _marker_forces (airfoil_moving.py:175-183)
     fxk = fmaxf(-f_max, fminf(f_max, -(su - VX[m])));
     fyk = fmaxf(-f_max, fminf(f_max, -(sv - VY[m])));
     if (lane == 0) {
@@ -280,32 +471,65 @@ __global__ __launch_bounds__(IBLK) void k_marker(
     base[3 * m + 1] = L.jb;
     base[3 * m + 2] = L.kb;
   }
-  if (SPREAD) {
-    const float px = fxk * L.w, py = fyk * L.w;
-    if (SMODE == IBM_SPREAD_FIXED) {
-      atomicAdd((unsigned long long*)accx + L.cell, (unsigned long long)__float2ll_rn(px * fscale));
-      atomicAdd((unsigned long long*)accy + L.cell, (unsigned long long)__float2ll_rn(py * fscale));
-    } else {
-      atomicAdd((float*)((unsigned long long*)accx + L.cell), px);
-      atomicAdd((float*)((unsigned long long*)accy + L.cell), py);
-    }
+  if (SPREAD) acc_add<SMODE>(accx, accy, L.cell, fxk * L.w, fyk * L.w, fscale);
+}
+
+// ---------------------------------------------------------------------------
+// Force-free velocity on the cells of a generation's box
+// ---------------------------------------------------------------------------
+template <int S, int PAR>
+__global__ __launch_bounds__(IBLK) void k_band_velocity(Geo g, Planes pl,
+                                                        const typename Store<S>::T* __restrict__ f,
+                                                        const uint64_t* __restrict__ box,
+                                                        uint32_t gen, float* __restrict__ uo,
+                                                        float* __restrict__ vo) {
+  int lo[3], hi[3];
+  if (!box_get_block(box, gen, lo, hi)) return;
+  const int bx = hi[0] - lo[0] + 1, by = hi[1] - lo[1] + 1, bz = hi[2] - lo[2] + 1;
+  const int vol = bx * by * bz;
+  for (int t = blockIdx.x * IBLK + threadIdx.x; t < vol; t += gridDim.x * IBLK) {
+    const int r = t / bx;
+    const int x = lo[0] + (t - r * bx);
+    const int k = r / by;
+    const int y = lo[1] + (r - k * by);
+    const int z = lo[2] + k;
+    const int c = (z * g.ny + y) * g.nx + x;
+    float u, v;
+    cell_velocity<S, PAR>(g, pl, f, c, x, y, z, u, v);
+    uo[c] = u;
+    vo[c] = v;
   }
 }
 
 // ---------------------------------------------------------------------------
-// K9 + K13: accumulators -> clamped float fields on the union of the boxes.
+// K9 + K13: accumulators -> clamped float fields on the union of the boxes
+// of generations gen and gen - 1; block 0 publishes gen's box for clbm_step.
 // ---------------------------------------------------------------------------
 template <int SMODE>
 __global__ __launch_bounds__(IBLK) void k_clip(Geo g, float f_max, float inv_scale,
                                                void* __restrict__ accx, void* __restrict__ accy,
                                                float* __restrict__ fx, float* __restrict__ fy,
-                                               const int32_t* __restrict__ box) {
-  int lo[3], hi[3];
-  for (int a = 0; a < 3; ++a) {
-    lo[a] = min(box[a], box[6 + a]);
-    hi[a] = max(box[3 + a], box[9 + a]);
+                                               uint64_t* __restrict__ box, uint32_t gen) {
+  int lo[3], hi[3], plo[3], phi[3];
+  const bool cur = box_get_block(box, gen, lo, hi);
+  const bool prev = box_get_block(box, gen - 1, plo, phi);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    int32_t* out = IBM_FORCE_BOX(box);
+    for (int a = 0; a < 3; ++a) {
+      out[a] = cur ? lo[a] : INT_MAX;
+      out[3 + a] = cur ? hi[a] : INT_MIN;
+    }
   }
-  if (lo[0] > hi[0] || lo[1] > hi[1] || lo[2] > hi[2]) return;  // both boxes empty
+  if (!cur && !prev) return;
+  for (int a = 0; a < 3; ++a) {
+    if (!cur) {
+      lo[a] = plo[a];
+      hi[a] = phi[a];
+    } else if (prev) {
+      lo[a] = min(lo[a], plo[a]);
+      hi[a] = max(hi[a], phi[a]);
+    }
+  }
   const int bx = hi[0] - lo[0] + 1, by = hi[1] - lo[1] + 1, bz = hi[2] - lo[2] + 1;
   const int vol = bx * by * bz;
   for (int t = blockIdx.x * IBLK + threadIdx.x; t < vol; t += gridDim.x * IBLK) {
@@ -338,31 +562,35 @@ __global__ __launch_bounds__(IBLK) void k_clip(Geo g, float f_max, float inv_sca
 // ---------------------------------------------------------------------------
 // FP64 marker force sums, one block, fixed reduction tree (deterministic).
 // ---------------------------------------------------------------------------
-constexpr int SBLK = 256;
+constexpr int SBLK = 1024;
 __global__ __launch_bounds__(SBLK) void k_force_sum(int n, const float* __restrict__ FX,
                                                     const float* __restrict__ FY,
                                                     const float* __restrict__ dV,
                                                     double* __restrict__ out2) {
-  __shared__ double sx[SBLK], sy[SBLK];
+  __shared__ double sx[SBLK / 64], sy[SBLK / 64];
   double ax = 0.0, ay = 0.0;
   for (int m = threadIdx.x; m < n; m += SBLK) {
     const double d = dV ? (double)dV[m] : 1.0;
     ax += (double)FX[m] * d;
     ay += (double)FY[m] * d;
   }
-  sx[threadIdx.x] = ax;
-  sy[threadIdx.x] = ay;
-  __syncthreads();
-  for (int s = SBLK / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) {
-      sx[threadIdx.x] += sx[threadIdx.x + s];
-      sy[threadIdx.x] += sy[threadIdx.x + s];
-    }
-    __syncthreads();
+  for (int o = 32; o > 0; o >>= 1) {
+    ax += __shfl_xor(ax, o);
+    ay += __shfl_xor(ay, o);
   }
+  if ((threadIdx.x & 63) == 0) {
+    sx[threadIdx.x >> 6] = ax;
+    sy[threadIdx.x >> 6] = ay;
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
-    out2[0] = sx[0];
-    out2[1] = sy[0];
+    double tx = 0.0, ty = 0.0;
+    for (int i = 0; i < SBLK / 64; ++i) {
+      tx += sx[i];
+      ty += sy[i];
+    }
+    out2[0] = tx;
+    out2[1] = ty;
   }
 }
 
@@ -375,6 +603,8 @@ static int check_ibm(const ibm_params_t* ip) {
     return fail(-1, "unknown spread mode");
   if (!(ip->f_max > 0.0f) || !std::isfinite(ip->f_max)) return fail(-1, "f_max must be > 0");
   if (ip->fixed_shift < 0 || ip->fixed_shift > 60) return fail(-1, "fixed_shift out of [0, 60]");
+  if (ip->group != 0 && ip->group != 4 && ip->group != 8 && ip->group != 16)
+    return fail(-1, "group must be 0, 4, 8 or 16");
   return 0;
 }
 
@@ -387,69 +617,104 @@ static int fixed_shift(const ibm_params_t* ip) {
   return s < 0 ? 0 : (s > 60 ? 60 : s);
 }
 
-static inline int mblocks(int n) { return (n + MPB - 1) / MPB; }
+static inline Kin make_kin(float xc, float yc, float theta, float vx, float vy, float omega) {
+  // cos/sin of the float32 angle in float64, rounded once (the same for every marker)
+  return Kin{xc, yc, (float)std::cos((double)theta), (float)std::sin((double)theta), vx, vy, omega};
+}
 
-// Blocks of the grid-stride box kernels: enough to cover a box of the whole
-// lattice in a few passes, and at least one block per CU.
+// Blocks of the grid-stride box kernels.
 static inline int box_blocks(const Geo& g) {
-  return std::max(1, std::min(4096, (g.ncell + IBLK - 1) / IBLK));
+  return std::max(1, std::min(1024, (g.ncell + IBLK - 1) / IBLK));
+}
+
+template <int S, int UMODE, int SMODE>
+static void launch_compute(const Geo& g, const Planes& pl, const ibm_params_t* ip, const void* f,
+                           const float* ux, const float* uy, int n, const int32_t* order,
+                           const float* X0, const float* Y0, const float* Z0, const Kin& kin,
+                           float* X, float* Y, float* Z, float* VX, float* VY, float* U, float* V,
+                           float* FX, float* FY, int32_t* base, void* ax, void* ay, uint64_t* box,
+                           uint32_t gen, hipStream_t st) {
+  const float scale = (float)std::ldexp(1.0, fixed_shift(ip));
+  const int q = ip->wrap_weight_quirk ? 1 : 0;
+  auto fp = (const typename Store<S>::T*)f;
+#define CLBM_CG(GG)                                                                              \
+  hipLaunchKernelGGL((k_compute_force<S, UMODE, SMODE, GG>), dim3((n + GG - 1) / GG),           \
+                     dim3(64 * GG), 0, st, g, pl, q, ip->f_max, scale, fp, ux, uy, n, order, X0, \
+                     Y0, Z0, kin, X, Y, Z, VX, VY, U, V, FX, FY, base, ax, ay, box, gen)
+  switch (ip->group) {
+    case 4: CLBM_CG(4); break;
+    case 16: CLBM_CG(16); break;
+    default: CLBM_CG(8); break;
+  }
+#undef CLBM_CG
+}
+
+template <int S, int SMODE>
+static void launch_compute_u(int umode, const Geo& g, const Planes& pl, const ibm_params_t* ip,
+                             const void* f, const float* ux, const float* uy, int n,
+                             const int32_t* order, const float* X0, const float* Y0,
+                             const float* Z0, const Kin& kin, float* X, float* Y, float* Z,
+                             float* VX, float* VY, float* U, float* V, float* FX, float* FY,
+                             int32_t* base, void* ax, void* ay, uint64_t* box, uint32_t gen,
+                             hipStream_t st) {
+#define CLBM_CF(UM)                                                                               \
+  launch_compute<S, UM, SMODE>(g, pl, ip, f, ux, uy, n, order, X0, Y0, Z0, kin, X, Y, Z, VX, VY, U, \
+                               V, FX, FY, base, ax, ay, box, gen, st)
+  if (umode == 0)
+    CLBM_CF(0);
+  else if (umode == 1)
+    CLBM_CF(1);
+  else
+    CLBM_CF(2);
+#undef CLBM_CF
 }
 
 template <int S, bool INTERP, bool SPREAD, int SMODE>
-static void launch_marker(const Geo& g, const ibm_params_t* ip, const void* f, int parity,
-                          const float* ux, const float* uy, int n, const float* X, const float* Y,
-                          const float* Z, const float* VX, const float* VY, float* U, float* V,
-                          float* FXo, float* FYo, const float* FXi, const float* FYi, int32_t* base,
-                          void* accx, void* accy, hipStream_t st) {
+static void launch_marker(const Geo& g, const ibm_params_t* ip, const Planes& pl, const void* f,
+                          int umode, const float* ux, const float* uy, int n, const float* X,
+                          const float* Y, const float* Z, const float* VX, const float* VY,
+                          float* U, float* V, float* FXo, float* FYo, const float* FXi,
+                          const float* FYi, int32_t* base, void* accx, void* accy,
+                          hipStream_t st) {
   const float scale = (float)std::ldexp(1.0, fixed_shift(ip));
   auto fp = (const typename Store<S>::T*)f;
-  const dim3 gr(mblocks(n)), bl(IBLK);
+  const dim3 gr((n + MPB - 1) / MPB), bl(IBLK);
   const int uq = ip->wrap_weight_quirk ? 1 : 0;
-  if (!INTERP || (ux && uy))
-    hipLaunchKernelGGL((k_marker<S, 0, INTERP, SPREAD, SMODE>), gr, bl, 0, st, g, uq, ip->f_max,
-                       scale, fp, ux, uy, n, X, Y, Z, VX, VY, U, V, FXo, FYo, FXi, FYi, base, accx,
-                       accy);
-  else if (parity == 0)
-    hipLaunchKernelGGL((k_marker<S, 1, INTERP, SPREAD, SMODE>), gr, bl, 0, st, g, uq, ip->f_max,
-                       scale, fp, ux, uy, n, X, Y, Z, VX, VY, U, V, FXo, FYo, FXi, FYi, base, accx,
-                       accy);
+#define CLBM_MK(UM)                                                                             \
+  hipLaunchKernelGGL((k_marker<S, UM, INTERP, SPREAD, SMODE>), gr, bl, 0, st, g, pl, uq,        \
+                     ip->f_max, scale, fp, ux, uy, n, X, Y, Z, VX, VY, U, V, FXo, FYo, FXi, FYi, \
+                     base, accx, accy)
+  if (umode == 0)
+    CLBM_MK(0);
+  else if (umode == 1)
+    CLBM_MK(1);
   else
-    hipLaunchKernelGGL((k_marker<S, 2, INTERP, SPREAD, SMODE>), gr, bl, 0, st, g, uq, ip->f_max,
-                       scale, fp, ux, uy, n, X, Y, Z, VX, VY, U, V, FXo, FYo, FXi, FYi, base, accx,
-                       accy);
-}
-
-template <bool INTERP, bool SPREAD, int SMODE>
-static int marker_dispatch(const clbm_grid_t* g, const ibm_params_t* ip, const void* f, int parity,
-                           const float* ux, const float* uy, int n, const float* X, const float* Y,
-                           const float* Z, const float* VX, const float* VY, float* U, float* V,
-                           float* FXo, float* FYo, const float* FXi, const float* FYi,
-                           int32_t* base, void* accx, void* accy, hipStream_t st) {
-  const Geo geo = make_geo(*g);
-#define CLBM_MK(SS)                                                                              \
-  launch_marker<SS, INTERP, SPREAD, SMODE>(geo, ip, f, parity, ux, uy, n, X, Y, Z, VX, VY, U, V, \
-                                           FXo, FYo, FXi, FYi, base, accx, accy, st)
-  switch (g->storage) {
-    case CLBM_STORE_F32: CLBM_MK(CLBM_STORE_F32); break;
-    case CLBM_STORE_F16: CLBM_MK(CLBM_STORE_F16); break;
-    case CLBM_STORE_BF16: CLBM_MK(CLBM_STORE_BF16); break;
-    case CLBM_STORE_F64: CLBM_MK(CLBM_STORE_F64); break;
-    default: return fail(-1, "unknown storage format");
-  }
+    CLBM_MK(2);
 #undef CLBM_MK
-  return check_launch("ibm marker kernel");
 }
 
-template <int S>
-static void launch_band(const Geo& g, const void* f, int parity, const int32_t* box, float* uo,
-                        float* vo, hipStream_t st) {
-  auto fp = (const typename Store<S>::T*)f;
-  if (parity == 0)
-    hipLaunchKernelGGL((k_band_velocity<S, 0>), dim3(box_blocks(g)), dim3(IBLK), 0, st, g, fp, box,
-                       uo, vo);
-  else
-    hipLaunchKernelGGL((k_band_velocity<S, 1>), dim3(box_blocks(g)), dim3(IBLK), 0, st, g, fp, box,
-                       uo, vo);
+#define CLBM_STORAGE_SWITCH(storage, CALL)                   \
+  switch (storage) {                                         \
+    case CLBM_STORE_F32: { constexpr int SS = CLBM_STORE_F32; CALL; } break;   \
+    case CLBM_STORE_F16: { constexpr int SS = CLBM_STORE_F16; CALL; } break;   \
+    case CLBM_STORE_BF16: { constexpr int SS = CLBM_STORE_BF16; CALL; } break; \
+    case CLBM_STORE_F64: { constexpr int SS = CLBM_STORE_F64; CALL; } break;   \
+    default: return fail(-1, "unknown storage format");      \
+  }
+
+// Velocity source: 0 = ux/uy fields, 1/2 = populations at parity 0/1.
+static int velocity_source(const clbm_grid_t* g, const clbm_params_t* p, const void* f, int state,
+                           const float* ux, const float* uy, Planes& pl, int& umode) {
+  pl.on = 0;
+  pl.ux_in = pl.uy_in = 0.0;
+  if (ux && uy) {
+    umode = 0;
+    return 0;
+  }
+  if (!f) return fail(-1, "need ux and uy, or the population state f");
+  if (int e = check_state(g, p, state, pl)) return e;
+  umode = 1 + (state & CLBM_STATE_PARITY);
+  return 0;
 }
 
 }  // namespace clbm
@@ -458,63 +723,93 @@ using namespace clbm;
 
 extern "C" {
 
+int ibm_compute_force(const clbm_grid_t* g, const ibm_params_t* ip, const clbm_params_t* p,
+                      const void* f, int state, const float* ux, const float* uy, int n,
+                      const int32_t* order, const float* X0, const float* Y0, const float* Z0,
+                      float xc, float yc, float theta, float vx, float vy, float omega, float* X,
+                      float* Y, float* Z, float* VX, float* VY, float* U, float* V, float* FX,
+                      float* FY, int32_t* stencil_base, void* acc_x, void* acc_y, uint64_t* box,
+                      uint32_t gen, void* stream) {
+  if (int e = check_grid(g, false)) return e;
+  if (int e = check_ibm(ip)) return e;
+  if (n < 0) return fail(-1, "ibm_compute_force: n < 0");
+  if (n == 0) return 0;
+  if (!X0 || !Y0 || !Z0 || !X || !Y || !Z || !VX || !VY || !U || !V || !FX || !FY)
+    return fail(-1, "ibm_compute_force: NULL marker array");
+  if (!acc_x || !acc_y) return fail(-1, "ibm_compute_force: NULL accumulator");
+  if (box && gen == 0) return fail(-1, "ibm_compute_force: generation must be >= 1");
+  Planes pl;
+  int umode;
+  if (int e = velocity_source(g, p, f, state, ux, uy, pl, umode)) return e;
+  const Geo geo = make_geo(*g);
+  const Kin kin = make_kin(xc, yc, theta, vx, vy, omega);
+  hipStream_t st = (hipStream_t)stream;
+  if (ip->spread_mode == IBM_SPREAD_FIXED) {
+    CLBM_STORAGE_SWITCH(g->storage, (launch_compute_u<SS, IBM_SPREAD_FIXED>(
+                                        umode, geo, pl, ip, f, ux, uy, n, order, X0, Y0, Z0, kin, X,
+                                        Y, Z, VX, VY, U, V, FX, FY, stencil_base, acc_x, acc_y,
+                                        box, gen, st)));
+  } else {
+    CLBM_STORAGE_SWITCH(g->storage, (launch_compute_u<SS, IBM_SPREAD_ATOMIC_F32>(
+                                        umode, geo, pl, ip, f, ux, uy, n, order, X0, Y0, Z0, kin, X,
+                                        Y, Z, VX, VY, U, V, FX, FY, stencil_base, acc_x, acc_y,
+                                        box, gen, st)));
+  }
+  return check_launch("ibm_compute_force");
+}
+
 int ibm_update_markers(const clbm_grid_t* g, int n, const float* X0, const float* Y0,
                        const float* Z0, float xc, float yc, float theta, float vx, float vy,
                        float omega, float* X, float* Y, float* Z, float* VX, float* VY,
-                       int32_t* box, void* stream) {
+                       uint64_t* box, uint32_t gen, void* stream) {
   if (int e = check_grid(g, false)) return e;
   if (n < 0) return fail(-1, "ibm_update_markers: n < 0");
-  if (n > 0 && (!X0 || !Y0 || !Z0 || !X || !Y || !Z || !VX || !VY))
+  if (n == 0) return 0;
+  if (!X0 || !Y0 || !Z0 || !X || !Y || !Z || !VX || !VY)
     return fail(-1, "ibm_update_markers: NULL marker array");
-  if (n == 0 && !box) return 0;
-  const int nb = std::max(1, std::min(UBLK_MAX, (n + IBLK - 1) / IBLK));
-  hipLaunchKernelGGL(k_update_markers, dim3(nb), dim3(IBLK), 0, (hipStream_t)stream,
-                     make_geo(*g), n, X0, Y0, Z0, xc, yc, theta, vx, vy, omega, X, Y, Z, VX, VY,
-                     box);
+  if (box && gen == 0) return fail(-1, "ibm_update_markers: generation must be >= 1");
+  hipLaunchKernelGGL(k_update_markers, dim3((n + IBLK - 1) / IBLK), dim3(IBLK), 0,
+                     (hipStream_t)stream, make_geo(*g), n, X0, Y0, Z0,
+                     make_kin(xc, yc, theta, vx, vy, omega), X, Y, Z, VX, VY, box, gen);
   return check_launch("ibm_update_markers");
 }
 
-int ibm_band_velocity(const clbm_grid_t* g, const void* f, int parity, const int32_t* box,
-                      float* ux, float* uy, void* stream) {
+int ibm_band_velocity(const clbm_grid_t* g, const clbm_params_t* p, const void* f, int state,
+                      const uint64_t* box, uint32_t gen, float* ux, float* uy, void* stream) {
   if (int e = check_grid(g, false)) return e;
   if (!f || !box || !ux || !uy) return fail(-1, "ibm_band_velocity: NULL buffer");
-  if (parity != 0 && parity != 1) return fail(-1, "ibm_band_velocity: parity must be 0 or 1");
+  Planes pl;
+  if (int e = check_state(g, p, state, pl)) return e;
+  const int par = state & CLBM_STATE_PARITY;
   const Geo geo = make_geo(*g);
   hipStream_t st = (hipStream_t)stream;
-  switch (g->storage) {
-    case CLBM_STORE_F32: launch_band<CLBM_STORE_F32>(geo, f, parity, box, ux, uy, st); break;
-    case CLBM_STORE_F16: launch_band<CLBM_STORE_F16>(geo, f, parity, box, ux, uy, st); break;
-    case CLBM_STORE_BF16: launch_band<CLBM_STORE_BF16>(geo, f, parity, box, ux, uy, st); break;
-    case CLBM_STORE_F64: launch_band<CLBM_STORE_F64>(geo, f, parity, box, ux, uy, st); break;
-    default: return fail(-1, "unknown storage format");
-  }
+#define CLBM_BAND(PP)                                                                          \
+  hipLaunchKernelGGL((k_band_velocity<SS, PP>), dim3(box_blocks(geo)), dim3(IBLK), 0, st, geo, \
+                     pl, (const typename Store<SS>::T*)f, box, gen, ux, uy)
+  CLBM_STORAGE_SWITCH(g->storage, if (par) { CLBM_BAND(1); } else { CLBM_BAND(0); });
+#undef CLBM_BAND
   return check_launch("ibm_band_velocity");
 }
 
-static int interp_args(const clbm_grid_t* g, const ibm_params_t* ip, const void* f, int parity,
-                       const float* ux, const float* uy, int n, const float* X, const float* Y,
-                       const float* Z, const float* VX, const float* VY) {
+int ibm_interp_force(const clbm_grid_t* g, const ibm_params_t* ip, const clbm_params_t* p,
+                     const void* f, int state, const float* ux, const float* uy, int n,
+                     const float* X, const float* Y, const float* Z, const float* VX,
+                     const float* VY, float* U, float* V, float* FX, float* FY,
+                     int32_t* stencil_base, void* stream) {
   if (int e = check_grid(g, false)) return e;
   if (int e = check_ibm(ip)) return e;
-  if (n < 0) return fail(-1, "n < 0");
-  if (n > 0 && (!X || !Y || !Z || !VX || !VY)) return fail(-1, "NULL marker array");
-  if (!(ux && uy)) {
-    if (!f) return fail(-1, "need ux and uy, or the population state f");
-    if (parity != 0 && parity != 1) return fail(-1, "parity must be 0 or 1");
-  }
-  return 0;
-}
-
-int ibm_interp_force(const clbm_grid_t* g, const ibm_params_t* ip, const void* f, int parity,
-                     const float* ux, const float* uy, int n, const float* X, const float* Y,
-                     const float* Z, const float* VX, const float* VY, float* U, float* V,
-                     float* FX, float* FY, int32_t* stencil_base, void* stream) {
-  if (int e = interp_args(g, ip, f, parity, ux, uy, n, X, Y, Z, VX, VY)) return e;
+  if (n < 0) return fail(-1, "ibm_interp_force: n < 0");
   if (n == 0) return 0;
-  return marker_dispatch<true, false, IBM_SPREAD_FIXED>(g, ip, f, parity, ux, uy, n, X, Y, Z, VX,
-                                                         VY, U, V, FX, FY, nullptr, nullptr,
-                                                         stencil_base, nullptr, nullptr,
-                                                         (hipStream_t)stream);
+  if (!X || !Y || !Z || !VX || !VY) return fail(-1, "ibm_interp_force: NULL marker array");
+  Planes pl;
+  int umode;
+  if (int e = velocity_source(g, p, f, state, ux, uy, pl, umode)) return e;
+  const Geo geo = make_geo(*g);
+  CLBM_STORAGE_SWITCH(g->storage, (launch_marker<SS, true, false, IBM_SPREAD_FIXED>(
+                                      geo, ip, pl, f, umode, ux, uy, n, X, Y, Z, VX, VY, U, V, FX,
+                                      FY, nullptr, nullptr, stencil_base, nullptr, nullptr,
+                                      (hipStream_t)stream)));
+  return check_launch("ibm_interp_force");
 }
 
 int ibm_spread(const clbm_grid_t* g, const ibm_params_t* ip, int n, const float* X, const float* Y,
@@ -524,49 +819,35 @@ int ibm_spread(const clbm_grid_t* g, const ibm_params_t* ip, int n, const float*
   if (int e = check_ibm(ip)) return e;
   if (n < 0) return fail(-1, "ibm_spread: n < 0");
   if (n == 0) return 0;
-  if (!X || !Y || !Z || !FX || !FY || !acc_x || !acc_y)
-    return fail(-1, "ibm_spread: NULL buffer");
+  if (!X || !Y || !Z || !FX || !FY || !acc_x || !acc_y) return fail(-1, "ibm_spread: NULL buffer");
+  const Geo geo = make_geo(*g);
+  const Planes pl{0, 0.0, 0.0};
+  hipStream_t st = (hipStream_t)stream;
   if (ip->spread_mode == IBM_SPREAD_FIXED)
-    return marker_dispatch<false, true, IBM_SPREAD_FIXED>(
-        g, ip, nullptr, 0, nullptr, nullptr, n, X, Y, Z, nullptr, nullptr, nullptr, nullptr,
-        nullptr, nullptr, FX, FY, nullptr, acc_x, acc_y, (hipStream_t)stream);
-  return marker_dispatch<false, true, IBM_SPREAD_ATOMIC_F32>(
-      g, ip, nullptr, 0, nullptr, nullptr, n, X, Y, Z, nullptr, nullptr, nullptr, nullptr, nullptr,
-      nullptr, FX, FY, nullptr, acc_x, acc_y, (hipStream_t)stream);
-}
-
-int ibm_interp_spread(const clbm_grid_t* g, const ibm_params_t* ip, const void* f, int parity,
-                      const float* ux, const float* uy, int n, const float* X, const float* Y,
-                      const float* Z, const float* VX, const float* VY, float* U, float* V,
-                      float* FX, float* FY, int32_t* stencil_base, void* acc_x, void* acc_y,
-                      void* stream) {
-  if (int e = interp_args(g, ip, f, parity, ux, uy, n, X, Y, Z, VX, VY)) return e;
-  if (n == 0) return 0;
-  if (!acc_x || !acc_y) return fail(-1, "ibm_interp_spread: NULL accumulator");
-  if (ip->spread_mode == IBM_SPREAD_FIXED)
-    return marker_dispatch<true, true, IBM_SPREAD_FIXED>(g, ip, f, parity, ux, uy, n, X, Y, Z, VX,
-                                                          VY, U, V, FX, FY, nullptr, nullptr,
-                                                          stencil_base, acc_x, acc_y,
-                                                          (hipStream_t)stream);
-  return marker_dispatch<true, true, IBM_SPREAD_ATOMIC_F32>(g, ip, f, parity, ux, uy, n, X, Y, Z,
-                                                             VX, VY, U, V, FX, FY, nullptr, nullptr,
-                                                             stencil_base, acc_x, acc_y,
-                                                             (hipStream_t)stream);
+    launch_marker<CLBM_STORE_F32, false, true, IBM_SPREAD_FIXED>(
+        geo, ip, pl, nullptr, 0, nullptr, nullptr, n, X, Y, Z, nullptr, nullptr, nullptr, nullptr,
+        nullptr, nullptr, FX, FY, nullptr, acc_x, acc_y, st);
+  else
+    launch_marker<CLBM_STORE_F32, false, true, IBM_SPREAD_ATOMIC_F32>(
+        geo, ip, pl, nullptr, 0, nullptr, nullptr, n, X, Y, Z, nullptr, nullptr, nullptr, nullptr,
+        nullptr, nullptr, FX, FY, nullptr, acc_x, acc_y, st);
+  return check_launch("ibm_spread");
 }
 
 int ibm_clip_forces(const clbm_grid_t* g, const ibm_params_t* ip, void* acc_x, void* acc_y,
-                    float* fx, float* fy, const int32_t* box, void* stream) {
+                    float* fx, float* fy, uint64_t* box, uint32_t gen, void* stream) {
   if (int e = check_grid(g, false)) return e;
   if (int e = check_ibm(ip)) return e;
   if (!acc_x || !acc_y || !box) return fail(-1, "ibm_clip_forces: NULL buffer");
+  if (gen == 0) return fail(-1, "ibm_clip_forces: generation must be >= 1");
   const Geo geo = make_geo(*g);
   const float inv = (float)std::ldexp(1.0, -fixed_shift(ip));
   if (ip->spread_mode == IBM_SPREAD_FIXED)
     hipLaunchKernelGGL(k_clip<IBM_SPREAD_FIXED>, dim3(box_blocks(geo)), dim3(IBLK), 0,
-                       (hipStream_t)stream, geo, ip->f_max, inv, acc_x, acc_y, fx, fy, box);
+                       (hipStream_t)stream, geo, ip->f_max, inv, acc_x, acc_y, fx, fy, box, gen);
   else
     hipLaunchKernelGGL(k_clip<IBM_SPREAD_ATOMIC_F32>, dim3(box_blocks(geo)), dim3(IBLK), 0,
-                       (hipStream_t)stream, geo, ip->f_max, inv, acc_x, acc_y, fx, fy, box);
+                       (hipStream_t)stream, geo, ip->f_max, inv, acc_x, acc_y, fx, fy, box, gen);
   return check_launch("ibm_clip_forces");
 }
 

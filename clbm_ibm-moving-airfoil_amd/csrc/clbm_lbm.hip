@@ -1,23 +1,23 @@
 // Lattice-Boltzmann kernels of the CLBM-IBM step for gfx950 (MI355X).
 //
-// One fused pass per step replaces K2 _macro_with_force, K3 _collide and
-// K4 _stream of CLBMSolver.step (clbm_moving.py:326-343): each thread loads
-// the 19 post-stream populations of its cell, applies the force, collides
-// and writes the post-collision values where the next step will read them as
-// post-stream values (AA pattern, in place, one population buffer).  A small
-// second launch applies K5 _bc_inlet and K6 _bc_outlet (clbm_moving.py:
-// 266-300) on the two x planes, so the stored state between steps is always
-// the reference's post-BC state.
+// One fused pass per step replaces all of CLBMSolver.step (clbm_moving.py:
+// 326-343): each thread loads the 19 post-stream populations of its cell,
+// applies the reference's boundary planes on load (inlet rule at x = 0,
+// outlet copy at x = Nx-1, left pending by the previous step), adds the
+// force, collides and writes the post-collision values where the next step
+// reads them as post-stream values (AA pattern: in place, one buffer).  The
+// macroscopic fields (K2, K7) live in registers; K7 is materialised only when
+// someone reads rho/u.
 #include "clbm_collide.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
-#include <mutex>
 
 namespace clbm {
 
 // ---------------------------------------------------------------------------
-// errors
+// errors and argument checks
 // ---------------------------------------------------------------------------
 static thread_local std::string g_last_error;
 void set_error(const std::string& s) { g_last_error = s; }
@@ -37,6 +37,19 @@ int check_grid(const clbm_grid_t* g, bool need_inlet_width) {
   long long nc = (long long)g->nx * g->ny * g->nz;
   if (nc >= (1ll << 31)) return fail(-1, "grid has 2^31 cells or more");
   if (elem_bytes(g->storage) == 0) return fail(-1, "unknown storage format");
+  return 0;
+}
+int check_state(const clbm_grid_t* g, const clbm_params_t* p, int state, Planes& pl) {
+  if (state & ~(CLBM_STATE_PARITY | CLBM_STATE_PLANES))
+    return fail(-1, "state has unknown bits (expected parity | planes flag)");
+  pl.on = (state & CLBM_STATE_PLANES) ? 1 : 0;
+  pl.ux_in = pl.uy_in = 0.0;
+  if (pl.on) {
+    if (!p) return fail(-1, "a state with pending boundary planes needs the params (inlet velocity)");
+    if (g->nx < 3) return fail(-1, "the inlet/outlet rule needs nx >= 3");
+    pl.ux_in = p->ux_in;
+    pl.uy_in = p->uy_in;
+  }
   return 0;
 }
 
@@ -64,11 +77,29 @@ __global__ __launch_bounds__(BLK) void k_init_feq(Geo g, typename Store<S>::T* _
 }
 
 // ---------------------------------------------------------------------------
-// Fused macro + collide + Guo + stream (AA pattern).
-// PAR = parity of the incoming state.  BOX: force reads limited to a box.
+// Fused boundary planes + macro + collide + Guo + stream (AA pattern).
+// PAR: parity of the incoming state.  FORCE: force fields present.  BOX: the
+// force is read only inside box[0..5].
 // ---------------------------------------------------------------------------
+template <typename R, bool FORCE, bool BOX>
+__device__ __forceinline__ void force_at(int c, int x, int y, int z, const float* __restrict__ fx,
+                                         const float* __restrict__ fy,
+                                         const float* __restrict__ fz,
+                                         const int32_t* __restrict__ box, R& Fx, R& Fy, R& Fz) {
+  Fx = Fy = Fz = R(0);
+  if (!FORCE) return;
+  if (BOX) {
+    const bool in = (x >= box[0]) & (y >= box[1]) & (z >= box[2]) & (x <= box[3]) &
+                    (y <= box[4]) & (z <= box[5]);
+    if (!in) return;
+  }
+  if (fx) Fx = R(fx[c]);
+  if (fy) Fy = R(fy[c]);
+  if (fz) Fz = R(fz[c]);
+}
+
 template <int S, int PAR, bool FORCE, bool BOX>
-__global__ __launch_bounds__(BLK) void k_step(Geo g, double s_v, double s_e,
+__global__ __launch_bounds__(BLK) void k_step(Geo g, double s_v, double s_e, Planes pl,
                                               typename Store<S>::T* __restrict__ f,
                                               const float* __restrict__ fx,
                                               const float* __restrict__ fy,
@@ -80,79 +111,29 @@ __global__ __launch_bounds__(BLK) void k_step(Geo g, double s_v, double s_e,
   if (c >= g.ncell) return;
   int x, y, z;
   cell_xyz(g, c, x, y, z);
-  const Nbr nb = make_nbr(g, x, y, z);
   const size_t N = (size_t)g.ncell;
-
-  R fq[Q];
-  load_state<S, PAR>(f, N, c, nb, fq);
-
-  R Fx = R(0), Fy = R(0), Fz = R(0);
-  if (FORCE) {
-    bool in = true;
-    if (BOX) {
-      in = (x >= box[0]) & (y >= box[1]) & (z >= box[2]) & (x <= box[3]) & (y <= box[4]) &
-           (z <= box[5]);
-    }
-    if (in) {
-      if (fx) Fx = R(fx[c]);
-      if (fy) Fy = R(fy[c]);
-      if (fz) Fz = R(fz[c]);
-    }
+  // With pending planes the cell at x = nx-1 holds the outlet copy of cell
+  // nx-2's post-stream populations.  Both cells' slots are handled by the
+  // thread of x = nx-2 (the x = nx-1 thread would race with it): it collides
+  // that state with the force of x = nx-1 and stores it in cell nx-1's slots,
+  // then reloads its own (still untouched) slots for its own cell.
+  if (pl.on && x == g.nx - 1) return;
+  R fq[Q], Fx, Fy, Fz;
+  const Nbr nb = make_nbr(g, x, y, z);
+  if (pl.on && x == g.nx - 2) {
+    load_state<S, PAR>(f, N, c, nb, fq);
+    force_at<R, FORCE, BOX>(c + 1, x + 1, y, z, fx, fy, fz, box, Fx, Fy, Fz);
+    collide_cell<R>(fq, Fx, Fy, Fz, R(s_v), R(s_e));
+    const Nbr n1 = make_nbr(g, x + 1, y, z);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) Store<S>::st(f + slot_out<PAR>(q, N, c + 1, n1), q, fq[q]);
   }
+  load_state<S, PAR>(f, N, c, nb, fq);
+  if (pl.on && x == 0) inlet_rule<R>(fq, R(pl.ux_in), R(pl.uy_in));
+  force_at<R, FORCE, BOX>(c, x, y, z, fx, fy, fz, box, Fx, Fy, Fz);
   collide_cell<R>(fq, Fx, Fy, Fz, R(s_v), R(s_e));
-
 #pragma unroll
   for (int q = 0; q < Q; ++q) Store<S>::st(f + slot_out<PAR>(q, N, c, nb), q, fq[q]);
-}
-
-// ---------------------------------------------------------------------------
-// K5 inlet rule + K6 outlet copy on the x planes of a state of parity PAR.
-// Threads [0, ny*nz) apply the inlet rule to cell (0,y,z); threads
-// [ny*nz, 2*ny*nz) copy cell (nx-2,y,z) to (nx-1,y,z).  The inlet touches
-// only cell (0,y,z)'s own slots, the outlet reads cell (nx-2,y,z)'s and
-// writes cell (nx-1,y,z)'s, so no slot is shared between threads.  Every
-// thread loads all it needs before it stores anything.
-// ---------------------------------------------------------------------------
-constexpr int BCBLK = 64;
-template <int S, int PAR>
-__global__ __launch_bounds__(BCBLK) void k_bc(Geo g, double ux_in_f, double uy_in_f,
-                                              typename Store<S>::T* __restrict__ f) {
-  using R = typename Store<S>::R;
-  using T = typename Store<S>::T;
-  const int nyz = g.ny * g.nz;
-  int t = blockIdx.x * BCBLK + threadIdx.x;
-  if (t >= 2 * nyz) return;
-  const bool inlet = t < nyz;
-  if (!inlet) t -= nyz;
-  const int z = t / g.ny, y = t - z * g.ny;
-  const size_t N = (size_t)g.ncell;
-  if (inlet) {  // clbm_moving.py:272-294
-    const R ux_in = R(ux_in_f), uy_in = R(uy_in_f);
-    const int c = (z * g.ny + y) * g.nx;
-    const Nbr nb = make_nbr(g, 0, y, z);
-    R a[Q];
-    load_state<S, PAR>(f, N, c, nb, a);
-    const R s_known = a[0] + a[3] + a[4] + a[5] + a[6] + a[15] + a[16] + a[17] + a[18];
-    const R s_out = a[2] + a[8] + a[10] + a[12] + a[14];
-    const R rho_in = (s_known + R(2) * s_out) / (R(1) - ux_in);
-    const R dy = a[3] - a[4], dz = a[5] - a[6];
-    const R e = (R(1) / R(6)) * rho_in * ux_in;
-    Store<S>::st(f + slot_in<PAR>(1, N, c, nb), 1, a[2] + (R(2) / R(3)) * rho_in * ux_in);
-    Store<S>::st(f + slot_in<PAR>(7, N, c, nb), 7, a[10] + e + R(0.5) * rho_in * uy_in - R(0.5) * dy);
-    Store<S>::st(f + slot_in<PAR>(9, N, c, nb), 9, a[8] + e - R(0.5) * rho_in * uy_in + R(0.5) * dy);
-    Store<S>::st(f + slot_in<PAR>(11, N, c, nb), 11, a[14] + e - R(0.5) * dz);
-    Store<S>::st(f + slot_in<PAR>(13, N, c, nb), 13, a[12] + e + R(0.5) * dz);
-  } else {  // clbm_moving.py:296-300
-    const int cs = (z * g.ny + y) * g.nx + (g.nx - 2);
-    const int cd = cs + 1;
-    const Nbr ns = make_nbr(g, g.nx - 2, y, z);
-    const Nbr nd = make_nbr(g, g.nx - 1, y, z);
-    T v[Q];
-#pragma unroll
-    for (int q = 0; q < Q; ++q) v[q] = f[slot_in<PAR>(q, N, cs, ns)];
-#pragma unroll
-    for (int q = 0; q < Q; ++q) f[slot_in<PAR>(q, N, cd, nd)] = v[q];
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -174,7 +155,7 @@ __device__ __forceinline__ void moments01(const R* a, R& r, R& mx, R& my, R& mz)
 }
 
 template <int S, int PAR>
-__global__ __launch_bounds__(BLK) void k_macro(Geo g, const typename Store<S>::T* __restrict__ f,
+__global__ __launch_bounds__(BLK) void k_macro(Geo g, Planes pl, const typename Store<S>::T* __restrict__ f,
                                                float* __restrict__ rho, float* __restrict__ ux,
                                                float* __restrict__ uy, float* __restrict__ uz) {
   using R = typename Store<S>::R;
@@ -182,9 +163,8 @@ __global__ __launch_bounds__(BLK) void k_macro(Geo g, const typename Store<S>::T
   if (c >= g.ncell) return;
   int x, y, z;
   cell_xyz(g, c, x, y, z);
-  const Nbr nb = make_nbr(g, x, y, z);
   R a[Q];
-  load_state<S, PAR>(f, (size_t)g.ncell, c, nb, a);
+  load_cell<S, PAR>(f, g, c, x, y, z, pl, a);
   R r, mx, my, mz;
   moments01(a, r, mx, my, mz);
   if (rho) rho[c] = (float)r;
@@ -194,16 +174,16 @@ __global__ __launch_bounds__(BLK) void k_macro(Geo g, const typename Store<S>::T
 }
 
 template <int S, int PAR, typename OutT>
-__global__ __launch_bounds__(BLK) void k_export(Geo g, const typename Store<S>::T* __restrict__ f,
+__global__ __launch_bounds__(BLK) void k_export(Geo g, Planes pl,
+                                                const typename Store<S>::T* __restrict__ f,
                                                 OutT* __restrict__ out) {
   using R = typename Store<S>::R;
   const int c = blockIdx.x * BLK + threadIdx.x;
   if (c >= g.ncell) return;
   int x, y, z;
   cell_xyz(g, c, x, y, z);
-  const Nbr nb = make_nbr(g, x, y, z);
   R a[Q];
-  load_state<S, PAR>(f, (size_t)g.ncell, c, nb, a);
+  load_cell<S, PAR>(f, g, c, x, y, z, pl, a);
 #pragma unroll
   for (int q = 0; q < Q; ++q) out[(size_t)q * g.ncell + c] = (OutT)a[q];
 }
@@ -235,25 +215,28 @@ __global__ void k_health_init(int* acc) {
 }
 
 template <int S, int PAR>
-__global__ __launch_bounds__(BLK) void k_health(Geo g, const typename Store<S>::T* __restrict__ f,
+__global__ __launch_bounds__(BLK) void k_health(Geo g, Planes pl,
+                                                const typename Store<S>::T* __restrict__ f,
                                                 int* __restrict__ acc) {
   using R = typename Store<S>::R;
   __shared__ int smin[BLK / 64], sfin[BLK / 64];
-  const int c = blockIdx.x * BLK + threadIdx.x;
   int key = 0x7f800000, fin = 1;
-  if (c < g.ncell) {
+  // grid-stride: a few hundred blocks, so that the per-block atomics on the
+  // two result words (which serialise) stay few
+  for (int c = blockIdx.x * BLK + threadIdx.x; c < g.ncell; c += gridDim.x * BLK) {
     int x, y, z;
     cell_xyz(g, c, x, y, z);
-    const Nbr nb = make_nbr(g, x, y, z);
     R a[Q];
-    load_state<S, PAR>(f, (size_t)g.ncell, c, nb, a);
+    load_cell<S, PAR>(f, g, c, x, y, z, pl, a);
     R r = R(0);
+    int ok = 1;
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       r += a[q];
-      fin &= (int)isfinite(a[q]);
+      ok &= (int)isfinite(a[q]);
     }
-    if (fin) key = fkey((float)r);
+    fin &= ok;
+    if (ok) key = min(key, fkey((float)r));
   }
   for (int o = 32; o > 0; o >>= 1) {
     key = min(key, __shfl_xor(key, o));
@@ -307,65 +290,56 @@ template <int S> struct InitRun {
 
 template <int S> struct StepRun {
   template <int PAR, bool FORCE, bool BOX>
-  static void launch(const Geo& g, const clbm_params_t* p, void* f, const float* fx,
-                     const float* fy, const float* fz, const int32_t* box, hipStream_t st) {
+  static void launch(const Geo& g, const clbm_params_t* p, const Planes& pl, void* f,
+                     const float* fx, const float* fy, const float* fz, const int32_t* box,
+                     hipStream_t st) {
     hipLaunchKernelGGL((k_step<S, PAR, FORCE, BOX>), dim3(nblocks(g.ncell)), dim3(BLK), 0, st, g,
-                       p->s_v, p->s_e, (typename Store<S>::T*)f, fx, fy, fz, box);
+                       p->s_v, p->s_e, pl, (typename Store<S>::T*)f, fx, fy, fz, box);
   }
   template <int PAR>
-  static void launch_par(const Geo& g, const clbm_params_t* p, void* f, const float* fx,
-                         const float* fy, const float* fz, const int32_t* box, hipStream_t st) {
+  static void launch_par(const Geo& g, const clbm_params_t* p, const Planes& pl, void* f,
+                         const float* fx, const float* fy, const float* fz, const int32_t* box,
+                         hipStream_t st) {
     const bool force = fx || fy || fz;
     if (!force)
-      launch<PAR, false, false>(g, p, f, fx, fy, fz, box, st);
+      launch<PAR, false, false>(g, p, pl, f, fx, fy, fz, box, st);
     else if (box)
-      launch<PAR, true, true>(g, p, f, fx, fy, fz, box, st);
+      launch<PAR, true, true>(g, p, pl, f, fx, fy, fz, box, st);
     else
-      launch<PAR, true, false>(g, p, f, fx, fy, fz, box, st);
+      launch<PAR, true, false>(g, p, pl, f, fx, fy, fz, box, st);
   }
-  static int run(const Geo& g, const clbm_params_t* p, void* f, int parity, const float* fx,
-                 const float* fy, const float* fz, const int32_t* box, hipStream_t st) {
+  static int run(const Geo& g, const clbm_params_t* p, const Planes& pl, void* f, int parity,
+                 const float* fx, const float* fy, const float* fz, const int32_t* box,
+                 hipStream_t st) {
     if (parity == 0)
-      launch_par<0>(g, p, f, fx, fy, fz, box, st);
+      launch_par<0>(g, p, pl, f, fx, fy, fz, box, st);
     else
-      launch_par<1>(g, p, f, fx, fy, fz, box, st);
-    int e = check_launch("clbm_step");
-    if (e) return e;
-    if (p->inlet) {
-      const int nbc = (int)((2ll * g.ny * g.nz + BCBLK - 1) / BCBLK);
-      // the state after the step has the opposite parity
-      if (parity == 0)
-        hipLaunchKernelGGL((k_bc<S, 1>), dim3(nbc), dim3(BCBLK), 0, st, g, p->ux_in, p->uy_in,
-                           (typename Store<S>::T*)f);
-      else
-        hipLaunchKernelGGL((k_bc<S, 0>), dim3(nbc), dim3(BCBLK), 0, st, g, p->ux_in, p->uy_in,
-                           (typename Store<S>::T*)f);
-      return check_launch("clbm_step (boundary planes)");
-    }
-    return 0;
+      launch_par<1>(g, p, pl, f, fx, fy, fz, box, st);
+    return check_launch("clbm_step");
   }
 };
 
 template <int S> struct MacroRun {
-  static int run(const Geo& g, const void* f, int parity, float* r, float* u, float* v, float* w,
-                 hipStream_t st) {
+  static int run(const Geo& g, const Planes& pl, const void* f, int parity, float* r, float* u,
+                 float* v, float* w, hipStream_t st) {
     auto fp = (const typename Store<S>::T*)f;
     if (parity == 0)
-      hipLaunchKernelGGL((k_macro<S, 0>), dim3(nblocks(g.ncell)), dim3(BLK), 0, st, g, fp, r, u, v, w);
+      hipLaunchKernelGGL((k_macro<S, 0>), dim3(nblocks(g.ncell)), dim3(BLK), 0, st, g, pl, fp, r, u, v, w);
     else
-      hipLaunchKernelGGL((k_macro<S, 1>), dim3(nblocks(g.ncell)), dim3(BLK), 0, st, g, fp, r, u, v, w);
+      hipLaunchKernelGGL((k_macro<S, 1>), dim3(nblocks(g.ncell)), dim3(BLK), 0, st, g, pl, fp, r, u, v, w);
     return check_launch("clbm_macro");
   }
 };
 
 template <int S> struct ExportRun {
   template <typename OutT>
-  static int run(const Geo& g, const void* f, int parity, OutT* out, hipStream_t st) {
+  static int run(const Geo& g, const Planes& pl, const void* f, int parity, OutT* out,
+                 hipStream_t st) {
     auto fp = (const typename Store<S>::T*)f;
     if (parity == 0)
-      hipLaunchKernelGGL((k_export<S, 0, OutT>), dim3(nblocks(g.ncell)), dim3(BLK), 0, st, g, fp, out);
+      hipLaunchKernelGGL((k_export<S, 0, OutT>), dim3(nblocks(g.ncell)), dim3(BLK), 0, st, g, pl, fp, out);
     else
-      hipLaunchKernelGGL((k_export<S, 1, OutT>), dim3(nblocks(g.ncell)), dim3(BLK), 0, st, g, fp, out);
+      hipLaunchKernelGGL((k_export<S, 1, OutT>), dim3(nblocks(g.ncell)), dim3(BLK), 0, st, g, pl, fp, out);
     return check_launch("clbm_export_state");
   }
 };
@@ -380,15 +354,16 @@ template <int S> struct ImportRun {
 };
 
 template <int S> struct HealthRun {
-  static int run(const Geo& g, const void* f, int parity, float* out2, hipStream_t st) {
+  static int run(const Geo& g, const Planes& pl, const void* f, int parity, float* out2,
+                 hipStream_t st) {
     // out2 is used as int scratch before the final conversion
     int* acc = (int*)out2;
     hipLaunchKernelGGL(k_health_init, dim3(1), dim3(1), 0, st, acc);
     auto fp = (const typename Store<S>::T*)f;
     if (parity == 0)
-      hipLaunchKernelGGL((k_health<S, 0>), dim3(nblocks(g.ncell)), dim3(BLK), 0, st, g, fp, acc);
+      hipLaunchKernelGGL((k_health<S, 0>), dim3(std::min(nblocks(g.ncell), 512)), dim3(BLK), 0, st, g, pl, fp, acc);
     else
-      hipLaunchKernelGGL((k_health<S, 1>), dim3(nblocks(g.ncell)), dim3(BLK), 0, st, g, fp, acc);
+      hipLaunchKernelGGL((k_health<S, 1>), dim3(std::min(nblocks(g.ncell), 512)), dim3(BLK), 0, st, g, pl, fp, acc);
     hipLaunchKernelGGL(k_health_fin, dim3(1), dim3(1), 0, st, acc, out2);
     return check_launch("clbm_health");
   }
@@ -409,6 +384,10 @@ size_t clbm_state_bytes(const clbm_grid_t* g) {
   return (size_t)Q * (size_t)g->nx * g->ny * g->nz * elem_bytes(g->storage);
 }
 
+int clbm_state_after_step(int state, const clbm_params_t* p) {
+  return ((state & CLBM_STATE_PARITY) ^ CLBM_STATE_PARITY) | ((p && p->inlet) ? CLBM_STATE_PLANES : 0);
+}
+
 int clbm_init_feq(const clbm_grid_t* g, void* f, double rho0, double ux0, double uy0, double uz0,
                   void* stream) {
   if (int e = check_grid(g, false)) return e;
@@ -417,39 +396,46 @@ int clbm_init_feq(const clbm_grid_t* g, void* f, double rho0, double ux0, double
                                    (hipStream_t)stream);
 }
 
-int clbm_step(const clbm_grid_t* g, const clbm_params_t* p, void* f, int parity, const float* fx,
+int clbm_step(const clbm_grid_t* g, const clbm_params_t* p, void* f, int state, const float* fx,
               const float* fy, const float* fz, const int32_t* force_box, void* stream) {
   if (!p) return fail(-1, "clbm_step: params is NULL");
   if (int e = check_grid(g, p->inlet != 0)) return e;
   if (!f) return fail(-1, "clbm_step: f is NULL");
-  if (parity != 0 && parity != 1) return fail(-1, "clbm_step: parity must be 0 or 1");
+  Planes pl;
+  if (int e = check_state(g, p, state, pl)) return e;
   if (p->inlet && !(p->ux_in < 1.0)) return fail(-1, "clbm_step: inlet needs ux_in < 1");
-  return dispatch_storage<StepRun>(g->storage, make_geo(*g), p, f, parity, fx, fy, fz, force_box,
-                                   (hipStream_t)stream);
+  return dispatch_storage<StepRun>(g->storage, make_geo(*g), p, pl, f, state & CLBM_STATE_PARITY,
+                                   fx, fy, fz, force_box, (hipStream_t)stream);
 }
 
-int clbm_macro(const clbm_grid_t* g, const void* f, int parity, float* rho, float* ux, float* uy,
-               float* uz, void* stream) {
+int clbm_macro(const clbm_grid_t* g, const clbm_params_t* p, const void* f, int state, float* rho,
+               float* ux, float* uy, float* uz, void* stream) {
   if (int e = check_grid(g, false)) return e;
   if (!f) return fail(-1, "clbm_macro: f is NULL");
-  if (parity != 0 && parity != 1) return fail(-1, "clbm_macro: parity must be 0 or 1");
-  return dispatch_storage<MacroRun>(g->storage, make_geo(*g), f, parity, rho, ux, uy, uz,
-                                    (hipStream_t)stream);
+  Planes pl;
+  if (int e = check_state(g, p, state, pl)) return e;
+  return dispatch_storage<MacroRun>(g->storage, make_geo(*g), pl, f, state & CLBM_STATE_PARITY, rho,
+                                    ux, uy, uz, (hipStream_t)stream);
 }
 
-int clbm_export_state(const clbm_grid_t* g, const void* f, int parity, float* out, void* stream) {
+int clbm_export_state(const clbm_grid_t* g, const clbm_params_t* p, const void* f, int state,
+                      float* out, void* stream) {
   if (int e = check_grid(g, false)) return e;
   if (!f || !out) return fail(-1, "clbm_export_state: NULL buffer");
-  if (parity != 0 && parity != 1) return fail(-1, "clbm_export_state: parity must be 0 or 1");
-  return dispatch_storage<ExportRun>(g->storage, make_geo(*g), f, parity, out, (hipStream_t)stream);
+  Planes pl;
+  if (int e = check_state(g, p, state, pl)) return e;
+  return dispatch_storage<ExportRun>(g->storage, make_geo(*g), pl, f, state & CLBM_STATE_PARITY, out,
+                                     (hipStream_t)stream);
 }
 
-int clbm_export_state_f64(const clbm_grid_t* g, const void* f, int parity, double* out,
-                          void* stream) {
+int clbm_export_state_f64(const clbm_grid_t* g, const clbm_params_t* p, const void* f, int state,
+                          double* out, void* stream) {
   if (int e = check_grid(g, false)) return e;
   if (!f || !out) return fail(-1, "clbm_export_state_f64: NULL buffer");
-  if (parity != 0 && parity != 1) return fail(-1, "clbm_export_state_f64: parity must be 0 or 1");
-  return dispatch_storage<ExportRun>(g->storage, make_geo(*g), f, parity, out, (hipStream_t)stream);
+  Planes pl;
+  if (int e = check_state(g, p, state, pl)) return e;
+  return dispatch_storage<ExportRun>(g->storage, make_geo(*g), pl, f, state & CLBM_STATE_PARITY, out,
+                                     (hipStream_t)stream);
 }
 
 int clbm_import_state(const clbm_grid_t* g, void* f, const float* in, void* stream) {
@@ -464,11 +450,14 @@ int clbm_import_state_f64(const clbm_grid_t* g, void* f, const double* in, void*
   return dispatch_storage<ImportRun>(g->storage, make_geo(*g), f, in, (hipStream_t)stream);
 }
 
-int clbm_health(const clbm_grid_t* g, const void* f, int parity, float* out2, void* stream) {
+int clbm_health(const clbm_grid_t* g, const clbm_params_t* p, const void* f, int state,
+                float* out2, void* stream) {
   if (int e = check_grid(g, false)) return e;
   if (!f || !out2) return fail(-1, "clbm_health: NULL buffer");
-  if (parity != 0 && parity != 1) return fail(-1, "clbm_health: parity must be 0 or 1");
-  return dispatch_storage<HealthRun>(g->storage, make_geo(*g), f, parity, out2, (hipStream_t)stream);
+  Planes pl;
+  if (int e = check_state(g, p, state, pl)) return e;
+  return dispatch_storage<HealthRun>(g->storage, make_geo(*g), pl, f, state & CLBM_STATE_PARITY,
+                                     out2, (hipStream_t)stream);
 }
 
 }  // extern "C"

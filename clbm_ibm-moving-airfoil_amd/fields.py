@@ -28,7 +28,7 @@ class Field:
         self._t = tensor
         self._refresh = refresh
         self.owner = owner      # the object whose state this field shows
-        self.box = box          # int32 force-box tensor when the field is a spread force
+        self.box = box          # device address of the int32[6] force box of a spread force
 
     @property
     def tensor(self):

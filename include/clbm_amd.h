@@ -14,16 +14,27 @@
  * Memory layout (the library's own, not the reference's):
  *   populations  structure of arrays, f[q][z][y][x], x fastest, 19 planes of
  *                Nx*Ny*Nz elements of the storage type; updated in place
- *                (AA pattern).  `parity` = number of steps taken mod 2:
+ *                (AA pattern).  A state word describes the buffer:
+ *                bit 0 (CLBM_STATE_PARITY) = number of steps taken mod 2;
  *                parity 0: post-stream population q of cell c sits at f[q][c];
  *                parity 1: it sits at f[opp(q)][c - e_q] (periodic index).
+ *                bit 1 (CLBM_STATE_PLANES): the x planes still hold raw streamed
+ *                values; every reader applies the inlet rule at x = 0 and takes
+ *                cell Nx-2's populations for x = Nx-1 as it loads (this is how
+ *                K5/K6 run without a pass of their own).  Readers of such a
+ *                state need the params (inlet velocity).
  *                16-bit storage holds f_q - w_q (deviation from the lattice weight).
  *   scalar fields  [z][y][x] float32, x fastest.
- *   force box    int32[IBM_BOX_INTS]: [0..5] = inclusive box (x0,y0,z0,x1,y1,z1)
- *                of the cells the current markers' stencils reach, [6..11] = the
- *                box before it, [13] = a counter that must start at 0, the rest
- *                scratch of ibm_update_markers.  Initialise [0..11] to empty
- *                boxes (lower corner INT32_MAX, upper corner INT32_MIN).
+ *   force box    uint64[IBM_BOX_WORDS], zero-initialised, owned by the caller.
+ *                Every IBM call carries a generation number gen >= 1 (one per
+ *                compute_force, increasing).  The words of parity gen&1
+ *                collect the bounding box of the cells that generation's
+ *                stencils reach as atomic maxima of (gen << 32 | coordinate
+ *                key), spread over IBM_BOX_SUBSLOTS copies, so a slot needs no
+ *                reset and a stale one is recognised by its gen.
+ *                ibm_clip_forces decodes the current box into the int32[6]
+ *                (x0,y0,z0,x1,y1,z1) at IBM_FORCE_BOX(box), the form clbm_step
+ *                reads as its force_box.
  *
  * Reference interfaces are cited as file:line of kioxli/CLBM_IBM-moving-airfoil.
  */
@@ -49,9 +60,13 @@ extern "C" {
 #define IBM_SPREAD_FIXED 0 /* order-independent 64-bit fixed-point sums (deterministic) */
 #define IBM_SPREAD_ATOMIC_F32 1 /* float32 atomics, as ti.atomic_add (order-dependent) */
 
-#define IBM_BOX_HEADER 16
-#define IBM_BOX_PARTIALS 64
-#define IBM_BOX_INTS (IBM_BOX_HEADER + 6 * IBM_BOX_PARTIALS)
+#define CLBM_STATE_PARITY 1
+#define CLBM_STATE_PLANES 2
+
+#define IBM_BOX_SUBSLOTS 64
+#define IBM_BOX_WORDS (2 * 6 * IBM_BOX_SUBSLOTS + 4)
+/* the decoded int32[6] box inside the uint64 box record */
+#define IBM_FORCE_BOX(box) ((int32_t*)((box) + 2 * 6 * IBM_BOX_SUBSLOTS))
 
 typedef struct {
   int32_t nx, ny, nz; /* lattice size, Nx >= 3 */
@@ -75,6 +90,9 @@ typedef struct {
                                 airfoil_moving.py:159-165 does; 0: unwrapped offsets */
   int32_t spread_mode;       /* IBM_SPREAD_* */
   int32_t fixed_shift;       /* 0 = automatic; else scale 2^shift of the fixed-point sums */
+  int32_t group;             /* markers per workgroup of ibm_compute_force: 4, 8 or 16
+                                (0 = 8); a tuning choice, results do not depend on it */
+  int32_t reserved;
 } ibm_params_t;
 
 /* ---- library ---------------------------------------------------------- */
@@ -83,95 +101,110 @@ const char* clbm_last_error(void);
 /* Bytes of one population buffer for `g` (19 * Nx*Ny*Nz * element size). */
 size_t clbm_state_bytes(const clbm_grid_t* g);
 
+/* State word after one clbm_step from `state` with params `p`. */
+int clbm_state_after_step(int state, const clbm_params_t* p);
+
 /* ---- lattice Boltzmann (replaces CLBMSolver, clbm_moving.py:41-343) ----- */
 
 /* Replaces CLBMSolver._init_feq (clbm_moving.py:75-91): second-order
- * equilibrium at (rho0, u0) in every cell; leaves a parity-0 state. */
+ * equilibrium at (rho0, u0) in every cell; leaves state 0. */
 int clbm_init_feq(const clbm_grid_t* g, void* f, double rho0, double ux0, double uy0, double uz0,
                   void* stream);
 
 /* Replaces CLBMSolver.step (clbm_moving.py:326-343): K2 _macro_with_force,
- * K3 _collide, K4 _stream in one fused pass, then K5 _bc_inlet and
- * K6 _bc_outlet on the two x planes.  fx/fy/fz are [z][y][x] float32 force
- * fields (each nullable = zero).  force_box (nullable) restricts the force
- * reads to the box in its first six ints; cells outside read zero.
- * `parity` is the state's parity before the step; afterwards it is 1-parity. */
-int clbm_step(const clbm_grid_t* g, const clbm_params_t* p, void* f, int parity,
+ * K3 _collide, K4 _stream, K5 _bc_inlet and K6 _bc_outlet in one pass (the
+ * planes of this step are left pending and applied by the next reader).
+ * fx/fy/fz are [z][y][x] float32 force fields (each nullable = zero).
+ * force_box (nullable) restricts the force reads to the box in its first six
+ * ints; cells outside read zero.  `state` describes f before the step; after
+ * it, f is in state clbm_state_after_step(state, p). */
+int clbm_step(const clbm_grid_t* g, const clbm_params_t* p, void* f, int state,
               const float* fx, const float* fy, const float* fz, const int32_t* force_box,
               void* stream);
 
 /* Replaces CLBMSolver._macro_post (clbm_moving.py:305-321): force-free rho
  * and u of the current state into [z][y][x] float32 fields (each nullable). */
-int clbm_macro(const clbm_grid_t* g, const void* f, int parity, float* rho, float* ux, float* uy,
-               float* uz, void* stream);
+int clbm_macro(const clbm_grid_t* g, const clbm_params_t* p, const void* f, int state, float* rho,
+               float* ux, float* uy, float* uz, void* stream);
 
 /* Replaces reads of CLBMSolver.f (clbm_moving.py:56): the post-stream state as
  * float32 planes out[q][z][y][x] in the reference's q order. */
-int clbm_export_state(const clbm_grid_t* g, const void* f, int parity, float* out, void* stream);
+int clbm_export_state(const clbm_grid_t* g, const clbm_params_t* p, const void* f, int state,
+                      float* out, void* stream);
 /* The same in float64 (for the float64 storage format). */
-int clbm_export_state_f64(const clbm_grid_t* g, const void* f, int parity, double* out,
-                          void* stream);
+int clbm_export_state_f64(const clbm_grid_t* g, const clbm_params_t* p, const void* f, int state,
+                          double* out, void* stream);
 
 /* Replaces writes of CLBMSolver.f (ti.field.from_numpy): loads float32 planes
- * in[q][z][y][x]; leaves a parity-0 state. */
+ * in[q][z][y][x]; leaves state 0. */
 int clbm_import_state(const clbm_grid_t* g, void* f, const float* in, void* stream);
 /* The same from float64 planes. */
 int clbm_import_state_f64(const clbm_grid_t* g, void* f, const double* in, void* stream);
 
 /* Replaces the driver's divergence guard (main_moving.py:130-145): out2[0] =
  * min rho over all cells, out2[1] = 1.0 if every population is finite else 0.0. */
-int clbm_health(const clbm_grid_t* g, const void* f, int parity, float* out2, void* stream);
+int clbm_health(const clbm_grid_t* g, const clbm_params_t* p, const void* f, int state,
+                float* out2, void* stream);
 
 /* ---- immersed boundary (replaces DroneMovingIBM, airfoil_moving.py:44-255) */
 
-/* Replaces DroneMovingIBM._update_markers (airfoil_moving.py:100-127).  cos
- * and sin of theta are evaluated in float64 and rounded once to float32; the
- * rotation is evaluated in float32 without contraction.  box (nullable)
- * receives the force box of the new positions (box[6..11] keeps the old one). */
+/* Replaces DroneMovingIBM.compute_force (airfoil_moving.py:230-255) as one
+ * launch: K8 _update_markers, K10 _interpolate, K11 _marker_forces and K12
+ * _spread; K9/K13 are ibm_clip_forces.  Markers are processed in the order
+ * `order` (nullable = 0..n-1; a spatially sorted order lets each workgroup
+ * pre-sum its stencils in LDS); outputs are indexed by marker.  Velocity
+ * comes from ux/uy ([z][y][x] float32) when both are given, else from the
+ * population state f (state word `state`, params p).  cos/sin of theta are
+ * evaluated in float64 and rounded once to float32; the rotation is float32
+ * without contraction.  F_k w is added into acc_x/acc_y (Nx*Ny*Nz 8-byte
+ * slots each, all zero before the first call; IBM_SPREAD_ATOMIC_F32 uses the
+ * first 4 bytes of a slot as a float) and the touched box into `box` for
+ * generation `gen`.  stencil_base (nullable): floor(pos)-1 per axis, n x 3. */
+int ibm_compute_force(const clbm_grid_t* g, const ibm_params_t* ip, const clbm_params_t* p,
+                      const void* f, int state, const float* ux, const float* uy, int n,
+                      const int32_t* order, const float* X0, const float* Y0, const float* Z0,
+                      float xc, float yc, float theta, float vx, float vy, float omega, float* X,
+                      float* Y, float* Z, float* VX, float* VY, float* U, float* V, float* FX,
+                      float* FY, int32_t* stencil_base, void* acc_x, void* acc_y, uint64_t* box,
+                      uint32_t gen, void* stream);
+
+/* Replaces DroneMovingIBM._update_markers (airfoil_moving.py:100-127) on its
+ * own (same arithmetic as in ibm_compute_force).  box (nullable) collects the
+ * box of the new stencils for generation `gen`. */
 int ibm_update_markers(const clbm_grid_t* g, int n, const float* X0, const float* Y0,
                        const float* Z0, float xc, float yc, float theta, float vx, float vy,
                        float omega, float* X, float* Y, float* Z, float* VX, float* VY,
-                       int32_t* box, void* stream);
+                       uint64_t* box, uint32_t gen, void* stream);
 
 /* Part of DroneMovingIBM._interpolate's input (the K7 velocity of
  * clbm_moving.py:305-321 that _interpolate reads): the force-free velocity
- * of the cells in box[0..5] only, computed from the population state f at
- * `parity` into [z][y][x] float32 fields ux/uy (other cells untouched). */
-int ibm_band_velocity(const clbm_grid_t* g, const void* f, int parity, const int32_t* box,
-                      float* ux, float* uy, void* stream);
+ * of the cells of generation gen's box only, computed from the population
+ * state f into [z][y][x] float32 fields ux/uy (other cells untouched). */
+int ibm_band_velocity(const clbm_grid_t* g, const clbm_params_t* p, const void* f, int state,
+                      const uint64_t* box, uint32_t gen, float* ux, float* uy, void* stream);
 
 /* Replaces DroneMovingIBM._interpolate + _marker_forces (airfoil_moving.py:
- * 142-183).  Velocity is read from ux/uy ([z][y][x] float32) when both are
- * given, else computed force-free from the population state f at `parity`.
- * stencil_base (nullable) receives floor(pos)-1 per axis, int32 n x 3. */
-int ibm_interp_force(const clbm_grid_t* g, const ibm_params_t* ip, const void* f, int parity,
-                     const float* ux, const float* uy, int n, const float* X, const float* Y,
-                     const float* Z, const float* VX, const float* VY, float* U, float* V,
-                     float* FX, float* FY, int32_t* stencil_base, void* stream);
+ * 142-183) on their own.  Velocity as in ibm_compute_force. */
+int ibm_interp_force(const clbm_grid_t* g, const ibm_params_t* ip, const clbm_params_t* p,
+                     const void* f, int state, const float* ux, const float* uy, int n,
+                     const float* X, const float* Y, const float* Z, const float* VX,
+                     const float* VY, float* U, float* V, float* FX, float* FY,
+                     int32_t* stencil_base, void* stream);
 
-/* Replaces DroneMovingIBM._spread (airfoil_moving.py:188-214): adds F_k * w
- * into the accumulators acc_x/acc_y (Nx*Ny*Nz 8-byte slots each, zero before
- * the first spread; IBM_SPREAD_ATOMIC_F32 uses the first 4 bytes of each slot
- * as a float). */
+/* Replaces DroneMovingIBM._spread (airfoil_moving.py:188-214) on its own:
+ * adds F_k w into acc_x/acc_y as ibm_compute_force does. */
 int ibm_spread(const clbm_grid_t* g, const ibm_params_t* ip, int n, const float* X, const float* Y,
                const float* Z, const float* FX, const float* FY, void* acc_x, void* acc_y,
                void* stream);
 
-/* K10 + K11 + K12 in one launch: ibm_interp_force followed by ibm_spread
- * with the same stencil weights. */
-int ibm_interp_spread(const clbm_grid_t* g, const ibm_params_t* ip, const void* f, int parity,
-                      const float* ux, const float* uy, int n, const float* X, const float* Y,
-                      const float* Z, const float* VX, const float* VY, float* U, float* V,
-                      float* FX, float* FY, int32_t* stencil_base, void* acc_x, void* acc_y,
-                      void* stream);
-
 /* Replaces DroneMovingIBM._zero_forces + _clip_forces (airfoil_moving.py:
- * 132-137,219-225): over the union of box[0..5] and box[6..11], converts the
- * accumulators into the clamped float32 fields fx/fy ([z][y][x]) and clears
- * the accumulators.  fx/fy are exact everywhere: outside both boxes they are
- * already zero. */
+ * 132-137,219-225): over the union of generation gen's box and generation
+ * gen-1's, converts the accumulators into the clamped float32 fields fx/fy
+ * ([z][y][x]; nullable) and clears them; fx/fy are then exact everywhere
+ * (outside both boxes they were already zero).  Writes generation gen's box
+ * to IBM_FORCE_BOX(box) for clbm_step. */
 int ibm_clip_forces(const clbm_grid_t* g, const ibm_params_t* ip, void* acc_x, void* acc_y,
-                    float* fx, float* fy, const int32_t* box, void* stream);
+                    float* fx, float* fy, uint64_t* box, uint32_t gen, void* stream);
 
 /* No reference counterpart (the reference never reduces forces): out2[0..1]
  * = sum_k FX_k dV_k, sum_k FY_k dV_k in float64 with a fixed summation order

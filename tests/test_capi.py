@@ -1,0 +1,87 @@
+"""The C ABI: the shared library loads on a CPU-only host, exports every
+function include/clbm_amd.h declares, and validates arguments before it
+launches anything (no GPU call is made here)."""
+
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from conftest import ROOT, load_sub
+
+HEADER = os.path.join(ROOT, "include", "clbm_amd.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s*([a-z_0-9]+)\s*\(", text, flags=re.M)
+    return sorted(set(n for n in names if n.startswith(("clbm_", "ibm_"))))
+
+
+def test_header_declares_the_expected_surface():
+    names = declared_functions()
+    for n in ("clbm_step", "clbm_init_feq", "clbm_macro", "ibm_update_markers", "ibm_compute_force",
+              "ibm_spread", "ibm_clip_forces", "ibm_force_sum_f64", "ibm_band_velocity"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = load_sub("_lib")
+    so = lib.load()
+    names = declared_functions()
+    missing = [n for n in names if not hasattr(so, n)]
+    assert not missing, missing
+    # the ctypes table binds exactly the declared functions
+    assert sorted(lib.exported_symbols()) == names
+
+
+def test_abi_version_and_state_bytes():
+    lib = load_sub("_lib")
+    so = lib.load()
+    assert so.clbm_abi_version() == lib.ABI_VERSION
+    g = lib.ClbmGrid(600, 200, 50, lib.STORE_F32)
+    assert so.clbm_state_bytes(C.byref(g)) == 19 * 600 * 200 * 50 * 4
+    g = lib.ClbmGrid(600, 200, 50, lib.STORE_F16)
+    assert so.clbm_state_bytes(C.byref(g)) == 19 * 600 * 200 * 50 * 2
+
+
+def test_state_after_step():
+    lib = load_sub("_lib")
+    so = lib.load()
+    p = lib.ClbmParams(1.7, 1.0, 0.05, 0.0, 1, 0)
+    q = lib.ClbmParams(1.7, 1.0, 0.05, 0.0, 0, 0)
+    assert so.clbm_state_after_step(0, C.byref(p)) == lib.STATE_PARITY | lib.STATE_PLANES
+    assert so.clbm_state_after_step(lib.STATE_PARITY | lib.STATE_PLANES, C.byref(p)) == lib.STATE_PLANES
+    assert so.clbm_state_after_step(lib.STATE_PARITY | lib.STATE_PLANES, C.byref(q)) == 0
+
+
+@pytest.mark.parametrize("call,match", [
+    (lambda lib: lib.call("clbm_step", None, None, None, 0, None, None, None, None, None), "params"),
+    (lambda lib: lib.call("clbm_init_feq", C.byref(lib.ClbmGrid(0, 4, 4, 0)), C.c_void_p(1),
+                          1.0, 0.0, 0.0, 0.0, None), "positive"),
+    (lambda lib: lib.call("clbm_init_feq", C.byref(lib.ClbmGrid(4, 4, 4, 9)), C.c_void_p(1),
+                          1.0, 0.0, 0.0, 0.0, None), "storage"),
+    (lambda lib: lib.call("clbm_init_feq", C.byref(lib.ClbmGrid(2048, 2048, 1024, 0)),
+                          C.c_void_p(1), 1.0, 0.0, 0.0, 0.0, None), "2\\^31"),
+    (lambda lib: lib.call("clbm_macro", C.byref(lib.ClbmGrid(8, 4, 4, 0)), None, None, 0, None,
+                          None, None, None, None), "NULL"),
+    (lambda lib: lib.call("ibm_update_markers", C.byref(lib.ClbmGrid(8, 4, 4, 0)), -1,
+                          *([None] * 3), *([0.0] * 6), *([None] * 6), 1, None), "n < 0"),
+    (lambda lib: lib.call("ibm_force_sum_f64", 5, None, None, None, None, None), "NULL"),
+])
+def test_argument_errors_are_reported(call, match):
+    lib = load_sub("_lib")
+    with pytest.raises(lib.ClbmError, match=match):
+        call(lib)
+
+
+def test_product_does_not_import_the_oracle():
+    """The package never reaches for the CPU oracle (no fallback path)."""
+    pkg_dir = os.path.join(ROOT, "clbm_ibm-moving-airfoil_amd")
+    for dirpath, _, files in os.walk(pkg_dir):
+        for fn in files:
+            if fn.endswith(".py"):
+                src = open(os.path.join(dirpath, fn)).read()
+                assert "clbm_oracle" not in src and "clbm_ref" not in src, fn
