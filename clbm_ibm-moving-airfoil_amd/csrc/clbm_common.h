@@ -175,6 +175,57 @@ __device__ __forceinline__ void load_state(const typename Store<S>::T* __restric
   for (int q = 0; q < Q; ++q) fq[q] = Store<S>::ld(f + slot_in<PAR>(q, N, c, nb), q);
 }
 
+// ---------------------------------------------------------------------------
+// Population access of the fused step (element = plane * N + cell).
+// ---------------------------------------------------------------------------
+template <int S> struct Pop {
+  using T = typename Store<S>::T;
+  using R = typename Store<S>::R;
+  T* f;
+  size_t N;
+  __device__ __forceinline__ R ld(int plane, int cell, int q) const {
+    return Store<S>::ld(f + (size_t)plane * N + cell, q);
+  }
+  __device__ __forceinline__ void st(int plane, int cell, int q, R v) const {
+    Store<S>::st(f + (size_t)plane * N + cell, q, v);
+  }
+};
+
+// Post-stream populations of cell c at parity PAR / post-collision stores of
+// a step from parity PAR, through a Pop accessor.
+template <int PAR, typename P>
+__device__ __forceinline__ void pop_load(const P& pop, int c, const Nbr& nb, typename P::R* fq) {
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    if (PAR == 0)
+      fq[q] = pop.ld(q, c, q);
+    else
+      fq[q] = pop.ld(opp(q), c + nb.off(-ex(q), -ey(q), -ez(q)), q);
+  }
+}
+// Stores fq as the post-stream populations of cell c (the slots pop_load reads).
+template <int PAR, typename P>
+__device__ __forceinline__ void pop_store_in(const P& pop, int c, const Nbr& nb,
+                                             const typename P::R* fq) {
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    if (PAR == 0)
+      pop.st(q, c, q, fq[q]);
+    else
+      pop.st(opp(q), c + nb.off(-ex(q), -ey(q), -ez(q)), q, fq[q]);
+  }
+}
+template <int PAR, typename P>
+__device__ __forceinline__ void pop_store(const P& pop, int c, const Nbr& nb, const typename P::R* fq) {
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    if (PAR == 0)
+      pop.st(opp(q), c, q, fq[q]);
+    else
+      pop.st(q, c + nb.off(ex(q), ey(q), ez(q)), q, fq[q]);
+  }
+}
+
 // The reference's inlet rule at x = 0 (CLBMSolver._bc_inlet, clbm_moving.py:
 // 266-294): the five east-going populations from the known ones, in place.
 template <typename R>

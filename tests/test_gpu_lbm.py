@@ -75,6 +75,34 @@ def test_steps_with_force_match_oracle(pkg, oracle, gpu, storage, atol, shape):
 
 
 @pytest.mark.parametrize("storage,atol", [("f32", 2e-6), ("f64", 1e-12)])
+@pytest.mark.parametrize("dense", [True, False])
+def test_outlet_rows_with_equal_and_unequal_force(pkg, oracle, gpu, storage, atol, dense):
+    """The x = Nx-1 column collides the outlet copy of x = Nx-2.  Rows whose
+    force equals that of their x = Nx-2 neighbour take one code path, the others
+    another; a force where half the rows match (and, without ``dense``, only a
+    force box well inside the grid) covers both against the oracle."""
+    shape = (11, 6, 5)
+    cfg = _case(pkg, *shape)
+    s = pkg.CLBMSolver(cfg, storage=storage)
+    f = _rand_state(oracle, shape, 3)
+    F = _rand_force(shape, 3)
+    if not dense:
+        for a in F:
+            a[:3] = 0.0
+            a[8:] = 0.0
+    else:
+        rows = np.random.default_rng(5).random(shape[1:]) < 0.5
+        for a in F:
+            a[-1][rows] = a[-2][rows]
+    s.f.from_numpy(f)
+    tF = _set_force(s, F)
+    for n in range(4):
+        s.step(*tF)
+        f, *_ = oracle.lbm_step(f, *F, cfg.s_v, cfg.s_e, cfg.ux_in, cfg.uy_in)
+        np.testing.assert_allclose(s.f.to_numpy(), f, rtol=0, atol=atol, err_msg=f"step {n + 1}")
+
+
+@pytest.mark.parametrize("storage,atol", [("f32", 2e-6), ("f64", 1e-12)])
 def test_periodic_box_matches_oracle_and_conserves_mass(pkg, oracle, gpu, storage, atol):
     shape = (10, 6, 5)
     cfg = _case(pkg, *shape)
