@@ -36,8 +36,9 @@ def parse(argv=None):
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--storage", default="f32", choices=sorted(ELEM))
     ap.add_argument("--case", default="c1", choices=["c0", "c1", "c2"])
-    ap.add_argument("--kernel-launches", type=int, default=400,
-                    help="launches of the fused step kernel timed on their own")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy a MIB-MiB device buffer 10 times (calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target duration of the CPU baseline sample (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=8)
@@ -89,12 +90,28 @@ def traffic_record(case: str, storage: str):
     return rec.get(f"{case}/{storage}")
 
 
+def copy_calibration(mib: int) -> int:
+    """A plain device-to-device copy of known size, profiled with the same
+    counters as the step kernel to turn counter readings into bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(10):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 10}))
+    return 0
+
+
 def main(argv=None) -> int:
     a = parse(argv)
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); the multi-GPU x-slab "
               f"decomposition is a design, not a measured path.", file=sys.stderr)
         return 2
+    if a.copy_calibration:
+        return copy_calibration(a.copy_calibration)
     import torch
     pkg = importlib.import_module(PKG)
     cfgmod = pkg.config
@@ -114,10 +131,27 @@ def main(argv=None) -> int:
     for _ in range(a.warmup):
         one(step)
         step += 1
+    # every CLBMSolver.step is exactly one launch of the fused step kernel on
+    # torch's current stream: HIP events around it in the timed region give
+    # the kernel's own average duration (on every EVERY-th step, alternating
+    # parities, so that the event records barely perturb the loop)
+    EVERY = 4
+    stream = torch.cuda.current_stream(dev)
+    pool = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+            for _ in range(a.steps // EVERY + 1)]
+    ev = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(a.steps):
-        one(step)
+    for i in range(a.steps):
+        xc, yc, th, vx, vy, om = cfg.motion_state(step)
+        fx, fy, fz = ibm.compute_force(lbm.ux, lbm.uy, xc, yc, th, vx, vy, om)
+        timed = i % EVERY == (i // EVERY) % 2
+        if timed:
+            ev.append(pool[len(ev)])
+            ev[-1][0].record(stream)
+        lbm.step(fx, fy, fz)
+        if timed:
+            ev[-1][1].record(stream)
         step += 1
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
@@ -125,29 +159,7 @@ def main(argv=None) -> int:
     if a.profile_only:
         print(json.dumps({"steps": a.steps, "seconds": dt, "finite": finite}))
         return 0
-
-    # fused step kernel alone: same state and force, the x planes left periodic
-    # so that each clbm_step call is exactly one launch of the fused kernel
-    lib = importlib.import_module(PKG + "._lib")
-    params = lib.ClbmParams(lbm.params.s_v, lbm.params.s_e, lbm.params.ux_in, lbm.params.uy_in, 0, 0)
-    stream = torch.cuda.current_stream(dev)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    fxp, fyp, boxp = ibm.fx._t.data_ptr(), ibm.fy._t.data_ptr(), ibm.box.data_ptr()
-    state = C.c_void_p(lbm.state.data_ptr())
-    sh = C.c_void_p(stream.cuda_stream)
-    word = lbm.state_word
-    for i in range(20):
-        lib.call("clbm_step", C.byref(lbm.grid), C.byref(params), state, word, C.c_void_p(fxp),
-                 C.c_void_p(fyp), None, C.c_void_p(boxp), sh)
-        word = (word & 1) ^ 1
-    e0.record(stream)
-    for i in range(a.kernel_launches):
-        lib.call("clbm_step", C.byref(lbm.grid), C.byref(params), state, word, C.c_void_p(fxp),
-                 C.c_void_p(fyp), None, C.c_void_p(boxp), sh)
-        word = (word & 1) ^ 1
-    e1.record(stream)
-    e1.synchronize()
-    k_ms = e0.elapsed_time(e1) / a.kernel_launches
+    k_ms = float(np.mean([e0.elapsed_time(e1) for e0, e1 in ev]))
     b_alg = 2 * 19 * ELEM[a.storage] * ncell
     achieved = b_alg / (k_ms * 1e-3) / 1e9
 

@@ -9,10 +9,12 @@
 //                        of its cell straight from the populations, the wave
 //                        reduces U_k, V_k and the clamped marker force, and
 //                        each lane spreads F_k w with the same weight.  A
-//                        workgroup holds 16 markers that are neighbours in
-//                        the body (the caller passes a spatially sorted
-//                        order), pre-sums their overlapping stencils in LDS
-//                        and sends one global atomic per touched cell.
+//                        workgroup holds G = 4, 8 or 16 markers that are
+//                        neighbours in the body (consecutive in the
+//                        template, which runs along each section, or in a
+//                        caller-supplied spatial order), pre-sums their
+//                        overlapping stencils in LDS and sends one global
+//                        atomic per touched cell.
 //   * ibm_clip_forces    K9 + K13 on the bounding box of the touched cells.
 // Spreading is order-independent by default: each product F_k w becomes a
 // 64-bit fixed-point integer and all sums (LDS and global) are integer sums,

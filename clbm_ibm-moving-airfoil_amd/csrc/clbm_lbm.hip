@@ -118,53 +118,32 @@ __global__ __launch_bounds__(BLK) void k_step(Geo g, double s_v, double s_e, Pla
   // own thread stands aside).  When the two cells see the same force (always,
   // unless the body reaches the outlet), their post-collision populations are
   // equal and the thread stores its result a second time, into cell nx-1's
-  // slots.  Otherwise it writes what it loaded into cell nx-1's slots (the
-  // outlet copy; no other thread touches them) and k_outlet collides it.
+  // slots.  Otherwise it first writes what it loaded into cell nx-1's slots
+  // (the outlet copy; no other thread touches them), and after its own cell
+  // reads them back and collides them with cell nx-1's force.
   if (pl.on && x == g.nx - 1) return;
-  R fq[Q], Fx, Fy, Fz;
-  const Nbr nb = make_nbr(g, x, y, z);
-  pop_load<PAR>(pop, c, nb, fq);
-  force_at<R, FORCE, BOX>(c, x, y, z, fx, fy, fz, box, Fx, Fy, Fz);
-  bool twin = false;
-  if (pl.on) {
-    if (x == g.nx - 2) {
-      R Gx, Gy, Gz;
-      force_at<R, FORCE, BOX>(c + 1, x + 1, y, z, fx, fy, fz, box, Gx, Gy, Gz);
-      twin = (Gx == Fx) & (Gy == Fy) & (Gz == Fz);
-      if (!twin) pop_store_in<PAR>(pop, c + 1, make_nbr(g, x + 1, y, z), fq);
-    }
-    if (x == 0) inlet_rule<R>(fq, R(pl.ux_in), R(pl.uy_in));
-  }
-  collide_cell<R>(fq, Fx, Fy, Fz, R(s_v), R(s_e));
-  pop_store<PAR>(pop, c, nb, fq);
-  if (twin) pop_store<PAR>(pop, c + 1, make_nbr(g, x + 1, y, z), fq);
-}
-
-// The x = nx-1 cells of a step with pending planes whose force differs from
-// that of their x = nx-2 neighbour (see k_step): their slots hold the outlet
-// copy; collide it with their own force.
-constexpr int OBLK = 64;
-template <int S, int PAR, bool FORCE, bool BOX>
-__global__ __launch_bounds__(OBLK) void k_outlet(Geo g, double s_v, double s_e,
-                                                 typename Store<S>::T* __restrict__ f,
-                                                 const float* __restrict__ fx,
-                                                 const float* __restrict__ fy,
-                                                 const float* __restrict__ fz,
-                                                 const int32_t* __restrict__ box) {
-  using R = typename Store<S>::R;
-  const int t = blockIdx.x * OBLK + threadIdx.x;
-  if (t >= g.ny * g.nz) return;
-  const int x = g.nx - 1, z = t / g.ny, y = t - z * g.ny;
-  const int c = t * g.nx + x;
   R fq[Q], Fx, Fy, Fz, Gx, Gy, Gz;
-  force_at<R, FORCE, BOX>(c, x, y, z, fx, fy, fz, box, Fx, Fy, Fz);
-  force_at<R, FORCE, BOX>(c - 1, x - 1, y, z, fx, fy, fz, box, Gx, Gy, Gz);
-  if ((Gx == Fx) & (Gy == Fy) & (Gz == Fz)) return;  // done by k_step
-  const Pop<S> pop{f, (size_t)g.ncell};
   const Nbr nb = make_nbr(g, x, y, z);
   pop_load<PAR>(pop, c, nb, fq);
+  force_at<R, FORCE, BOX>(c, x, y, z, fx, fy, fz, box, Fx, Fy, Fz);
+  const bool outlet = pl.on && x == g.nx - 2;
+  bool twin = true;
+  if (outlet) {
+    force_at<R, FORCE, BOX>(c + 1, x + 1, y, z, fx, fy, fz, box, Gx, Gy, Gz);
+    twin = (Gx == Fx) & (Gy == Fy) & (Gz == Fz);
+    if (!twin) pop_store_in<PAR>(pop, c + 1, make_nbr(g, x + 1, y, z), fq);
+  }
+  if (pl.on && x == 0) inlet_rule<R>(fq, R(pl.ux_in), R(pl.uy_in));
   collide_cell<R>(fq, Fx, Fy, Fz, R(s_v), R(s_e));
   pop_store<PAR>(pop, c, nb, fq);
+  if (outlet) {
+    const Nbr nb1 = make_nbr(g, x + 1, y, z);
+    if (!twin) {
+      pop_load<PAR>(pop, c + 1, nb1, fq);
+      collide_cell<R>(fq, Gx, Gy, Gz, R(s_v), R(s_e));
+    }
+    pop_store<PAR>(pop, c + 1, nb1, fq);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -327,9 +306,6 @@ template <int S> struct StepRun {
     auto fp = (typename Store<S>::T*)f;
     hipLaunchKernelGGL((k_step<S, PAR, FORCE, BOX>), dim3(nblocks(g.ncell)), dim3(BLK), 0, st, g,
                        p->s_v, p->s_e, pl, fp, fx, fy, fz, box);
-    if (pl.on && FORCE)  // without a force field every outlet cell is done by k_step
-      hipLaunchKernelGGL((k_outlet<S, PAR, FORCE, BOX>), dim3((g.ny * g.nz + OBLK - 1) / OBLK),
-                         dim3(OBLK), 0, st, g, p->s_v, p->s_e, fp, fx, fy, fz, box);
   }
   template <int PAR>
   static void launch_par(const Geo& g, const clbm_params_t* p, const Planes& pl, void* f,

@@ -1,0 +1,53 @@
+"""The driver counterpart and its on-disk format (main_moving.py:116-200):
+frames ``frame_XXXXXX.npz`` {ux, uy, uz, rho} float32 (Nx, Ny, Nz), ``meta.npz``
+with the reference's keys and the 24-entry ``grid_meta``, plus ``forces.npz``."""
+
+import os
+
+import numpy as np
+import pytest
+
+from conftest import load_sub
+
+META_KEYS = {"xc", "yc", "theta", "step", "X_template", "Y_template", "X_htail_tmpl",
+             "Y_htail_tmpl", "grid_meta"}
+
+
+def test_grid_meta_layout(pkg):
+    drv = load_sub("driver")
+    cfg = pkg.config.naca_case(40, 24, 4, 8.0, name="meta")
+    g = drv.grid_meta(cfg)
+    assert g.dtype == np.float64 and g.shape == (24,)
+    # main_moving.py:185-199: Nx, Ny, Nz, U_inf, Re, x0, y0, z0 first; vis_interval at 15
+    assert tuple(g[:3]) == (40, 24, 4)
+    assert g[3] == pytest.approx(0.05) and g[4] == pytest.approx(200.0)
+    assert g[5] == pytest.approx(cfg.x0) and g[6] == pytest.approx(cfg.y0)
+    assert g[15] == cfg.vis_interval
+
+
+@pytest.mark.gpu
+def test_driver_writes_reference_format_and_matches_oracle(pkg, oracle, gpu, tmp_path):
+    drv = load_sub("driver")
+    cfg = pkg.config.naca_case(40, 24, 4, 8.0, name="drv", N_steps=4).with_(vis_interval=2)
+    res = drv.run(cfg, 4, str(tmp_path), verbose=False)
+    assert not res["diverged"] and res["steps"] == 4
+    files = sorted(os.listdir(tmp_path))
+    assert files == ["forces.npz", "frame_000001.npz", "frame_000002.npz", "meta.npz"]
+    fr = np.load(tmp_path / "frame_000002.npz")
+    assert set(fr.files) == {"ux", "uy", "uz", "rho"}
+    for k in fr.files:
+        assert fr[k].dtype == np.float32 and fr[k].shape == (40, 24, 4)
+    meta = np.load(tmp_path / "meta.npz")
+    assert set(meta.files) == META_KEYS
+    assert list(meta["step"]) == [2, 4] and meta["grid_meta"].shape == (24,)
+    forces = np.load(tmp_path / "forces.npz")
+    assert forces["sums"].shape == (4, 2) and forces["cl"].shape == (4,)
+    # the frame after 4 steps against the FP64 oracle's coupled loop
+    f, *_ = oracle.init_feq(cfg.Nx, cfg.Ny, cfg.Nz, cfg.ux_in, cfg.uy_in)
+    f, hist = oracle.run_coupled(f, cfg.X0, cfg.Y0, cfg.Z0, 4, cfg.motion_state, cfg.s_v, cfg.s_e,
+                                 cfg.ux_in, cfg.uy_in, f_max=cfg.f_max, dV=cfg.dV,
+                                 dtype=np.float64)
+    rho, ux, uy, uz = oracle.macro_post(f)
+    np.testing.assert_allclose(fr["ux"], ux, rtol=0, atol=2e-6)
+    np.testing.assert_allclose(fr["rho"], rho, rtol=0, atol=2e-6)
+    np.testing.assert_allclose(forces["sums"], hist, rtol=0, atol=1e-4 * np.abs(hist).max())
