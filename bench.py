@@ -42,6 +42,14 @@ def parse(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target duration of the CPU baseline sample (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=8)
+    ap.add_argument("--group", type=int, default=None,
+                    help="markers per workgroup of the marker kernel (4, 8, 16)")
+    ap.add_argument("--outside-lds", type=int, default=None,
+                    help="LDS bytes reserved per workgroup by the outside part of the step")
+    ap.add_argument("--no-overlap", dest="overlap", action="store_false",
+                    help="keep the marker kernels on the step's stream")
+    ap.add_argument("--no-16bit", dest="with_16bit", action="store_false",
+                    help="skip the FP16/BF16-storage runs of the same workload")
     ap.add_argument("--profile-only", action="store_true",
                     help="run warmup + timed steps only (for rocprofv3)")
     return ap.parse_args(argv)
@@ -104,6 +112,49 @@ def copy_calibration(mib: int) -> int:
     return 0
 
 
+def measure(pkg, cfg, storage, steps, warmup, dev, tune):
+    """Warm-up, then ``steps`` timed coupled steps (compute_force + step) of a
+    fresh solver/IBM pair with ``storage`` populations."""
+    import torch
+    lbm = pkg.CLBMSolver(cfg, storage=storage, device=dev, **tune["lbm"])
+    ibm = pkg.DroneMovingIBM(cfg, device=dev, **tune["ibm"])
+    step = 1
+    for _ in range(warmup):
+        fx, fy, fz = ibm.compute_force(lbm.ux, lbm.uy, *cfg.motion_state(step))
+        lbm.step(fx, fy, fz)
+        step += 1
+    # The dominant kernel is the fused step kernel over the lattice away from
+    # the body (the whole lattice when the step is not split).  CLBMSolver
+    # records a pair of HIP events around it on the stream it launches on;
+    # that is done on every EVERY-th step, alternating parities, so that the
+    # event records barely perturb the loop.
+    EVERY = 4
+    pool = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+            for _ in range(steps // EVERY + 1)]
+    ev, cells = [], []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        xc, yc, th, vx, vy, om = cfg.motion_state(step)
+        fx, fy, fz = ibm.compute_force(lbm.ux, lbm.uy, xc, yc, th, vx, vy, om)
+        timed = i % EVERY == (i // EVERY) % 2
+        if timed:
+            ev.append(pool[len(ev)])
+            lbm.kernel_events = ev[-1]
+        lbm.step(fx, fy, fz)
+        if timed:
+            cells.append(lbm.last_kernel_cells)
+        step += 1
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    rho_min, finite = lbm.health()
+    k_ms = float(np.mean([e0.elapsed_time(e1) for e0, e1 in ev]))
+    out = dict(dt=dt, k_ms=k_ms, k_cells=float(np.mean(cells)), rho_min=rho_min, finite=finite)
+    del lbm, ibm
+    torch.cuda.empty_cache()
+    return out
+
+
 def main(argv=None) -> int:
     a = parse(argv)
     if a.gpus != 1:
@@ -118,50 +169,31 @@ def main(argv=None) -> int:
     cfg = {"c0": cfgmod.case_c0, "c1": cfgmod.case_c1, "c2": cfgmod.case_c2_large}[a.case]()
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
-    lbm = pkg.CLBMSolver(cfg, storage=a.storage, device=dev)
-    ibm = pkg.DroneMovingIBM(cfg, device=dev)
     ncell = cfg.Nx * cfg.Ny * cfg.Nz
-
-    def one(step):
-        xc, yc, th, vx, vy, om = cfg.motion_state(step)
-        fx, fy, fz = ibm.compute_force(lbm.ux, lbm.uy, xc, yc, th, vx, vy, om)
-        lbm.step(fx, fy, fz)
-
-    step = 1
-    for _ in range(a.warmup):
-        one(step)
-        step += 1
-    # every CLBMSolver.step is exactly one launch of the fused step kernel on
-    # torch's current stream: HIP events around it in the timed region give
-    # the kernel's own average duration (on every EVERY-th step, alternating
-    # parities, so that the event records barely perturb the loop)
-    EVERY = 4
-    stream = torch.cuda.current_stream(dev)
-    pool = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-            for _ in range(a.steps // EVERY + 1)]
-    ev = []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(a.steps):
-        xc, yc, th, vx, vy, om = cfg.motion_state(step)
-        fx, fy, fz = ibm.compute_force(lbm.ux, lbm.uy, xc, yc, th, vx, vy, om)
-        timed = i % EVERY == (i // EVERY) % 2
-        if timed:
-            ev.append(pool[len(ev)])
-            ev[-1][0].record(stream)
-        lbm.step(fx, fy, fz)
-        if timed:
-            ev[-1][1].record(stream)
-        step += 1
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    rho_min, finite = lbm.health()
+    tune = {"lbm": {}, "ibm": {"overlap": a.overlap}}
+    if a.outside_lds is not None:
+        tune["lbm"]["outside_lds"] = a.outside_lds
+    if a.group is not None:
+        tune["ibm"]["group"] = a.group
+    r = measure(pkg, cfg, a.storage, a.steps, a.warmup, dev, tune)
     if a.profile_only:
-        print(json.dumps({"steps": a.steps, "seconds": dt, "finite": finite}))
+        print(json.dumps({"steps": a.steps, "seconds": r["dt"], "finite": r["finite"]}))
         return 0
-    k_ms = float(np.mean([e0.elapsed_time(e1) for e0, e1 in ev]))
-    b_alg = 2 * 19 * ELEM[a.storage] * ncell
+    dt, k_ms, k_cells = r["dt"], r["k_ms"], r["k_cells"]
+    rho_min, finite = r["rho_min"], r["finite"]
+    b_alg = 2 * 19 * ELEM[a.storage] * k_cells
     achieved = b_alg / (k_ms * 1e-3) / 1e9
+    # the same workload with 16-bit deviation storage (the metric's FP16 line)
+    others = {}
+    if a.storage == "f32" and a.with_16bit:
+        for st in ("f16", "bf16"):
+            o = measure(pkg, cfg, st, a.steps, a.warmup, dev, tune)
+            m = a.steps * ncell / o["dt"] / 1e6
+            others[st] = {"value": round(m, 1), "ms_per_step": round(o["dt"] / a.steps * 1e3, 5),
+                          "kernel_ms": round(o["k_ms"], 5),
+                          "kernel_frac": round(2 * 19 * 2 * o["k_cells"] / (o["k_ms"] * 1e-3)
+                                               / 1e9 / HBM_PEAK_GBS, 4),
+                          "finite": o["finite"]}
 
     mlups = a.steps * ncell / dt / 1e6
     cpu = cpu_baseline(cfg, a.cpu_seconds, a.cpu_threads)
@@ -176,15 +208,21 @@ def main(argv=None) -> int:
                                f"{cfg.N_MARKERS} markers, Re {cfg.Re:g}, tau {cfg.tau:.3f}",
                    "grid": [cfg.Nx, cfg.Ny, cfg.Nz], "markers": cfg.N_MARKERS,
                    "storage": a.storage, "spread": "fixed-point", "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": "k_step (fused collide+force+stream)",
+        "roofline": {"bound": "hbm",
+                     "kernel": ("k_step outside the body region (fused collide+stream), "
+                                "concurrent with the marker kernels"
+                                if k_cells < ncell else "k_step (fused collide+force+stream)"),
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic_record(a.case, a.storage),
                      "kernel_ms": round(k_ms, 5),
-                     "bytes_per_launch": b_alg,
+                     "cells_per_launch": round(k_cells),
+                     "bytes_per_launch": round(b_alg),
                      "step_frac": round(mlups * 1e6 * 2 * 19 * ELEM[a.storage] / 1e9 / HBM_PEAK_GBS, 4)},
         "cpu_baseline": cpu,
         "health": {"rho_min": rho_min, "finite": finite},
+        "storage_16bit": {k: dict(v, ratio_vs_f32=round(v["value"] / mlups, 3))
+                          for k, v in others.items()},
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -21,7 +21,8 @@ SPREAD_MODES = {"fixed": SPREAD_FIXED, "atomic": SPREAD_ATOMIC_F32}
 BOX_SUBSLOTS = 64
 BOX_WORDS = 2 * 6 * BOX_SUBSLOTS + 4        # uint64 words of a force-box record
 FORCE_BOX_OFFSET = 2 * 6 * BOX_SUBSLOTS     # word offset of the decoded int32[6] box
-ABI_VERSION = 1
+REGION_FLAG_OFFSET = 2 * 6 * BOX_SUBSLOTS + 3   # word offset of the region-violation flag
+ABI_VERSION = 2
 
 
 class ClbmGrid(C.Structure):
@@ -30,12 +31,13 @@ class ClbmGrid(C.Structure):
 
 class ClbmParams(C.Structure):
     _fields_ = [("s_v", C.c_double), ("s_e", C.c_double), ("ux_in", C.c_double),
-                ("uy_in", C.c_double), ("inlet", C.c_int32), ("reserved", C.c_int32)]
+                ("uy_in", C.c_double), ("inlet", C.c_int32), ("outside_lds", C.c_int32)]
 
 
 class IbmParams(C.Structure):
     _fields_ = [("f_max", C.c_float), ("wrap_weight_quirk", C.c_int32), ("spread_mode", C.c_int32),
-                ("fixed_shift", C.c_int32), ("group", C.c_int32), ("reserved", C.c_int32)]
+                ("fixed_shift", C.c_int32), ("group", C.c_int32), ("check_region", C.c_int32),
+                ("region", C.c_int32 * 6)]
 
 
 _P = C.c_void_p
@@ -51,6 +53,8 @@ _SIGNATURES = {
     "clbm_state_after_step": (_I, [_I, _LP]),
     "clbm_init_feq": (_I, [_G, _P, C.c_double, C.c_double, C.c_double, C.c_double, _P]),
     "clbm_step": (_I, [_G, _LP, _P, _I, _P, _P, _P, _P, _P]),
+    "clbm_step_part": (_I, [_G, _LP, _P, _I, _P, _P, _P, _P, _P, _I, _P]),
+    "clbm_step_region": (_I, [_G, _LP, _P, _P]),
     "clbm_macro": (_I, [_G, _LP, _P, _I, _P, _P, _P, _P, _P]),
     "clbm_export_state": (_I, [_G, _LP, _P, _I, _P, _P]),
     "clbm_export_state_f64": (_I, [_G, _LP, _P, _I, _P, _P]),

@@ -10,6 +10,15 @@ markers visited in a spatially sorted order so that each workgroup can
 pre-sum its stencils), and one that clamps the accumulated force into the
 fields (K9 + K13) on the bounding box of the touched cells.
 
+When the velocity comes from a CLBMSolver (the reference's call
+``compute_force(lbm.ux, lbm.uy, ...)``), the two launches run on a stream of
+this object's own (``overlap=True``): the host predicts the box of cells the
+stencils can reach from the kinematics (the convex hull of the template,
+rotated in float32 as the kernel does, plus one cell), the kernel checks
+that no stencil leaves it, and the solver's next ``step`` updates the rest
+of the lattice while these kernels run.  Reading any of this object's fields
+first orders the current stream after them.
+
 Added readout with no reference counterpart: FP64 sums of the marker forces
 (``force_sums``) and a device-side history of them for C_D / C_L.
 """
@@ -22,8 +31,47 @@ import math
 import numpy as np
 
 from . import _lib
-from .config import CaseConfig, resolve, spatial_order
+from .config import CaseConfig, convex_hull_2d, resolve, spatial_order
 from .fields import Field, MarkerField
+
+
+def stencil_region(hx, hy, zr, shape, xc, yc, theta):
+    """Inclusive (x0, y0, z0, x1, y1, z1) box holding every stencil cell of a
+    rigid template placed by (xc, yc, theta), from the template's hull
+    vertices (hx, hy) and the z range ``zr`` of its stencils.
+
+    The hull vertices are placed with the kernel's float32 arithmetic (cos/sin
+    of the float32 angle rounded once from float64, no contraction).  The
+    extreme coordinates of the exact placement are attained at hull vertices;
+    every other marker's float32 position differs from its exact one by a few
+    ulp, so one cell of margin on each side covers it.  x is clamped to the
+    grid; a y range that crosses the periodic boundary takes the whole axis.
+    None for non-finite kinematics."""
+    nx, ny, nz = shape
+    f32 = np.float32
+    t = np.float64(f32(theta))
+    c, s = f32(math.cos(t)), f32(math.sin(t))
+    X = (f32(xc) + hx * c) - hy * s
+    Y = (f32(yc) + hx * s) + hy * c
+    if len(X) == 0 or not (np.all(np.isfinite(X)) and np.all(np.isfinite(Y))):
+        return None
+    # stencil cells floor(p) - 1 .. floor(p) + 2, plus one cell of margin
+    x0, x1 = int(np.floor(X.min())) - 2, int(np.floor(X.max())) + 3
+    y0, y1 = int(np.floor(Y.min())) - 2, int(np.floor(Y.max())) + 3
+    x0, x1 = max(x0, 0), min(x1, nx - 1)
+    if x0 > x1:
+        return None
+    if y0 < 0 or y1 > ny - 1:
+        y0, y1 = 0, ny - 1
+    return (x0, y0, int(zr[0]), x1, y1, int(zr[1]))
+
+
+def stencil_z_range(Z0, nz):
+    """The z range of the stencils of markers at heights Z0 (z never moves):
+    floor(z) - 1 .. floor(z) + 2, or the whole axis when a stencil wraps."""
+    z = np.asarray(Z0, dtype=np.float32)
+    k0, k1 = int(np.floor(z.min())) - 1, int(np.floor(z.max())) + 2
+    return (k0, k1) if (k0 >= 0 and k1 <= nz - 1) else (0, nz - 1)
 
 
 class DroneMovingIBM:
@@ -36,12 +84,13 @@ class DroneMovingIBM:
     per marker (for index checks); ``order`` "natural" (the template's own
     order, neighbours along each section) or "morton" (a Z-order sort of the
     template) for the marker groups of the fused kernel; ``group`` markers
-    per workgroup (4, 8 or 16).  None of these tuning options change results
-    in the fixed spread mode."""
+    per workgroup (4, 8 or 16); ``overlap`` False to keep the marker kernels
+    on the caller's stream.  None of these tuning options change results in
+    the fixed spread mode."""
 
     def __init__(self, config=None, device=None, spread_mode: str = "fixed",
                  f_max=None, wrap_weight_quirk=None, record_stencil: bool = False,
-                 order: str = "natural", group: int = 8):
+                 order: str = "natural", group: int = 8, overlap: bool = True):
         import torch
         cfg: CaseConfig = resolve(config)
         self.cfg = cfg
@@ -54,8 +103,11 @@ class DroneMovingIBM:
 
         def marr(a=None):
             if a is None:
-                return MarkerField(torch.zeros(self.N, dtype=f32, device=dev))
-            return MarkerField(torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).to(dev))
+                return MarkerField(torch.zeros(self.N, dtype=f32, device=dev), sync=self._join)
+            return MarkerField(torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).to(dev),
+                               sync=self._join)
+
+        self._pending = False
 
         self.X0, self.Y0, self.Z0 = marr(cfg.X0), marr(cfg.Y0), marr(cfg.Z0)
         self.X_marker, self.Y_marker, self.Z_marker = marr(), marr(), marr()
@@ -73,18 +125,26 @@ class DroneMovingIBM:
         self.gen = 0
         box_ptr = self.box.data_ptr() + 8 * _lib.FORCE_BOX_OFFSET
         self.fx = Field(shape, torch.zeros((self.Nz, self.Ny, self.Nx), dtype=f32, device=dev),
-                        owner=self, box=box_ptr)
+                        refresh=self._join, owner=self, box=box_ptr)
         self.fy = Field(shape, torch.zeros((self.Nz, self.Ny, self.Nx), dtype=f32, device=dev),
-                        owner=self, box=box_ptr)
+                        refresh=self._join, owner=self, box=box_ptr)
         self.fz = Field(shape, torch.zeros((self.Nz, self.Ny, self.Nx), dtype=f32, device=dev),
-                        owner=self, box=box_ptr)
+                        refresh=self._join, owner=self, box=box_ptr)
         self._acc = torch.zeros((2, ncell), dtype=torch.int64, device=dev)
         self.ip = _lib.IbmParams(float(cfg.f_max if f_max is None else f_max),
                                  int(cfg.wrap_weight_quirk if wrap_weight_quirk is None
                                      else wrap_weight_quirk),
                                  _lib.SPREAD_MODES[spread_mode], 0, int(group), 0)
-        self.stencil_base = (torch.zeros((self.N, 3), dtype=torch.int32, device=dev)
-                             if record_stencil else None)
+        self._stencil_base = (torch.zeros((self.N, 3), dtype=torch.int32, device=dev)
+                              if record_stencil else None)
+        # region prediction: hull of the template in the body frame, z range of its stencils
+        self.overlap = bool(overlap)
+        hull = convex_hull_2d(cfg.X0, cfg.Y0) if self.N else np.zeros(0, np.int64)
+        self._hx = np.asarray(cfg.X0, dtype=np.float32)[hull]
+        self._hy = np.asarray(cfg.Y0, dtype=np.float32)[hull]
+        self._zr = stencil_z_range(cfg.Z0, self.Nz) if self.N else (0, self.Nz - 1)
+        self._side = None
+        self._ev_start = self._ev_done = None
         self._hist = None
         self._hist_n = 0
         self._sums = torch.zeros(2, dtype=torch.float64, device=dev)
@@ -96,9 +156,29 @@ class DroneMovingIBM:
         (airfoil_moving.py:230-255).  ``ux``/``uy`` are the solver's velocity
         fields (read straight from its populations) or any [Nz,Ny,Nx] float32
         device Field/tensor.  Returns (fx, fy, fz), this object's own fields."""
-        st = self._stream()
-        p = lambda m: C.c_void_p(m.data_ptr())
+        import torch
+        p = lambda m: C.c_void_p(m._t.data_ptr() if isinstance(m, MarkerField) else m.data_ptr())
         grid, lp, f, state, uxp, uyp = self._velocity_source(ux, uy)
+        from .clbm_moving import CLBMSolver
+        solver = getattr(ux, "owner", None) if uxp == 0 else None
+        solver = solver if isinstance(solver, CLBMSolver) else None
+        region = self._region(solver, xc, yc, theta) if solver is not None else None
+        main = torch.cuda.current_stream(self.device)
+        if region is not None:
+            # run on this object's stream once everything queued so far is done
+            if self._side is None:
+                self._side = torch.cuda.Stream(device=self.device)
+                self._ev_start = torch.cuda.Event()
+                self._ev_done = torch.cuda.Event()
+            self._ev_start.record(main)
+            self._side.wait_event(self._ev_start)
+            st = C.c_void_p(self._side.cuda_stream)
+            self.ip.check_region = 1
+            self.ip.region[:] = list(region)
+        else:
+            self._join()
+            st = C.c_void_p(main.cuda_stream)
+            self.ip.check_region = 0
         self.gen += 1
         boxp = C.c_void_p(self.box.data_ptr())
         acc0, acc1 = C.c_void_p(self._acc[0].data_ptr()), C.c_void_p(self._acc[1].data_ptr())
@@ -110,15 +190,60 @@ class DroneMovingIBM:
                   float(omega), p(self.X_marker), p(self.Y_marker), p(self.Z_marker),
                   p(self.VX_body), p(self.VY_body), p(self.U_k), p(self.V_k), p(self.FX_k),
                   p(self.FY_k),
-                  C.c_void_p(self.stencil_base.data_ptr() if self.stencil_base is not None else 0),
+                  C.c_void_p(self._stencil_base.data_ptr() if self._stencil_base is not None
+                             else 0),
                   acc0, acc1, boxp, self.gen, st)
         _lib.call("ibm_clip_forces", C.byref(grid), C.byref(self.ip), acc0, acc1,
                   C.c_void_p(self.fx._t.data_ptr()), C.c_void_p(self.fy._t.data_ptr()), boxp,
                   self.gen, st)
+        if region is not None:
+            self._ev_done.record(self._side)
+            self._pending = True
+            solver._add_reader(self, region, self._ev_done)
         return self.fx, self.fy, self.fz
+
+    # -- the concurrent stream ------------------------------------------------
+    def _region(self, solver, xc, yc, theta):
+        """The normalised int32[6] region holding every stencil cell of this
+        call, or None when overlapping would not pay (region over half the
+        lattice) or is switched off."""
+        if not self.overlap or self.N == 0:
+            return None
+        r = stencil_region(self._hx, self._hy, self._zr, (self.Nx, self.Ny, self.Nz), xc, yc, theta)
+        if r is None:
+            return None
+        raw = (C.c_int32 * 6)(*r)
+        out = (C.c_int32 * 6)()
+        _lib.call("clbm_step_region", C.byref(solver.grid), C.byref(solver.params), raw, out)
+        vol = (out[3] - out[0] + 1) * (out[4] - out[1] + 1) * (out[5] - out[2] + 1)
+        if 2 * vol > self.Nx * self.Ny * self.Nz:
+            return None
+        return out
+
+    def _join(self) -> None:
+        """Order the current stream after this object's pending kernels."""
+        if self._pending:
+            import torch
+            torch.cuda.current_stream(self.device).wait_event(self._ev_done)
+            self._pending = False
+
+    def _joined(self) -> None:
+        self._pending = False
+
+    @property
+    def stencil_base(self):
+        """(N, 3) int32 device tensor of floor(pos) - 1 per marker (record_stencil=True)."""
+        self._join()
+        return self._stencil_base
+
+    def region_violations(self) -> int:
+        """The last generation whose stencils left the predicted region (0 if none)."""
+        self._join()
+        return int(self.box[_lib.REGION_FLAG_OFFSET].item())
 
     def force_box(self):
         """The (x0, y0, z0, x1, y1, z1) box of the cells the last compute_force reached."""
+        self._join()
         return self.box[_lib.FORCE_BOX_OFFSET:_lib.FORCE_BOX_OFFSET + 3].cpu().numpy().view(
             np.int32).copy()
 
@@ -141,6 +266,7 @@ class DroneMovingIBM:
         forces into ``out`` (a float64 device tensor of 2, default an internal
         one) and return it; nothing is synchronised."""
         out = self._sums if out is None else out
+        self._join()
         _lib.call("ibm_force_sum_f64", self.N, C.c_void_p(self.FX_k.data_ptr()),
                   C.c_void_p(self.FY_k.data_ptr()),
                   C.c_void_p(self.dV.data_ptr() if self.dV is not None else 0),

@@ -13,7 +13,12 @@ Differences of mechanism (not of results):
   run without a pass of their own;
 * ``rho, ux, uy, uz`` are computed on demand (K7 of clbm_moving.py:305-321)
   when read, instead of after every step; the immersed-boundary object reads
-  the force-free velocity straight from the populations.
+  the force-free velocity straight from the populations;
+* when the forces come from an immersed-boundary object that computed them on
+  its own stream (``DroneMovingIBM(overlap=True)``), the step runs in two
+  parts: the cells outside the body's region first, concurrently with the
+  marker kernels (they touch no population the markers read), then the
+  region with the force once the marker kernels are done.
 """
 
 from __future__ import annotations
@@ -73,6 +78,7 @@ class PopulationField:
         if a.shape != self.shape:
             raise ValueError(f"shape {a.shape} != {self.shape}")
         src = torch.from_numpy(np.ascontiguousarray(a.transpose(3, 2, 1, 0))).to(s.device)
+        s._wait_readers()
         _lib.call("clbm_import_state_f64" if wide else "clbm_import_state", C.byref(s.grid),
                   C.c_void_p(s.state.data_ptr()), C.c_void_p(src.data_ptr()), s._stream())
         s.state_word = 0
@@ -88,9 +94,13 @@ class CLBMSolver:
     ``config`` (CaseConfig or config module; default: ``sys.modules
     ['config_moving']`` or the reference airplane case), ``storage`` one of
     f32 (drop-in), f16, bf16 (deviation storage) or f64 (error reference),
-    ``inlet`` False for an x-periodic box."""
+    ``inlet`` False for an x-periodic box, ``outside_lds`` the LDS bytes per
+    workgroup the outside part of a split step reserves (a tuning choice:
+    33 KiB holds that launch to 4 workgroups per CU, which leaves each SIMD
+    room for one concurrent marker wavefront)."""
 
-    def __init__(self, config=None, storage: str = "f32", device=None, inlet: bool = True):
+    def __init__(self, config=None, storage: str = "f32", device=None, inlet: bool = True,
+                 outside_lds: int = 33792):
         import torch
         cfg: CaseConfig = resolve(config)
         self.cfg = cfg
@@ -102,7 +112,7 @@ class CLBMSolver:
         _lib.load()
         self.grid = _lib.ClbmGrid(self.Nx, self.Ny, self.Nz, _lib.STORAGE[storage])
         self.params = _lib.ClbmParams(float(cfg.s_v), float(cfg.s_e), float(cfg.ux_in),
-                                      float(cfg.uy_in), 1 if inlet else 0, 0)
+                                      float(cfg.uy_in), 1 if inlet else 0, int(outside_lds))
         nbytes = Q * self.Nx * self.Ny * self.Nz * _lib.ELEM_BYTES[self.grid.storage]
         self.state = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
         self.state_word = 0
@@ -116,11 +126,19 @@ class CLBMSolver:
         self.uz = Field(shape, None, refresh=self._refresh_macro, owner=self)
         self.f = PopulationField(self)
         self.M_inv = HostConstant(_m_raw_inv_f32())
+        # immersed-boundary objects reading the populations on their own
+        # stream: {id: (ibm, region int32[6], event after their last read)}
+        self._readers = {}
+        # optional (start, end) events recorded around the first kernel of the
+        # next step (the whole step, or its outside part); cells that kernel updated
+        self.kernel_events = None
+        self.last_kernel_cells = 0
         self._init_feq()
 
     # -- reference surface ------------------------------------------------
     def _init_feq(self) -> None:
         """K1 (clbm_moving.py:75-91): equilibrium at the inlet velocity, rho = 1."""
+        self._wait_readers()
         _lib.call("clbm_init_feq", C.byref(self.grid), C.c_void_p(self.state.data_ptr()), 1.0,
                   float(self.cfg.ux_in), float(self.cfg.uy_in), 0.0, self._stream())
         self.state_word = 0
@@ -128,22 +146,79 @@ class CLBMSolver:
 
     def step(self, fx=None, fy=None, fz=None) -> None:
         """Advance one step with body force (fx, fy, fz) (clbm_moving.py:326-343)."""
-        shape = (self.Nx, self.Ny, self.Nz)
-        box = 0
-        for fld in (fx, fy):
-            if isinstance(fld, Field) and fld.box is not None:
-                box = fld.box
-        fzp = as_device_ptr(fz, shape)
-        if isinstance(fz, Field) and fz.box is not None:
-            fzp = 0     # the immersed-boundary fz is identically zero (airfoil_moving.py:225)
-        _lib.call("clbm_step", C.byref(self.grid), C.byref(self.params),
-                  C.c_void_p(self.state.data_ptr()), self.state_word,
-                  C.c_void_p(as_device_ptr(fx, shape)), C.c_void_p(as_device_ptr(fy, shape)),
-                  C.c_void_p(fzp), C.c_void_p(box),
-                  self._stream())
+        import torch
+        split = self._split_source(fx, fy, fz)
+        ev = self.kernel_events
+        self.kernel_events = None
+        main = torch.cuda.current_stream(self.device)
+        st = C.c_void_p(main.cuda_stream)
+        state = C.c_void_p(self.state.data_ptr())
+        if split is not None:
+            ibm, region, done = split
+            if ev is not None:
+                ev[0].record(main)
+            _lib.call("clbm_step_part", C.byref(self.grid), C.byref(self.params), state,
+                      self.state_word, None, None, None, None, region, 0, st)
+            if ev is not None:
+                ev[1].record(main)
+            main.wait_event(done)
+            ibm._joined()
+            self._readers.clear()
+            _lib.call("clbm_step_part", C.byref(self.grid), C.byref(self.params), state,
+                      self.state_word, C.c_void_p(ibm.fx._t.data_ptr()),
+                      C.c_void_p(ibm.fy._t.data_ptr()), None, C.c_void_p(ibm.fx.box), region, 1,
+                      st)
+            r = list(region)
+            self.last_kernel_cells = self.ncell - ((r[3] - r[0] + 1) * (r[4] - r[1] + 1)
+                                                   * (r[5] - r[2] + 1))
+        else:
+            self._wait_readers()
+            shape = (self.Nx, self.Ny, self.Nz)
+            box = 0
+            for fld in (fx, fy):
+                if isinstance(fld, Field) and fld.box is not None:
+                    box = fld.box
+            fzp = as_device_ptr(fz, shape)
+            if isinstance(fz, Field) and fz.box is not None:
+                fzp = 0     # the immersed-boundary fz is identically zero (airfoil_moving.py:225)
+            fxp, fyp = as_device_ptr(fx, shape), as_device_ptr(fy, shape)
+            if ev is not None:
+                ev[0].record(main)
+            _lib.call("clbm_step", C.byref(self.grid), C.byref(self.params), state,
+                      self.state_word, C.c_void_p(fxp), C.c_void_p(fyp), C.c_void_p(fzp),
+                      C.c_void_p(box), st)
+            if ev is not None:
+                ev[1].record(main)
+            self.last_kernel_cells = self.ncell
         self.state_word = _lib.load().clbm_state_after_step(self.state_word, C.byref(self.params))
         self.steps += 1
         self._macro_valid = False
+
+    # -- concurrent readers (immersed-boundary objects on their own stream) --
+    def _add_reader(self, ibm, region, done_event) -> None:
+        self._readers[id(ibm)] = (ibm, region, done_event)
+
+    def _wait_readers(self) -> None:
+        """Order the current stream after every pending concurrent reader
+        (before anything that writes the populations)."""
+        if not self._readers:
+            return
+        import torch
+        main = torch.cuda.current_stream(self.device)
+        for ibm, _, done in self._readers.values():
+            main.wait_event(done)
+            ibm._joined()
+        self._readers.clear()
+
+    def _split_source(self, fx, fy, fz):
+        """(ibm, region, event) when (fx, fy, fz) are the fields of the one
+        immersed-boundary object reading this solver on its own stream."""
+        if len(self._readers) != 1:
+            return None
+        ibm, region, done = next(iter(self._readers.values()))
+        if fx is not ibm.fx or fy is not ibm.fy or (fz is not None and fz is not ibm.fz):
+            return None
+        return ibm, region, done
 
     @property
     def parity(self) -> int:

@@ -137,6 +137,35 @@ def spatial_order(X0, Y0, Z0, cell: float = 1.0) -> np.ndarray:
     return np.argsort(key, kind="stable").astype(np.int32)
 
 
+def convex_hull_2d(X, Y) -> np.ndarray:
+    """Indices of the vertices of the convex hull of the points (X, Y)
+    (Andrew's monotone chain, counter-clockwise, collinear points dropped).
+    Under any rotation and translation, the extreme coordinates of the point
+    set are attained at these vertices."""
+    x = np.asarray(X, dtype=np.float64)
+    y = np.asarray(Y, dtype=np.float64)
+    n = len(x)
+    if n <= 2:
+        return np.arange(n, dtype=np.int64)
+    idx = np.lexsort((y, x))
+
+    def half(order):
+        out = []
+        for i in order:
+            while len(out) >= 2:
+                a, b = out[-2], out[-1]
+                cross = (x[b] - x[a]) * (y[i] - y[a]) - (y[b] - y[a]) * (x[i] - x[a])
+                if cross > 0:
+                    break
+                out.pop()
+            out.append(i)
+        return out
+
+    lower = half(idx)
+    upper = half(idx[::-1])
+    return np.array(lower[:-1] + upper[:-1], dtype=np.int64)
+
+
 # ---------------------------------------------------------------------------
 # case configuration
 # ---------------------------------------------------------------------------

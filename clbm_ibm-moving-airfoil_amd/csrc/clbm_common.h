@@ -143,6 +143,31 @@ __device__ __forceinline__ void cell_xyz(const Geo& g, int c, int& x, int& y, in
   y = (int)row - z * g.ny;
 }
 
+// An inclusive box of cells [lo, hi] and the divisors that enumerate it
+// (x fastest), for the two parts of a split step.
+struct Region {
+  int lo[3], hi[3];
+  int rx, ry, vol;
+  FastDiv drx, dry;
+};
+inline Region make_region(const int32_t* r) {
+  Region o;
+  for (int a = 0; a < 3; ++a) {
+    o.lo[a] = r[a];
+    o.hi[a] = r[3 + a];
+  }
+  o.rx = o.hi[0] - o.lo[0] + 1;
+  o.ry = o.hi[1] - o.lo[1] + 1;
+  o.vol = o.rx * o.ry * (o.hi[2] - o.lo[2] + 1);
+  o.drx = make_fastdiv((uint32_t)o.rx);
+  o.dry = make_fastdiv((uint32_t)o.ry);
+  return o;
+}
+__device__ __forceinline__ bool in_region(const Region& r, int x, int y, int z) {
+  return (x >= r.lo[0]) & (x <= r.hi[0]) & (y >= r.lo[1]) & (y <= r.hi[1]) & (z >= r.lo[2]) &
+         (z <= r.hi[2]);
+}
+
 // Block index remap so that consecutive logical tiles run on the same XCD
 // (the dispatcher hands workgroups to the 8 XCDs round robin).
 __device__ __forceinline__ int xcd_block(int bid, int nblk) {

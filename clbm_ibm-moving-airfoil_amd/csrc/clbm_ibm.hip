@@ -277,6 +277,13 @@ template <int G> struct Group {
   static constexpr int tile = 128 * G;  // cells
 };
 
+// The region every stencil cell must lie in (when `on`): the cells a
+// concurrent outside-part step leaves alone.
+struct RegionCheck {
+  int on;
+  int lo[3], hi[3];
+};
+
 template <int S, int UMODE, int SMODE, int G>
 __global__ __launch_bounds__(64 * G) void k_compute_force(
     Geo g, Planes pl, int quirk, float f_max, float fscale, const typename Store<S>::T* __restrict__ f,
@@ -285,14 +292,17 @@ __global__ __launch_bounds__(64 * G) void k_compute_force(
     const float* __restrict__ Z0, Kin kin, float* __restrict__ X, float* __restrict__ Y,
     float* __restrict__ Z, float* __restrict__ VX, float* __restrict__ VY, float* __restrict__ U,
     float* __restrict__ V, float* __restrict__ FX, float* __restrict__ FY, int32_t* __restrict__ base,
-    void* __restrict__ accx, void* __restrict__ accy, uint64_t* __restrict__ box, uint32_t gen) {
+    void* __restrict__ accx, void* __restrict__ accy, uint64_t* __restrict__ box, uint32_t gen,
+    RegionCheck rc) {
   constexpr int NT = Group<G>::threads, TC = Group<G>::tile;
   __shared__ unsigned long long sacc[2 * TC];  // spread sums (x then y)
   __shared__ float svel[2 * TC];               // staged velocity (u then v)
   __shared__ int sbase[G][3];
   __shared__ int srng[G][6];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int slot = blockIdx.x * G + wv;
+  // consecutive groups (neighbours along the body, then the next layers) go
+  // to the same XCD, so their overlapping tiles meet in that XCD's L2
+  const int slot = xcd_block(blockIdx.x, gridDim.x) * G + wv;
   const bool live = slot < n;  // uniform per wavefront
   const int m = live ? (order ? order[slot] : slot) : 0;
 
@@ -431,6 +441,12 @@ __global__ __launch_bounds__(64 * G) void k_compute_force(
         hi[a] = max(hi[a], srng[i][3 + a]);
       }
     box_grow(box, gen, lo, hi);
+    if (rc.on && lo[0] != INT_MAX) {
+      bool out = false;
+      for (int a = 0; a < 3; ++a) out |= (lo[a] < rc.lo[a]) | (hi[a] > rc.hi[a]);
+      if (out)
+        atomicMax((unsigned long long*)IBM_REGION_FLAG(box), (unsigned long long)gen);
+    }
   }
 }
 
@@ -607,7 +623,21 @@ static int check_ibm(const ibm_params_t* ip) {
   if (ip->fixed_shift < 0 || ip->fixed_shift > 60) return fail(-1, "fixed_shift out of [0, 60]");
   if (ip->group != 0 && ip->group != 4 && ip->group != 8 && ip->group != 16)
     return fail(-1, "group must be 0, 4, 8 or 16");
+  if (ip->check_region != 0 && ip->check_region != 1) return fail(-1, "check_region must be 0 or 1");
+  if (ip->check_region)
+    for (int a = 0; a < 3; ++a)
+      if (ip->region[a] > ip->region[3 + a]) return fail(-1, "region has lo > hi");
   return 0;
+}
+
+static inline RegionCheck region_check(const ibm_params_t* ip) {
+  RegionCheck r{};
+  r.on = ip->check_region;
+  for (int a = 0; a < 3; ++a) {
+    r.lo[a] = ip->region[a];
+    r.hi[a] = ip->region[3 + a];
+  }
+  return r;
 }
 
 // Largest shift that keeps 2^20 clamped contributions |F w| <= f_max / 8
@@ -642,7 +672,8 @@ static void launch_compute(const Geo& g, const Planes& pl, const ibm_params_t* i
 #define CLBM_CG(GG)                                                                              \
   hipLaunchKernelGGL((k_compute_force<S, UMODE, SMODE, GG>), dim3((n + GG - 1) / GG),           \
                      dim3(64 * GG), 0, st, g, pl, q, ip->f_max, scale, fp, ux, uy, n, order, X0, \
-                     Y0, Z0, kin, X, Y, Z, VX, VY, U, V, FX, FY, base, ax, ay, box, gen)
+                     Y0, Z0, kin, X, Y, Z, VX, VY, U, V, FX, FY, base, ax, ay, box, gen,          \
+                     region_check(ip))
   switch (ip->group) {
     case 4: CLBM_CG(4); break;
     case 16: CLBM_CG(16); break;
@@ -740,6 +771,7 @@ int ibm_compute_force(const clbm_grid_t* g, const ibm_params_t* ip, const clbm_p
     return fail(-1, "ibm_compute_force: NULL marker array");
   if (!acc_x || !acc_y) return fail(-1, "ibm_compute_force: NULL accumulator");
   if (box && gen == 0) return fail(-1, "ibm_compute_force: generation must be >= 1");
+  if (ip->check_region && !box) return fail(-1, "ibm_compute_force: check_region needs the box record");
   Planes pl;
   int umode;
   if (int e = velocity_source(g, p, f, state, ux, uy, pl, umode)) return e;

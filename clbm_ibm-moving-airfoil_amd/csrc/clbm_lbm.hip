@@ -98,19 +98,34 @@ __device__ __forceinline__ void force_at(int c, int x, int y, int z, const float
   if (fz) Fz = R(fz[c]);
 }
 
-template <int S, int PAR, bool FORCE, bool BOX>
+// REG selects the cells of the launch: 0 every cell; 1 the cells outside
+// region rg (one thread per cell of the grid); 2 the cells inside it (one
+// thread per cell of the region).
+template <int S, int PAR, bool FORCE, bool BOX, int REG>
 __global__ __launch_bounds__(BLK) void k_step(Geo g, double s_v, double s_e, Planes pl,
                                               typename Store<S>::T* __restrict__ f,
                                               const float* __restrict__ fx,
                                               const float* __restrict__ fy,
                                               const float* __restrict__ fz,
-                                              const int32_t* __restrict__ box) {
+                                              const int32_t* __restrict__ box, Region rg) {
   using R = typename Store<S>::R;
   const int lb = xcd_block(blockIdx.x, gridDim.x);
-  const int c = lb * BLK + threadIdx.x;
-  if (c >= g.ncell) return;
-  int x, y, z;
-  cell_xyz(g, c, x, y, z);
+  const int t = lb * BLK + threadIdx.x;
+  int c, x, y, z;
+  if (REG == 2) {
+    if (t >= rg.vol) return;
+    const uint32_t row = fdiv((uint32_t)t, rg.drx);
+    const int iz = (int)fdiv(row, rg.dry);
+    x = rg.lo[0] + (t - (int)row * rg.rx);
+    y = rg.lo[1] + ((int)row - iz * rg.ry);
+    z = rg.lo[2] + iz;
+    c = (z * g.ny + y) * g.nx + x;
+  } else {
+    c = t;
+    if (c >= g.ncell) return;
+    cell_xyz(g, c, x, y, z);
+    if (REG == 1 && in_region(rg, x, y, z)) return;
+  }
   const Pop<S> pop{f, (size_t)g.ncell};
   // With pending planes, cell x = nx-1 collides the outlet copy of cell nx-2's
   // post-stream populations, but those slots are overwritten by the thread of
@@ -298,35 +313,50 @@ template <int S> struct InitRun {
   }
 };
 
+// part: -1 the whole grid, 0 the cells outside the region (force-free), 1 the
+// cells inside it.
 template <int S> struct StepRun {
-  template <int PAR, bool FORCE, bool BOX>
+  template <int PAR, bool FORCE, bool BOX, int REG>
   static void launch(const Geo& g, const clbm_params_t* p, const Planes& pl, void* f,
                      const float* fx, const float* fy, const float* fz, const int32_t* box,
-                     hipStream_t st) {
+                     const Region& rg, hipStream_t st) {
     auto fp = (typename Store<S>::T*)f;
-    hipLaunchKernelGGL((k_step<S, PAR, FORCE, BOX>), dim3(nblocks(g.ncell)), dim3(BLK), 0, st, g,
-                       p->s_v, p->s_e, pl, fp, fx, fy, fz, box);
+    const int n = REG == 2 ? rg.vol : g.ncell;
+    const unsigned lds = REG == 1 ? (unsigned)p->outside_lds : 0u;
+    hipLaunchKernelGGL((k_step<S, PAR, FORCE, BOX, REG>), dim3(nblocks(n)), dim3(BLK), lds, st, g,
+                       p->s_v, p->s_e, pl, fp, fx, fy, fz, box, rg);
+  }
+  template <int PAR, int REG>
+  static void launch_force(const Geo& g, const clbm_params_t* p, const Planes& pl, void* f,
+                           const float* fx, const float* fy, const float* fz, const int32_t* box,
+                           const Region& rg, hipStream_t st) {
+    const bool force = fx || fy || fz;
+    if (!force)
+      launch<PAR, false, false, REG>(g, p, pl, f, fx, fy, fz, box, rg, st);
+    else if (box)
+      launch<PAR, true, true, REG>(g, p, pl, f, fx, fy, fz, box, rg, st);
+    else
+      launch<PAR, true, false, REG>(g, p, pl, f, fx, fy, fz, box, rg, st);
   }
   template <int PAR>
   static void launch_par(const Geo& g, const clbm_params_t* p, const Planes& pl, void* f,
                          const float* fx, const float* fy, const float* fz, const int32_t* box,
-                         hipStream_t st) {
-    const bool force = fx || fy || fz;
-    if (!force)
-      launch<PAR, false, false>(g, p, pl, f, fx, fy, fz, box, st);
-    else if (box)
-      launch<PAR, true, true>(g, p, pl, f, fx, fy, fz, box, st);
+                         const Region& rg, int part, hipStream_t st) {
+    if (part < 0)
+      launch_force<PAR, 0>(g, p, pl, f, fx, fy, fz, box, rg, st);
+    else if (part == 0)
+      launch<PAR, false, false, 1>(g, p, pl, f, nullptr, nullptr, nullptr, nullptr, rg, st);
     else
-      launch<PAR, true, false>(g, p, pl, f, fx, fy, fz, box, st);
+      launch_force<PAR, 2>(g, p, pl, f, fx, fy, fz, box, rg, st);
   }
   static int run(const Geo& g, const clbm_params_t* p, const Planes& pl, void* f, int parity,
                  const float* fx, const float* fy, const float* fz, const int32_t* box,
-                 hipStream_t st) {
+                 const Region& rg, int part, hipStream_t st) {
     if (parity == 0)
-      launch_par<0>(g, p, pl, f, fx, fy, fz, box, st);
+      launch_par<0>(g, p, pl, f, fx, fy, fz, box, rg, part, st);
     else
-      launch_par<1>(g, p, pl, f, fx, fy, fz, box, st);
-    return check_launch("clbm_step");
+      launch_par<1>(g, p, pl, f, fx, fy, fz, box, rg, part, st);
+    return check_launch(part < 0 ? "clbm_step" : "clbm_step_part");
   }
 };
 
@@ -415,8 +445,51 @@ int clbm_step(const clbm_grid_t* g, const clbm_params_t* p, void* f, int state, 
   Planes pl;
   if (int e = check_state(g, p, state, pl)) return e;
   if (p->inlet && !(p->ux_in < 1.0)) return fail(-1, "clbm_step: inlet needs ux_in < 1");
+  const Region none{};
   return dispatch_storage<StepRun>(g->storage, make_geo(*g), p, pl, f, state & CLBM_STATE_PARITY,
-                                   fx, fy, fz, force_box, (hipStream_t)stream);
+                                   fx, fy, fz, force_box, none, -1, (hipStream_t)stream);
+}
+
+int clbm_step_region(const clbm_grid_t* g, const clbm_params_t* p, const int32_t* region,
+                     int32_t* out) {
+  if (!p || !region || !out) return fail(-1, "clbm_step_region: NULL argument");
+  if (int e = check_grid(g, p->inlet != 0)) return e;
+  const int n[3] = {g->nx, g->ny, g->nz};
+  for (int a = 0; a < 3; ++a) {
+    if (region[a] > region[3 + a]) return fail(-1, "clbm_step_region: region has lo > hi");
+    out[a] = std::max(region[a], 0);
+    out[3 + a] = std::min(region[3 + a], n[a] - 1);
+    if (out[a] > out[3 + a]) return fail(-1, "clbm_step_region: region lies outside the grid");
+  }
+  // the outlet cell x = nx-1 is stepped by the thread of x = nx-2, and a reader
+  // of cell nx-1 reads cell nx-2's populations: keep both in the same part
+  if (p->inlet && out[3] >= g->nx - 2) {
+    out[0] = std::min(out[0], g->nx - 2);
+    out[3] = g->nx - 1;
+  }
+  return 0;
+}
+
+int clbm_step_part(const clbm_grid_t* g, const clbm_params_t* p, void* f, int state,
+                   const float* fx, const float* fy, const float* fz, const int32_t* force_box,
+                   const int32_t* region, int part, void* stream) {
+  if (!p) return fail(-1, "clbm_step_part: params is NULL");
+  if (int e = check_grid(g, p->inlet != 0)) return e;
+  if (!f) return fail(-1, "clbm_step_part: f is NULL");
+  if (part != 0 && part != 1) return fail(-1, "clbm_step_part: part must be 0 or 1");
+  if (p->outside_lds < 0 || p->outside_lds > 64 * 1024)
+    return fail(-1, "clbm_step_part: outside_lds must be in [0, 65536]");
+  Planes pl;
+  if (int e = check_state(g, p, state, pl)) return e;
+  if (p->inlet && !(p->ux_in < 1.0)) return fail(-1, "clbm_step_part: inlet needs ux_in < 1");
+  int32_t r[6];
+  if (int e = clbm_step_region(g, p, region, r)) return e;
+  for (int a = 0; a < 6; ++a)
+    if (r[a] != region[a])
+      return fail(-1, "clbm_step_part: region is not normalised (pass it through clbm_step_region)");
+  return dispatch_storage<StepRun>(g->storage, make_geo(*g), p, pl, f, state & CLBM_STATE_PARITY,
+                                   fx, fy, fz, force_box, make_region(r), part,
+                                   (hipStream_t)stream);
 }
 
 int clbm_macro(const clbm_grid_t* g, const clbm_params_t* p, const void* f, int state, float* rho,

@@ -62,29 +62,35 @@ class Field:
 
 
 class MarkerField:
-    """A float32 per-marker array of shape (N,) (the reference's (N,) ti.fields)."""
+    """A float32 per-marker array of shape (N,) (the reference's (N,) ti.fields).
 
-    def __init__(self, tensor):
+    ``sync`` (optional) is called before any access so that work queued on
+    another stream (the owner's marker kernels) is ordered first."""
+
+    def __init__(self, tensor, sync=None):
         self._t = tensor
+        self._sync = sync
         self.shape = (int(tensor.shape[0]),)
 
     @property
     def tensor(self):
+        if self._sync is not None:
+            self._sync()
         return self._t
 
     def data_ptr(self) -> int:
-        return self._t.data_ptr()
+        return self.tensor.data_ptr()
 
     def to_numpy(self) -> np.ndarray:
-        return self._t.detach().to("cpu").numpy().copy()
+        return self.tensor.detach().to("cpu").numpy().copy()
 
     def to_torch(self):
-        return self._t
+        return self.tensor
 
     def from_numpy(self, arr) -> None:
         torch = _torch()
         a = np.asarray(arr, dtype=np.float32).reshape(self.shape)
-        self._t.copy_(torch.from_numpy(a))
+        self.tensor.copy_(torch.from_numpy(a))
 
     def __repr__(self) -> str:
         return f"MarkerField(shape={self.shape})"

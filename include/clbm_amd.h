@@ -48,7 +48,7 @@
 extern "C" {
 #endif
 
-#define CLBM_ABI_VERSION 1
+#define CLBM_ABI_VERSION 2
 
 /* population storage formats */
 #define CLBM_STORE_F32 0  /* float32 populations, float32 arithmetic (the drop-in) */
@@ -67,6 +67,9 @@ extern "C" {
 #define IBM_BOX_WORDS (2 * 6 * IBM_BOX_SUBSLOTS + 4)
 /* the decoded int32[6] box inside the uint64 box record */
 #define IBM_FORCE_BOX(box) ((int32_t*)((box) + 2 * 6 * IBM_BOX_SUBSLOTS))
+/* the word of the box record where ibm_compute_force marks (as an atomic
+ * maximum of gen) a generation whose stencils left ibm_params_t.region */
+#define IBM_REGION_FLAG(box) ((box) + 2 * 6 * IBM_BOX_SUBSLOTS + 3)
 
 typedef struct {
   int32_t nx, ny, nz; /* lattice size, Nx >= 3 */
@@ -80,7 +83,9 @@ typedef struct {
                           velocities are rounded to the arithmetic type of the storage format */
   int32_t inlet;      /* 1: inlet rule at x=0 and outlet copy at x=Nx-1 after every
                          stream (clbm_moving.py:266-300); 0: x stays periodic */
-  int32_t reserved;
+  int32_t outside_lds; /* bytes of LDS each workgroup of clbm_step_part part 0 reserves
+                          (unused by it): caps that launch's workgroups per CU so that
+                          kernels on another stream find room beside it; 0 = none */
 } clbm_params_t;
 
 typedef struct {
@@ -92,7 +97,9 @@ typedef struct {
   int32_t fixed_shift;       /* 0 = automatic; else scale 2^shift of the fixed-point sums */
   int32_t group;             /* markers per workgroup of ibm_compute_force: 4, 8 or 16
                                 (0 = 8); a tuning choice, results do not depend on it */
-  int32_t reserved;
+  int32_t check_region;      /* 1: ibm_compute_force checks that every stencil cell lies in
+                                region and marks IBM_REGION_FLAG otherwise */
+  int32_t region[6];         /* inclusive x0,y0,z0,x1,y1,z1 (see clbm_step_part) */
 } ibm_params_t;
 
 /* ---- library ---------------------------------------------------------- */
@@ -121,6 +128,28 @@ int clbm_init_feq(const clbm_grid_t* g, void* f, double rho0, double ux0, double
 int clbm_step(const clbm_grid_t* g, const clbm_params_t* p, void* f, int state,
               const float* fx, const float* fy, const float* fz, const int32_t* force_box,
               void* stream);
+
+/* The same step in two launches, so that the marker kernels can run while the
+ * lattice away from the body is updated.  part 0 steps every cell outside
+ * `region` (host int32[6], inclusive x0,y0,z0,x1,y1,z1) without reading any
+ * force; part 1 steps the cells inside it, with the force as clbm_step reads
+ * it.  Parts 0 then 1 with the same `state` equal one clbm_step, bit for bit,
+ * when no force falls outside the region; the state word advances after
+ * part 1.  Each cell's populations are read and written only by that cell's
+ * own update (AA pattern), so part 0 touches no population of a cell inside
+ * the region: readers of those (ibm_compute_force with velocity from f, whose
+ * stencils lie in the region) may run concurrently with part 0, on another
+ * stream.  `region` must be normalised by clbm_step_region. */
+int clbm_step_part(const clbm_grid_t* g, const clbm_params_t* p, void* f, int state,
+                   const float* fx, const float* fy, const float* fz, const int32_t* force_box,
+                   const int32_t* region, int part, void* stream);
+
+/* Host-only: clip `region` (int32[6]) to the grid and, when the inlet/outlet
+ * planes are on and it reaches x >= Nx-2, extend it to hold both x = Nx-2 and
+ * x = Nx-1 (the outlet cell is stepped with, and read through, its
+ * neighbour); result in `out`. */
+int clbm_step_region(const clbm_grid_t* g, const clbm_params_t* p, const int32_t* region,
+                     int32_t* out);
 
 /* Replaces CLBMSolver._macro_post (clbm_moving.py:305-321): force-free rho
  * and u of the current state into [z][y][x] float32 fields (each nullable). */

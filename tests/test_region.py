@@ -1,0 +1,188 @@
+"""The region a split step leaves to the marker kernels.
+
+CPU: the host prediction holds every stencil cell of the oracle's float32
+marker placement over many kinematics (including markers on cell faces and
+near the y and x boundaries); the C-ABI normalisation (host-only) keeps the
+outlet pair x = Nx-2, Nx-1 together.  GPU: the two parts of a split step
+equal one full step bit for bit, and a coupled run with the marker kernels on
+their own stream equals the same run on one stream bit for bit.
+"""
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import load_sub
+
+
+def _template(pkg, chord=80.0):
+    X, Y, ds = pkg.config.naca0012_arclength(chord, 1.0, 0.25)
+    return X, Y
+
+
+def test_hull_extremes_under_rotation(pkg):
+    X, Y = _template(pkg)
+    h = pkg.config.convex_hull_2d(X, Y)
+    rng = np.random.default_rng(3)
+    for th in rng.uniform(-np.pi, np.pi, 50):
+        R = X.astype(np.float64) * np.cos(th) - Y.astype(np.float64) * np.sin(th)
+        assert R.max() == R[h].max() and R.min() == R[h].min()
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_predicted_region_holds_every_stencil_cell(pkg, oracle, seed):
+    am = load_sub("airfoil_moving")
+    X0, Y0 = _template(pkg)
+    nx, ny, nz = 600, 200, 3
+    Z0 = np.full(len(X0), 1.0, np.float32)
+    h = pkg.config.convex_hull_2d(X0, Y0)
+    hx, hy = X0[h], Y0[h]
+    zr = am.stencil_z_range(Z0, nz)
+    rng = np.random.default_rng(seed)
+    for _ in range(200):
+        xc = float(rng.uniform(-10, nx + 10))
+        yc = float(rng.uniform(-10, ny + 10))
+        th = float(rng.uniform(-np.pi, np.pi))
+        if rng.random() < 0.3:       # integer pivot and round angles
+            xc, yc = float(np.round(xc)), float(np.round(yc))
+        r = am.stencil_region(hx, hy, zr, (nx, ny, nz), xc, yc, th)
+        X, Y, Z, *_ = oracle.update_markers(X0, Y0, Z0, xc, yc, th, 0, 0, 0)
+        base, ii, jj, kk, *_ = oracle.stencil(X, Y, Z, nx, ny, nz)
+        if r is None:       # the whole body is off the grid in x: no split, one stream
+            assert X.max() < 0 or X.min() > nx - 1
+            continue
+        x0, y0, z0, x1, y1, z1 = r
+        assert ii.min() >= x0 and ii.max() <= x1
+        assert jj.min() >= y0 and jj.max() <= y1
+        assert kk.min() >= z0 and kk.max() <= z1
+
+
+def test_z_range(pkg):
+    am = load_sub("airfoil_moving")
+    assert am.stencil_z_range(np.array([3.0, 4.5], np.float32), 10) == (2, 6)
+    assert am.stencil_z_range(np.array([0.0, 4.5], np.float32), 10) == (0, 9)   # wraps
+    assert am.stencil_z_range(np.array([7.0], np.float32), 10) == (6, 9)
+    assert am.stencil_z_range(np.array([7.5, 8.0], np.float32), 10) == (0, 9)
+
+
+def _region_call(lib, grid, params, r):
+    raw = (C.c_int32 * 6)(*r)
+    out = (C.c_int32 * 6)()
+    rc = lib.load().clbm_step_region(C.byref(grid), C.byref(params), raw, out)
+    return rc, list(out)
+
+
+def test_region_normalisation_keeps_outlet_pair(pkg):
+    lib = load_sub("_lib")
+    grid = lib.ClbmGrid(40, 20, 6, lib.STORE_F32)
+    inlet = lib.ClbmParams(1.6, 1.0, 0.05, 0.0, 1, 0)
+    periodic = lib.ClbmParams(1.6, 1.0, 0.05, 0.0, 0, 0)
+    assert _region_call(lib, grid, inlet, (5, 2, 0, 20, 10, 5)) == (0, [5, 2, 0, 20, 10, 5])
+    assert _region_call(lib, grid, inlet, (-3, -1, 0, 50, 30, 9)) == (0, [0, 0, 0, 39, 19, 5])
+    # reaching x = Nx-2 pulls in x = Nx-1; x = Nx-1 alone pulls in Nx-2
+    assert _region_call(lib, grid, inlet, (30, 2, 0, 38, 10, 5)) == (0, [30, 2, 0, 39, 10, 5])
+    assert _region_call(lib, grid, inlet, (39, 2, 0, 39, 10, 5)) == (0, [38, 2, 0, 39, 10, 5])
+    assert _region_call(lib, grid, periodic, (39, 2, 0, 39, 10, 5)) == (0, [39, 2, 0, 39, 10, 5])
+    assert _region_call(lib, grid, inlet, (10, 2, 0, 37, 10, 5))[1][3] == 37
+    assert _region_call(lib, grid, inlet, (20, 2, 0, 10, 10, 5))[0] == -1       # lo > hi
+    assert _region_call(lib, grid, inlet, (50, 2, 0, 60, 10, 5))[0] == -1       # outside
+
+
+# ---------------------------------------------------------------------------
+# GPU
+# ---------------------------------------------------------------------------
+def _dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(np.asarray(a, np.float32).transpose(2, 1, 0))).cuda()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("storage", ["f32", "f16", "f64"])
+@pytest.mark.parametrize("region", [(4, 3, 1, 12, 9, 4), (0, 0, 0, 5, 11, 5), (14, 2, 0, 23, 7, 5),
+                                    (9, 0, 2, 17, 11, 2)])
+def test_split_step_equals_full_step(pkg, oracle, gpu, storage, region):
+    import torch
+    lib = load_sub("_lib")
+    nx, ny, nz = 24, 12, 6
+    cfg = pkg.config.CaseConfig(name="split", Nx=nx, Ny=ny, Nz=nz, U_inf=0.05, Re=200, chord=10,
+                                s_v=1 / 0.56, ux_in=0.05, uy_in=-0.007)
+    a = pkg.CLBMSolver(cfg, storage=storage)
+    b = pkg.CLBMSolver(cfg, storage=storage)
+    rng = np.random.default_rng(11)
+    f, *_ = oracle.init_feq(nx, ny, nz, cfg.ux_in, cfg.uy_in)
+    f = f * (1 + 0.02 * rng.standard_normal(f.shape))
+    a.f.from_numpy(f)
+    b.f.from_numpy(f)
+    out = (C.c_int32 * 6)()
+    assert lib.load().clbm_step_region(C.byref(a.grid), C.byref(a.params), (C.c_int32 * 6)(*region),
+                                       out) == 0
+    x0, y0, z0, x1, y1, z1 = list(out)
+    F = np.zeros((nx, ny, nz), np.float32)
+    F[x0:x1 + 1, y0:y1 + 1, z0:z1 + 1] = 1e-3 * rng.standard_normal(
+        (x1 - x0 + 1, y1 - y0 + 1, z1 - z0 + 1))
+    fx, fy = _dev(F), _dev(-0.5 * F)
+    st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    for _ in range(4):
+        a.step(fx, fy)
+        for part in (0, 1):
+            lib.call("clbm_step_part", C.byref(b.grid), C.byref(b.params),
+                     C.c_void_p(b.state.data_ptr()), b.state_word, C.c_void_p(fx.data_ptr()),
+                     C.c_void_p(fy.data_ptr()), None, None, out, part, st)
+        b.state_word = lib.load().clbm_state_after_step(b.state_word, C.byref(b.params))
+        b._macro_valid = False
+    np.testing.assert_array_equal(a.f.to_numpy(), b.f.to_numpy())
+
+
+@pytest.mark.gpu
+def test_step_part_rejects_unnormalised_region(pkg, gpu):
+    lib = load_sub("_lib")
+    cfg = pkg.config.CaseConfig(name="split", Nx=24, Ny=12, Nz=6, U_inf=0.05, Re=200, chord=10,
+                                s_v=1.6, ux_in=0.05, uy_in=0.0)
+    a = pkg.CLBMSolver(cfg)
+    bad = (C.c_int32 * 6)(20, 0, 0, 22, 5, 5)   # reaches Nx-2 without Nx-1
+    with pytest.raises(lib.ClbmError, match="normalised"):
+        lib.call("clbm_step_part", C.byref(a.grid), C.byref(a.params),
+                 C.c_void_p(a.state.data_ptr()), a.state_word, None, None, None, None, bad, 0, None)
+
+
+def _coupled(pkg, cfg, steps, overlap, spread="fixed"):
+    lbm = pkg.CLBMSolver(cfg)
+    ibm = pkg.DroneMovingIBM(cfg, overlap=overlap, spread_mode=spread)
+    split = 0
+    for n in range(1, steps + 1):
+        fx, fy, fz = ibm.compute_force(lbm.ux, lbm.uy, *cfg.motion_state(n))
+        lbm.step(fx, fy, fz)
+        split += lbm.last_kernel_cells < lbm.ncell
+        ibm.record_force()
+    return lbm, ibm, split
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("where", ["centre", "outlet", "y-seam"])
+def test_overlapped_run_equals_single_stream_run(pkg, gpu, where):
+    nx, ny, nz = 64, 32, 4
+    kw = dict(le_x=20.0)
+    if where == "outlet":
+        kw = dict(le_x=nx - 10.0)      # the trailing stencils reach x = Nx-1
+    cfg = pkg.config.naca_case(nx, ny, nz, 12.0, name=where, **kw)
+    if where == "y-seam":
+        cfg = cfg.with_(y0=1.0)        # the body straddles the periodic y boundary
+    a_lbm, a_ibm, split = _coupled(pkg, cfg, 12, True)
+    b_lbm, b_ibm, nosplit = _coupled(pkg, cfg, 12, False)
+    assert nosplit == 0 and split == 12
+    for name in ("rho", "ux", "uy", "uz"):
+        np.testing.assert_array_equal(getattr(a_lbm, name).to_numpy(),
+                                      getattr(b_lbm, name).to_numpy(), err_msg=name)
+    np.testing.assert_array_equal(a_ibm.force_history(), b_ibm.force_history())
+    np.testing.assert_array_equal(a_ibm.FX_k.to_numpy(), b_ibm.FX_k.to_numpy())
+    assert a_ibm.region_violations() == 0
+
+
+@pytest.mark.gpu
+def test_overlap_splits_the_benchmark_shape(pkg, gpu):
+    """On a body small against the lattice every step takes the split path."""
+    cfg = pkg.config.naca_case(96, 48, 4, 12.0, name="small")
+    lbm, ibm, split = _coupled(pkg, cfg, 6, True)
+    assert split == 6
+    assert ibm.region_violations() == 0
