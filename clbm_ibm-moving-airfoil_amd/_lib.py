@@ -22,11 +22,12 @@ BOX_SUBSLOTS = 64
 BOX_WORDS = 2 * 6 * BOX_SUBSLOTS + 4        # uint64 words of a force-box record
 FORCE_BOX_OFFSET = 2 * 6 * BOX_SUBSLOTS     # word offset of the decoded int32[6] box
 REGION_FLAG_OFFSET = 2 * 6 * BOX_SUBSLOTS + 3   # word offset of the region-violation flag
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 
 class ClbmGrid(C.Structure):
-    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("storage", C.c_int32)]
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("storage", C.c_int32),
+                ("dev_ux", C.c_float), ("dev_uy", C.c_float)]
 
 
 class ClbmParams(C.Structure):
@@ -38,6 +39,26 @@ class IbmParams(C.Structure):
     _fields_ = [("f_max", C.c_float), ("wrap_weight_quirk", C.c_int32), ("spread_mode", C.c_int32),
                 ("fixed_shift", C.c_int32), ("group", C.c_int32), ("check_region", C.c_int32),
                 ("region", C.c_int32 * 6)]
+
+
+class ClbmBatch(C.Structure):
+    _fields_ = [("n", C.c_int32), ("storage", C.c_int32), ("block", C.c_int32),
+                ("blocks", C.c_int32), ("args", C.c_void_p), ("inst", C.c_void_p),
+                ("first", C.c_void_p)]
+
+
+class IbmBuffers(C.Structure):
+    _fields_ = [("n", C.c_int32), ("order", C.c_void_p)] + [
+        (k, C.c_void_p) for k in ("X0", "Y0", "Z0", "X", "Y", "Z", "VX", "VY", "U", "V", "FX",
+                                  "FY", "acc_x", "acc_y", "box", "fx", "fy", "kin")] + [
+        ("gen0", C.c_uint32)]
+
+
+class IbmBatch(C.Structure):
+    _fields_ = [("n", C.c_int32), ("storage", C.c_int32), ("spread_mode", C.c_int32),
+                ("group", C.c_int32), ("blocks", C.c_int32), ("clip_blocks", C.c_int32),
+                ("args", C.c_void_p), ("clip_args", C.c_void_p), ("inst", C.c_void_p),
+                ("first", C.c_void_p), ("counter", C.c_void_p)]
 
 
 _P = C.c_void_p
@@ -72,6 +93,15 @@ _SIGNATURES = {
     "ibm_spread": (_I, [_G, _IP, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
     "ibm_clip_forces": (_I, [_G, _IP, _P, _P, _P, _P, _P, C.c_uint32, _P]),
     "ibm_force_sum_f64": (_I, [_I, _P, _P, _P, _P, _P]),
+    "clbm_batch_bytes": (C.c_size_t, [_I, _G, _I]),
+    "clbm_batch_build": (_I, [_I, _G, _LP, _P, _P, _P, _P, _I, _P, C.c_size_t,
+                              C.POINTER(ClbmBatch)]),
+    "clbm_batch_step": (_I, [C.POINTER(ClbmBatch), _I, _P]),
+    "ibm_batch_bytes": (C.c_size_t, [_I, C.POINTER(IbmBuffers), _I]),
+    "ibm_batch_build": (_I, [_I, _G, _LP, _P, _IP, C.POINTER(IbmBuffers), _P, C.c_size_t,
+                             C.POINTER(IbmBatch)]),
+    "ibm_batch_compute": (_I, [C.POINTER(IbmBatch), _I, _P]),
+    "ibm_batch_advance": (_I, [C.POINTER(IbmBatch), _P]),
 }
 STATE_PARITY, STATE_PLANES = 1, 2
 

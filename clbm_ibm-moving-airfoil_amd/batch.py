@@ -1,0 +1,350 @@
+"""Batched sweeps: many independent moving-airfoil instances advanced together.
+
+No reference counterpart: the reference runs one case per process
+(main_moving.py).  A sweep holds n instances, each a CLBMSolver and a
+DroneMovingIBM of its own grid, rates and kinematics.  All of them advance
+in lockstep, four launches per step for the whole batch:
+
+* ``ibm_batch_compute``: the marker kernel and the clamp of every body;
+* ``clbm_batch_step``: the fused step of every lattice;
+* ``ibm_batch_advance``: bump the device step counter.
+
+Kinematics come from per-instance device tables indexed by that counter.
+The step sequence therefore needs no host argument and is captured once as a
+graph of two steps (both AA parities), which is replayed.  Every instance
+computes exactly what it computes when run on its own (fixed-point
+spreading), which ``sequential`` checks and times.
+
+``autotune`` picks the lattice workgroup size and the markers per workgroup
+by timing the candidates on the batch itself, then restores the initial
+state.  The choice is cached per batch signature in a JSON file.
+
+Usage: python -m <package>.batch [--instances 64] [--steps 5000] [--seed 0] [--json out.json]
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import hashlib
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import _lib
+from . import config as cfgmod
+from .airfoil_moving import DroneMovingIBM
+from .clbm_moving import CLBMSolver
+
+GRIDS = [(200, 64, 16), (300, 100, 24), (450, 150, 36), (600, 200, 50)]
+CANDIDATES = [(256, 8), (256, 4), (128, 8), (128, 4)]   # (lattice block, markers per group)
+TUNE_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "clbm_amd_batch_tune.json")
+
+
+def draw_instances(n: int, seed: int = 0, grids=GRIDS):
+    """Seeded instances: a grid from ``grids``, chord Ny/2.5, Re in [100, 400],
+    heave in [0.1c, 0.5c], pitch amplitude in [0, 20] deg, phase in [0, 180] deg,
+    reduced frequency in [0.25, 1], mean angle of attack in [0, 12] deg."""
+    out = []
+    for i in range(n):
+        rng = np.random.default_rng(seed * 100003 + i)
+        nx, ny, nz = grids[int(rng.integers(len(grids)))]
+        chord = ny / 2.5
+        out.append(cfgmod.naca_case(
+            nx, ny, nz, chord, Re=float(rng.uniform(100, 400)), k=float(rng.uniform(0.25, 1.0)),
+            heave_frac=float(rng.uniform(0.1, 0.5)), aoa_mean_deg=float(rng.uniform(0, 12)),
+            pitch_amp_deg=float(rng.uniform(0, 20)), phase_deg=float(rng.uniform(0, 180)),
+            le_x=nx / 4.0, name=f"inst{i}-{nx}x{ny}x{nz}"))
+    return out
+
+
+def kinematics_table(cfg, first_step: int, steps: int) -> np.ndarray:
+    """(steps, 7) float32: xc, yc, cos, sin, vx, vy, omega for steps first_step,
+    first_step+1, ...  Each entry is what ibm_compute_force receives for that
+    step: float32 arguments, cos/sin of the float32 angle in float64 rounded once."""
+    t = np.zeros((steps, 7), dtype=np.float32)
+    for j in range(steps):
+        xc, yc, th, vx, vy, om = cfg.motion_state(first_step + j)
+        a = np.float64(np.float32(th))
+        t[j] = (np.float32(xc), np.float32(yc), np.float32(math.cos(a)), np.float32(math.sin(a)),
+                np.float32(vx), np.float32(vy), np.float32(om))
+    return t
+
+
+class BatchSweep:
+    """n instances on one GPU, stepped together (see the module docstring)."""
+
+    def __init__(self, cfgs, storage: str = "f32", device=None, steps: int = 1000,
+                 block: int = 256, group: int = 8, first_step: int = 1):
+        import torch
+        self.cfgs = list(cfgs)
+        self.device = torch.device(device if device is not None else "cuda")
+        self.storage = storage
+        self.first_step = first_step
+        self.capacity = int(steps)
+        self.lbms = [CLBMSolver(c, storage=storage, device=self.device) for c in self.cfgs]
+        self.ibms = [DroneMovingIBM(c, device=self.device, overlap=False) for c in self.cfgs]
+        self._kin = [torch.from_numpy(kinematics_table(c, first_step, self.capacity)).to(self.device)
+                     for c in self.cfgs]
+        self.steps_done = 0
+        self.state = 0
+        self._graph = None
+        self.build(block, group)
+
+    # -- tables ---------------------------------------------------------------
+    def build(self, block: int, group: int) -> None:
+        import torch
+        n = len(self.cfgs)
+        self.block, self.group = int(block), int(group)
+        grids = (_lib.ClbmGrid * n)(*[l.grid for l in self.lbms])
+        params = (_lib.ClbmParams * n)(*[l.params for l in self.lbms])
+        f = (C.c_void_p * n)(*[l.state.data_ptr() for l in self.lbms])
+        fx = (C.c_void_p * n)(*[i.fx._t.data_ptr() for i in self.ibms])
+        fy = (C.c_void_p * n)(*[i.fy._t.data_ptr() for i in self.ibms])
+        boxes = (C.c_void_p * n)(*[i.box.data_ptr() for i in self.ibms])
+        lib = _lib.load()
+        nb = lib.clbm_batch_bytes(n, grids, self.block)
+        if nb == 0:
+            raise _lib.ClbmError(lib.clbm_last_error().decode())
+        self._lwork = torch.empty(nb, dtype=torch.uint8, device=self.device)
+        self.lb = _lib.ClbmBatch()
+        torch.cuda.synchronize(self.device)
+        _lib.call("clbm_batch_build", n, grids, params, f, fx, fy, boxes, self.block,
+                  C.c_void_p(self._lwork.data_ptr()), nb, C.byref(self.lb))
+        bufs = (_lib.IbmBuffers * n)()
+        for k, ibm in enumerate(self.ibms):
+            b = bufs[k]
+            b.n = ibm.N
+            b.order = ibm.order.data_ptr() if ibm.order is not None else None
+            for name, t in (("X0", ibm.X0), ("Y0", ibm.Y0), ("Z0", ibm.Z0), ("X", ibm.X_marker),
+                            ("Y", ibm.Y_marker), ("Z", ibm.Z_marker), ("VX", ibm.VX_body),
+                            ("VY", ibm.VY_body), ("U", ibm.U_k), ("V", ibm.V_k),
+                            ("FX", ibm.FX_k), ("FY", ibm.FY_k)):
+                setattr(b, name, t._t.data_ptr())
+            b.acc_x, b.acc_y = ibm._acc[0].data_ptr(), ibm._acc[1].data_ptr()
+            b.box = ibm.box.data_ptr()
+            b.fx, b.fy = ibm.fx._t.data_ptr(), ibm.fy._t.data_ptr()
+            b.kin = self._kin[k].data_ptr()
+            b.gen0 = ibm.gen - self.steps_done    # the generation of counter value c is gen0 + 1 + c
+        ip = _lib.IbmParams(float(self.ibms[0].ip.f_max), int(self.ibms[0].ip.wrap_weight_quirk),
+                            int(self.ibms[0].ip.spread_mode), 0, self.group, 0)
+        for ibm in self.ibms[1:]:
+            if (ibm.ip.f_max, ibm.ip.wrap_weight_quirk, ibm.ip.spread_mode) != \
+                    (ip.f_max, ip.wrap_weight_quirk, ip.spread_mode):
+                raise ValueError("instances of a batch share the spread mode, clamp and quirk")
+        ib = lib.ibm_batch_bytes(n, bufs, self.group)
+        if ib == 0:
+            raise _lib.ClbmError(lib.clbm_last_error().decode())
+        self._iwork = torch.empty(ib, dtype=torch.uint8, device=self.device)
+        self.ib = _lib.IbmBatch()
+        _lib.call("ibm_batch_build", n, grids, params, f, C.byref(ip), bufs,
+                  C.c_void_p(self._iwork.data_ptr()), ib, C.byref(self.ib))
+        # the counter starts at 0: move it to the steps already taken
+        self._iwork.view(torch.int32)[(self.ib.counter - self._iwork.data_ptr()) // 4] = \
+            self.steps_done
+        self._graph = None
+
+    # -- stepping -------------------------------------------------------------
+    def _launch_step(self, state: int, stream) -> int:
+        _lib.call("ibm_batch_compute", C.byref(self.ib), state, stream)
+        _lib.call("clbm_batch_step", C.byref(self.lb), state, stream)
+        _lib.call("ibm_batch_advance", C.byref(self.ib), stream)
+        return _lib.load().clbm_state_after_step(state, C.byref(self.lbms[0].params))
+
+    def _advance_objects(self, k: int) -> None:
+        """Keep the solver/IBM objects consistent with k batched steps."""
+        self.steps_done += k
+        for l in self.lbms:
+            l.state_word = self.state
+            l.steps += k
+            l._macro_valid = False
+        for i in self.ibms:
+            i.gen += k
+
+    def step(self, k: int = 1) -> None:
+        """k eager steps of the whole batch."""
+        import torch
+        if self.steps_done + k > self.capacity:
+            raise ValueError("kinematics table exhausted; build the sweep with more steps")
+        st = C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+        for _ in range(k):
+            self.state = self._launch_step(self.state, st)
+            self._advance_objects(1)
+
+    def run(self, k: int, graph: bool = True) -> None:
+        """k steps; with ``graph``, steps after the first go through a captured
+        two-step graph (the inlet/outlet planes are pending from step 2 on, so
+        the state word cycles with period two)."""
+        import torch
+        if not graph:
+            self.step(k)
+            return
+        if self.steps_done + k > self.capacity:
+            raise ValueError("kinematics table exhausted; build the sweep with more steps")
+        if self.steps_done == 0 and k > 0:
+            self.step(1)
+            k -= 1
+        if k >= 2 and self._graph is not None and self._graph_state != self.state:
+            self.step(1)      # the state word alternates; realign with the captured pair
+            k -= 1
+        if k >= 2 and self._graph is None:
+            self._capture()
+        while k >= 2:
+            self._graph.replay()
+            s = self.state
+            for _ in range(2):
+                s = _lib.load().clbm_state_after_step(s, C.byref(self.lbms[0].params))
+            self.state = s
+            self._advance_objects(2)
+            k -= 2
+        if k:
+            self.step(k)
+
+    def _capture(self) -> None:
+        import torch
+        g = torch.cuda.CUDAGraph()
+        side = torch.cuda.Stream(device=self.device)
+        side.wait_stream(torch.cuda.current_stream(self.device))
+        with torch.cuda.stream(side):
+            with torch.cuda.graph(g, stream=side):
+                st = C.c_void_p(side.cuda_stream)
+                s = self._launch_step(self.state, st)
+                self._launch_step(s, st)
+        torch.cuda.current_stream(self.device).wait_stream(side)
+        self._graph = g
+        self._graph_state = self.state
+
+    # -- tuning and reference timings -------------------------------------------
+    def signature(self) -> str:
+        key = json.dumps([[c.Nx, c.Ny, c.Nz, c.N_MARKERS] for c in self.cfgs] + [self.storage])
+        return hashlib.sha1(key.encode()).hexdigest()[:16]
+
+    def reset(self) -> None:
+        """Back to the initial state (equilibrium, step counter 0)."""
+        import torch
+        for l in self.lbms:
+            l._init_feq()
+            l.steps = 0
+        for i in self.ibms:
+            i.gen += 2     # generations stay increasing; the stale boxes are cleared by clip
+        self.steps_done = 0
+        self.state = 0
+        self.build(self.block, self.group)
+        torch.cuda.synchronize(self.device)
+
+    def autotune(self, steps: int = 6, cache: str = TUNE_CACHE, use_cache: bool = True):
+        """Time each (block, group) candidate over ``steps`` eager steps, keep the
+        fastest, restore the initial state; returns {candidate: ms per step}."""
+        import torch
+        sig = self.signature()
+        table = {}
+        if use_cache and cache and os.path.exists(cache):
+            try:
+                with open(cache) as fh:
+                    table = json.load(fh)
+            except (OSError, ValueError):
+                table = {}
+        if sig in table:
+            block, group = table[sig]["best"]
+            self.build(block, group)
+            return table[sig]["times"]
+        times = {}
+        for block, group in CANDIDATES:
+            self.build(block, group)
+            self.step(1)                      # warm-up (first-use costs)
+            torch.cuda.synchronize(self.device)
+            t0 = time.perf_counter()
+            self.step(steps)
+            torch.cuda.synchronize(self.device)
+            times[f"{block}/{group}"] = (time.perf_counter() - t0) / steps * 1e3
+            self.reset()
+        best = min(times, key=times.get)
+        block, group = (int(v) for v in best.split("/"))
+        self.build(block, group)
+        table[sig] = {"best": [block, group], "times": times}
+        if cache:
+            try:
+                os.makedirs(os.path.dirname(cache), exist_ok=True)
+                with open(cache, "w") as fh:
+                    json.dump(table, fh)
+            except OSError:
+                pass
+        return times
+
+    @property
+    def cells(self) -> int:
+        return sum(c.Nx * c.Ny * c.Nz for c in self.cfgs)
+
+
+def sequential(cfgs, steps: int, storage: str = "f32", device=None, first_step: int = 1):
+    """Each instance on its own (its CLBMSolver and DroneMovingIBM with the
+    usual per-instance launches), one after the other; returns the objects and
+    the elapsed seconds."""
+    import torch
+    objs, total = [], 0.0
+    for c in cfgs:
+        lbm = CLBMSolver(c, storage=storage, device=device)
+        ibm = DroneMovingIBM(c, device=device, overlap=False)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for n in range(first_step, first_step + steps):
+            fx, fy, fz = ibm.compute_force(lbm.ux, lbm.uy, *c.motion_state(n))
+            lbm.step(fx, fy, fz)
+        torch.cuda.synchronize()
+        total += time.perf_counter() - t0
+        objs.append((lbm, ibm))
+    return objs, total
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--instances", type=int, default=64)
+    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16", "f64"])
+    ap.add_argument("--sequential-steps", type=int, default=200,
+                    help="steps of the per-instance sequential timing (0 = skip)")
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--json", default=None)
+    a = ap.parse_args(argv)
+    import torch
+    cfgs = draw_instances(a.instances, a.seed)
+    sweep = BatchSweep(cfgs, storage=a.storage, steps=a.steps + 64)
+    tune = None if a.no_tune else sweep.autotune()
+    print(f"[batch] {len(cfgs)} instances, {sweep.cells / 1e6:.1f} M cells, block {sweep.block}, "
+          f"group {sweep.group}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    done = 0
+    while done < a.steps:
+        k = min(500, a.steps - done)
+        sweep.run(k)
+        done += k
+        torch.cuda.synchronize()
+        print(f"[batch] {done} steps", file=sys.stderr, flush=True)
+    dt = time.perf_counter() - t0
+    res = {"instances": len(cfgs), "cells": sweep.cells, "steps": a.steps, "storage": a.storage,
+           "block": sweep.block, "group": sweep.group, "tuning_ms_per_step": tune,
+           "seconds": dt, "mlups": sweep.cells * a.steps / dt / 1e6,
+           "finite": all(l.health()[1] for l in sweep.lbms)}
+    if a.sequential_steps:
+        del sweep
+        torch.cuda.empty_cache()
+        _, ts = sequential(cfgs, a.sequential_steps, a.storage)
+        res["sequential_steps"] = a.sequential_steps
+        res["sequential_mlups"] = sum(c.Nx * c.Ny * c.Nz for c in cfgs) * a.sequential_steps / ts / 1e6
+        res["speedup_vs_sequential"] = res["mlups"] / res["sequential_mlups"]
+    text = json.dumps(res, indent=1)
+    if a.json:
+        with open(a.json, "w") as fh:
+            fh.write(text)
+    print(text, flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

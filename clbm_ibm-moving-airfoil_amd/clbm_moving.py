@@ -97,10 +97,13 @@ class CLBMSolver:
     ``inlet`` False for an x-periodic box, ``outside_lds`` the LDS bytes per
     workgroup the outside part of a split step reserves (a tuning choice:
     33 KiB holds that launch to 4 workgroups per CU, which leaves each SIMD
-    room for one concurrent marker wavefront)."""
+    room for one concurrent marker wavefront), ``deviation`` what 16-bit
+    storage subtracts before rounding: "weights" f_q - w_q, or "inflow"
+    f_q - feq_q(1, inlet velocity), which is near zero wherever the flow is
+    near the free stream."""
 
     def __init__(self, config=None, storage: str = "f32", device=None, inlet: bool = True,
-                 outside_lds: int = 33792):
+                 outside_lds: int = 33792, deviation: str = "weights"):
         import torch
         cfg: CaseConfig = resolve(config)
         self.cfg = cfg
@@ -110,7 +113,11 @@ class CLBMSolver:
             raise ValueError(f"storage must be one of {sorted(_lib.STORAGE)}")
         self.storage = storage
         _lib.load()
-        self.grid = _lib.ClbmGrid(self.Nx, self.Ny, self.Nz, _lib.STORAGE[storage])
+        if deviation not in ("weights", "inflow"):
+            raise ValueError("deviation must be 'weights' or 'inflow'")
+        self.deviation = deviation
+        du, dv = (float(cfg.ux_in), float(cfg.uy_in)) if deviation == "inflow" else (0.0, 0.0)
+        self.grid = _lib.ClbmGrid(self.Nx, self.Ny, self.Nz, _lib.STORAGE[storage], du, dv)
         self.params = _lib.ClbmParams(float(cfg.s_v), float(cfg.s_e), float(cfg.ux_in),
                                       float(cfg.uy_in), 1 if inlet else 0, int(outside_lds))
         nbytes = Q * self.Nx * self.Ny * self.Nz * _lib.ELEM_BYTES[self.grid.storage]

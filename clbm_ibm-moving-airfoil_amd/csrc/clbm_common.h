@@ -40,32 +40,39 @@ __host__ __device__ constexpr R wq(int q) {
 
 // ---------------------------------------------------------------------------
 // Population storage.  Each format maps a stored element to the compute type
-// R and back.  16-bit formats hold the deviation f - w.
+// R and back.
 // ---------------------------------------------------------------------------
 template <int S> struct Store;
+// ld/st take the population's deviation base: 16-bit formats store f - base
+// (base = the equilibrium at the grid's deviation velocity, the lattice
+// weight when that is zero); the wide formats ignore it.
 template <> struct Store<CLBM_STORE_F32> {
   using T = float;
   using R = float;
-  __device__ static R ld(const T* p, int) { return *p; }
-  __device__ static void st(T* p, int, R v) { *p = v; }
+  static constexpr bool dev = false;
+  __device__ static R ld(const T* p, float) { return *p; }
+  __device__ static void st(T* p, R v, float) { *p = v; }
 };
 template <> struct Store<CLBM_STORE_F64> {
   using T = double;
   using R = double;
-  __device__ static R ld(const T* p, int) { return *p; }
-  __device__ static void st(T* p, int, R v) { *p = v; }
+  static constexpr bool dev = false;
+  __device__ static R ld(const T* p, float) { return *p; }
+  __device__ static void st(T* p, R v, float) { *p = v; }
 };
 template <> struct Store<CLBM_STORE_F16> {
   using T = __half;
   using R = float;
-  __device__ static R ld(const T* p, int q) { return wq<float>(q) + __half2float(*p); }
-  __device__ static void st(T* p, int q, R v) { *p = __float2half_rn(v - wq<float>(q)); }
+  static constexpr bool dev = true;
+  __device__ static R ld(const T* p, float b) { return b + __half2float(*p); }
+  __device__ static void st(T* p, R v, float b) { *p = __float2half_rn(v - b); }
 };
 template <> struct Store<CLBM_STORE_BF16> {
   using T = __hip_bfloat16;
   using R = float;
-  __device__ static R ld(const T* p, int q) { return wq<float>(q) + __bfloat162float(*p); }
-  __device__ static void st(T* p, int q, R v) { *p = __float2bfloat16(v - wq<float>(q)); }
+  static constexpr bool dev = true;
+  __device__ static R ld(const T* p, float b) { return b + __bfloat162float(*p); }
+  __device__ static void st(T* p, R v, float b) { *p = __float2bfloat16(v - b); }
 };
 
 inline size_t elem_bytes(int storage) {
@@ -102,6 +109,7 @@ struct Geo {
   int ncell;      // nx*ny*nz
   FastDiv dx;     // divide by nx
   FastDiv dy;     // divide by ny
+  float dev[Q];   // deviation base of 16-bit storage per direction
 };
 inline Geo make_geo(const clbm_grid_t& g) {
   Geo o;
@@ -112,8 +120,30 @@ inline Geo make_geo(const clbm_grid_t& g) {
   o.ncell = g.nx * g.ny * g.nz;
   o.dx = make_fastdiv((uint32_t)g.nx);
   o.dy = make_fastdiv((uint32_t)g.ny);
+  // second-order equilibrium at rho = 1 and the deviation velocity, in double,
+  // rounded once (= the lattice weights for a zero velocity)
+  const double u = g.dev_ux, v = g.dev_uy, u2 = u * u + v * v;
+  for (int q = 0; q < Q; ++q) {
+    const double eu = ex(q) * u + ey(q) * v;
+    o.dev[q] = (float)(wq<double>(q) * (1.0 + 3.0 * eu + 4.5 * eu * eu - 1.5 * u2));
+  }
   return o;
 }
+
+// The deviation bases of a Geo as values (uniform across the launch); empty
+// for the wide formats.
+template <int S, bool D = Store<S>::dev> struct DevBase {
+  __device__ __forceinline__ explicit DevBase(const Geo&) {}
+  __device__ __forceinline__ float operator[](int) const { return 0.0f; }
+};
+template <int S> struct DevBase<S, true> {
+  float v[Q];
+  __device__ __forceinline__ explicit DevBase(const Geo& g) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) v[q] = g.dev[q];
+  }
+  __device__ __forceinline__ float operator[](int q) const { return v[q]; }
+};
 
 // Linear offsets from a cell to its neighbours along +-x/y/z with periodic wrap.
 struct Nbr {
@@ -194,10 +224,13 @@ __device__ __forceinline__ size_t slot_out(int q, size_t N, int c, const Nbr& nb
 }
 
 template <int S, int PAR>
-__device__ __forceinline__ void load_state(const typename Store<S>::T* __restrict__ f, size_t N,
-                                           int c, const Nbr& nb, typename Store<S>::R* fq) {
+__device__ __forceinline__ void load_state(const typename Store<S>::T* __restrict__ f,
+                                           const Geo& g, int c, const Nbr& nb,
+                                           typename Store<S>::R* fq) {
+  const DevBase<S> dv(g);
+  const size_t N = (size_t)g.ncell;
 #pragma unroll
-  for (int q = 0; q < Q; ++q) fq[q] = Store<S>::ld(f + slot_in<PAR>(q, N, c, nb), q);
+  for (int q = 0; q < Q; ++q) fq[q] = Store<S>::ld(f + slot_in<PAR>(q, N, c, nb), dv[q]);
 }
 
 // ---------------------------------------------------------------------------
@@ -208,11 +241,13 @@ template <int S> struct Pop {
   using R = typename Store<S>::R;
   T* f;
   size_t N;
+  DevBase<S> dv;
+  __device__ __forceinline__ Pop(T* f_, const Geo& g) : f(f_), N((size_t)g.ncell), dv(g) {}
   __device__ __forceinline__ R ld(int plane, int cell, int q) const {
-    return Store<S>::ld(f + (size_t)plane * N + cell, q);
+    return Store<S>::ld(f + (size_t)plane * N + cell, dv[q]);
   }
   __device__ __forceinline__ void st(int plane, int cell, int q, R v) const {
-    Store<S>::st(f + (size_t)plane * N + cell, q, v);
+    Store<S>::st(f + (size_t)plane * N + cell, v, dv[q]);
   }
 };
 
@@ -288,7 +323,7 @@ __device__ __forceinline__ void load_cell(const typename Store<S>::T* __restrict
     x -= 1;
   }
   const Nbr nb = make_nbr(g, x, y, z);
-  load_state<S, PAR>(f, (size_t)g.ncell, c, nb, a);
+  load_state<S, PAR>(f, g, c, nb, a);
   if (pl.on && x == 0) inlet_rule<R>(a, R(pl.ux_in), R(pl.uy_in));
 }
 

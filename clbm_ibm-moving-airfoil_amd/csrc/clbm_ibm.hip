@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <vector>
 
 namespace clbm {
 
@@ -55,9 +56,9 @@ __device__ __forceinline__ int pmod(int a, int n) {
 // serialise, so spreading the workgroups' maxima over separate words keeps
 // them off the critical path; readers take the maximum over the copies.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void box_grow(uint64_t* box, uint32_t gen, const int* lo, const int* hi) {
-  uint64_t* slot =
-      box + 6 * ((gen & 1u) * IBM_BOX_SUBSLOTS + (blockIdx.x & (IBM_BOX_SUBSLOTS - 1)));
+__device__ __forceinline__ void box_grow(uint64_t* box, uint32_t gen, const int* lo, const int* hi,
+                                         int sub) {
+  uint64_t* slot = box + 6 * ((gen & 1u) * IBM_BOX_SUBSLOTS + (sub & (IBM_BOX_SUBSLOTS - 1)));
   const uint64_t g = (uint64_t)gen << 32;
   for (int a = 0; a < 3; ++a) {
     if (lo[a] > hi[a]) continue;
@@ -171,7 +172,7 @@ __global__ __launch_bounds__(IBLK) void k_update_markers(
         sb[0][a] = min(sb[0][a], sb[i][a]);
         sb[0][3 + a] = max(sb[0][3 + a], sb[i][3 + a]);
       }
-    box_grow(box, gen, &sb[0][0], &sb[0][3]);
+    box_grow(box, gen, &sb[0][0], &sb[0][3], blockIdx.x);
   }
 }
 
@@ -284,25 +285,53 @@ struct RegionCheck {
   int lo[3], hi[3];
 };
 
+// Everything the marker kernel needs for one body on one lattice.
+struct IbmArgs {
+  Geo g;
+  Planes pl;
+  int quirk;
+  float f_max, fscale;
+  const void* f;
+  const float *uxf, *uyf;
+  int n;
+  const int32_t* order;
+  const float *X0, *Y0, *Z0;
+  float *X, *Y, *Z, *VX, *VY, *U, *V, *FX, *FY;
+  int32_t* base;
+  void *accx, *accy;
+  uint64_t* box;
+  RegionCheck rc;
+  const float* kin;  // batched runs: per step xc, yc, cos, sin, vx, vy, omega
+  uint32_t gen0;     // batched runs: generation of step 0 of the step counter, minus one
+};
+
+// Marker group `grp` (G markers, one wavefront each) of generation gen;
+// `sub` picks the force-box copy this workgroup updates.
 template <int S, int UMODE, int SMODE, int G>
-__global__ __launch_bounds__(64 * G) void k_compute_force(
-    Geo g, Planes pl, int quirk, float f_max, float fscale, const typename Store<S>::T* __restrict__ f,
-    const float* __restrict__ uxf, const float* __restrict__ uyf, int n,
-    const int32_t* __restrict__ order, const float* __restrict__ X0, const float* __restrict__ Y0,
-    const float* __restrict__ Z0, Kin kin, float* __restrict__ X, float* __restrict__ Y,
-    float* __restrict__ Z, float* __restrict__ VX, float* __restrict__ VY, float* __restrict__ U,
-    float* __restrict__ V, float* __restrict__ FX, float* __restrict__ FY, int32_t* __restrict__ base,
-    void* __restrict__ accx, void* __restrict__ accy, uint64_t* __restrict__ box, uint32_t gen,
-    RegionCheck rc) {
+__device__ __forceinline__ void compute_group(const IbmArgs& a, const Kin& kin, uint32_t gen,
+                                              int grp, int sub) {
+  const Geo& g = a.g;
+  const Planes& pl = a.pl;
+  const int quirk = a.quirk, n = a.n;
+  const float f_max = a.f_max, fscale = a.fscale;
+  const auto* __restrict__ f = (const typename Store<S>::T*)a.f;
+  const float *__restrict__ uxf = a.uxf, *__restrict__ uyf = a.uyf;
+  const int32_t* __restrict__ order = a.order;
+  const float *__restrict__ X0 = a.X0, *__restrict__ Y0 = a.Y0, *__restrict__ Z0 = a.Z0;
+  float *__restrict__ X = a.X, *__restrict__ Y = a.Y, *__restrict__ Z = a.Z;
+  float *__restrict__ VX = a.VX, *__restrict__ VY = a.VY, *__restrict__ U = a.U, *__restrict__ V = a.V;
+  float *__restrict__ FX = a.FX, *__restrict__ FY = a.FY;
+  int32_t* __restrict__ base = a.base;
+  void *accx = a.accx, *accy = a.accy;
+  uint64_t* box = a.box;
+  const RegionCheck& rc = a.rc;
   constexpr int NT = Group<G>::threads, TC = Group<G>::tile;
   __shared__ unsigned long long sacc[2 * TC];  // spread sums (x then y)
   __shared__ float svel[2 * TC];               // staged velocity (u then v)
   __shared__ int sbase[G][3];
   __shared__ int srng[G][6];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  // consecutive groups (neighbours along the body, then the next layers) go
-  // to the same XCD, so their overlapping tiles meet in that XCD's L2
-  const int slot = xcd_block(blockIdx.x, gridDim.x) * G + wv;
+  const int slot = grp * G + wv;
   const bool live = slot < n;  // uniform per wavefront
   const int m = live ? (order ? order[slot] : slot) : 0;
 
@@ -440,7 +469,7 @@ __global__ __launch_bounds__(64 * G) void k_compute_force(
         lo[a] = min(lo[a], srng[i][a]);
         hi[a] = max(hi[a], srng[i][3 + a]);
       }
-    box_grow(box, gen, lo, hi);
+    box_grow(box, gen, lo, hi, sub);
     if (rc.on && lo[0] != INT_MAX) {
       bool out = false;
       for (int a = 0; a < 3; ++a) out |= (lo[a] < rc.lo[a]) | (hi[a] > rc.hi[a]);
@@ -448,6 +477,31 @@ __global__ __launch_bounds__(64 * G) void k_compute_force(
         atomicMax((unsigned long long*)IBM_REGION_FLAG(box), (unsigned long long)gen);
     }
   }
+}
+
+template <int S, int UMODE, int SMODE, int G>
+__global__ __launch_bounds__(64 * G) void k_compute_force(IbmArgs a, Kin kin, uint32_t gen) {
+  // consecutive groups (neighbours along the body, then the next layers) go
+  // to the same XCD, so their overlapping tiles meet in that XCD's L2
+  compute_group<S, UMODE, SMODE, G>(a, kin, gen, xcd_block(blockIdx.x, gridDim.x), blockIdx.x);
+}
+
+// Many bodies on many lattices in one launch (block -> instance as in
+// k_step_batch); kinematics and generation come from the step counter.
+template <int S, int UMODE, int SMODE, int G>
+__global__ __launch_bounds__(64 * G) void k_compute_force_batch(const IbmArgs* __restrict__ args,
+                                                                const uint16_t* __restrict__ inst,
+                                                                const int32_t* __restrict__ first,
+                                                                const int32_t* __restrict__ step,
+                                                                int planes_on) {
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int i = inst[lb];
+  IbmArgs a = args[i];
+  a.pl.on &= planes_on;
+  const int st = *step;
+  const float* k = a.kin + 7 * (size_t)st;
+  const Kin kin{k[0], k[1], k[2], k[3], k[4], k[5], k[6]};
+  compute_group<S, UMODE, SMODE, G>(a, kin, a.gen0 + (uint32_t)st + 1u, lb - first[i], blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -523,34 +577,46 @@ __global__ __launch_bounds__(IBLK) void k_band_velocity(Geo g, Planes pl,
 // K9 + K13: accumulators -> clamped float fields on the union of the boxes
 // of generations gen and gen - 1; block 0 publishes gen's box for clbm_step.
 // ---------------------------------------------------------------------------
+struct ClipArgs {
+  Geo g;
+  float f_max, inv_scale;
+  void *accx, *accy;
+  float *fx, *fy;
+  uint64_t* box;
+  uint32_t gen0;  // batched runs, as IbmArgs::gen0
+};
+
+// Block `bid` of `nblk` working on one body.
 template <int SMODE>
-__global__ __launch_bounds__(IBLK) void k_clip(Geo g, float f_max, float inv_scale,
-                                               void* __restrict__ accx, void* __restrict__ accy,
-                                               float* __restrict__ fx, float* __restrict__ fy,
-                                               uint64_t* __restrict__ box, uint32_t gen) {
+__device__ __forceinline__ void clip_body(const ClipArgs& a, uint32_t gen, int bid, int nblk) {
+  const Geo& g = a.g;
+  uint64_t* box = a.box;
   int lo[3], hi[3], plo[3], phi[3];
   const bool cur = box_get_block(box, gen, lo, hi);
   const bool prev = box_get_block(box, gen - 1, plo, phi);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (bid == 0 && threadIdx.x == 0) {
     int32_t* out = IBM_FORCE_BOX(box);
-    for (int a = 0; a < 3; ++a) {
-      out[a] = cur ? lo[a] : INT_MAX;
-      out[3 + a] = cur ? hi[a] : INT_MIN;
+    for (int i = 0; i < 3; ++i) {
+      out[i] = cur ? lo[i] : INT_MAX;
+      out[3 + i] = cur ? hi[i] : INT_MIN;
     }
   }
   if (!cur && !prev) return;
-  for (int a = 0; a < 3; ++a) {
+  for (int i = 0; i < 3; ++i) {
     if (!cur) {
-      lo[a] = plo[a];
-      hi[a] = phi[a];
+      lo[i] = plo[i];
+      hi[i] = phi[i];
     } else if (prev) {
-      lo[a] = min(lo[a], plo[a]);
-      hi[a] = max(hi[a], phi[a]);
+      lo[i] = min(lo[i], plo[i]);
+      hi[i] = max(hi[i], phi[i]);
     }
   }
   const int bx = hi[0] - lo[0] + 1, by = hi[1] - lo[1] + 1, bz = hi[2] - lo[2] + 1;
   const int vol = bx * by * bz;
-  for (int t = blockIdx.x * IBLK + threadIdx.x; t < vol; t += gridDim.x * IBLK) {
+  float* fx = a.fx;
+  float* fy = a.fy;
+  const float f_max = a.f_max;
+  for (int t = bid * IBLK + threadIdx.x; t < vol; t += nblk * IBLK) {
     const int r = t / bx;
     const int i = t - r * bx;
     const int k = r / by;
@@ -558,15 +624,15 @@ __global__ __launch_bounds__(IBLK) void k_clip(Geo g, float f_max, float inv_sca
     const int c = ((lo[2] + k) * g.ny + (lo[1] + j)) * g.nx + (lo[0] + i);
     float vx, vy;
     if (SMODE == IBM_SPREAD_FIXED) {
-      long long* ax = (long long*)accx;
-      long long* ay = (long long*)accy;
-      vx = (float)((double)ax[c] * (double)inv_scale);
-      vy = (float)((double)ay[c] * (double)inv_scale);
+      long long* ax = (long long*)a.accx;
+      long long* ay = (long long*)a.accy;
+      vx = (float)((double)ax[c] * (double)a.inv_scale);
+      vy = (float)((double)ay[c] * (double)a.inv_scale);
       ax[c] = 0;
       ay[c] = 0;
     } else {
-      float* ax = (float*)((unsigned long long*)accx + c);
-      float* ay = (float*)((unsigned long long*)accy + c);
+      float* ax = (float*)((unsigned long long*)a.accx + c);
+      float* ay = (float*)((unsigned long long*)a.accy + c);
       vx = *ax;
       vy = *ay;
       *ax = 0.0f;
@@ -575,6 +641,25 @@ __global__ __launch_bounds__(IBLK) void k_clip(Geo g, float f_max, float inv_sca
     if (fx) fx[c] = fmaxf(-f_max, fminf(f_max, vx));
     if (fy) fy[c] = fmaxf(-f_max, fminf(f_max, vy));
   }
+}
+
+template <int SMODE>
+__global__ __launch_bounds__(IBLK) void k_clip(ClipArgs a, uint32_t gen) {
+  clip_body<SMODE>(a, gen, blockIdx.x, gridDim.x);
+}
+
+// bpi blocks per body, body i = blockIdx.x / bpi.
+template <int SMODE>
+__global__ __launch_bounds__(IBLK) void k_clip_batch(const ClipArgs* __restrict__ args, int bpi,
+                                                     const int32_t* __restrict__ step) {
+  const int i = blockIdx.x / bpi;
+  const ClipArgs a = args[i];
+  clip_body<SMODE>(a, a.gen0 + (uint32_t)*step + 1u, blockIdx.x - i * bpi, bpi);
+}
+
+// The step counter of a batch (one thread).
+__global__ void k_counter_add(int32_t* c, int d) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *c += d;
 }
 
 // ---------------------------------------------------------------------------
@@ -666,14 +751,37 @@ static void launch_compute(const Geo& g, const Planes& pl, const ibm_params_t* i
                            float* X, float* Y, float* Z, float* VX, float* VY, float* U, float* V,
                            float* FX, float* FY, int32_t* base, void* ax, void* ay, uint64_t* box,
                            uint32_t gen, hipStream_t st) {
-  const float scale = (float)std::ldexp(1.0, fixed_shift(ip));
-  const int q = ip->wrap_weight_quirk ? 1 : 0;
-  auto fp = (const typename Store<S>::T*)f;
-#define CLBM_CG(GG)                                                                              \
-  hipLaunchKernelGGL((k_compute_force<S, UMODE, SMODE, GG>), dim3((n + GG - 1) / GG),           \
-                     dim3(64 * GG), 0, st, g, pl, q, ip->f_max, scale, fp, ux, uy, n, order, X0, \
-                     Y0, Z0, kin, X, Y, Z, VX, VY, U, V, FX, FY, base, ax, ay, box, gen,          \
-                     region_check(ip))
+  IbmArgs a{};
+  a.g = g;
+  a.pl = pl;
+  a.quirk = ip->wrap_weight_quirk ? 1 : 0;
+  a.f_max = ip->f_max;
+  a.fscale = (float)std::ldexp(1.0, fixed_shift(ip));
+  a.f = f;
+  a.uxf = ux;
+  a.uyf = uy;
+  a.n = n;
+  a.order = order;
+  a.X0 = X0;
+  a.Y0 = Y0;
+  a.Z0 = Z0;
+  a.X = X;
+  a.Y = Y;
+  a.Z = Z;
+  a.VX = VX;
+  a.VY = VY;
+  a.U = U;
+  a.V = V;
+  a.FX = FX;
+  a.FY = FY;
+  a.base = base;
+  a.accx = ax;
+  a.accy = ay;
+  a.box = box;
+  a.rc = region_check(ip);
+#define CLBM_CG(GG)                                                                    \
+  hipLaunchKernelGGL((k_compute_force<S, UMODE, SMODE, GG>), dim3((n + GG - 1) / GG), \
+                     dim3(64 * GG), 0, st, a, kin, gen)
   switch (ip->group) {
     case 4: CLBM_CG(4); break;
     case 16: CLBM_CG(16); break;
@@ -724,6 +832,58 @@ static void launch_marker(const Geo& g, const ibm_params_t* ip, const Planes& pl
   else
     CLBM_MK(2);
 #undef CLBM_MK
+}
+
+// ---------------------------------------------------------------------------
+// batches of bodies
+// ---------------------------------------------------------------------------
+constexpr int CLIP_BPI = 32;  // clamp blocks per body in a batch
+
+static inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+struct IbmBatchLayout {
+  size_t args, clip, inst, first, counter, total;
+  int blocks;
+};
+static int ibm_batch_layout(int n, const ibm_buffers_t* bufs, int group, IbmBatchLayout& L) {
+  if (n <= 0 || n > 65535) return fail(-1, "ibm batch: n must be in [1, 65535]");
+  if (!bufs) return fail(-1, "ibm batch: bufs is NULL");
+  if (group != 4 && group != 8) return fail(-1, "ibm batch: group must be 4 or 8");
+  long long blocks = 0;
+  for (int i = 0; i < n; ++i) {
+    if (bufs[i].n <= 0) return fail(-1, "ibm batch: every body needs markers");
+    blocks += (bufs[i].n + group - 1) / group;
+  }
+  if (blocks >= (1ll << 31)) return fail(-1, "ibm batch: too many blocks");
+  L.blocks = (int)blocks;
+  L.args = 0;
+  L.clip = align256(sizeof(IbmArgs) * n);
+  L.inst = L.clip + align256(sizeof(ClipArgs) * n);
+  L.first = L.inst + align256(sizeof(uint16_t) * (size_t)blocks);
+  L.counter = L.first + align256(sizeof(int32_t) * n);
+  L.total = L.counter + 256;
+  return 0;
+}
+
+template <int S, int SMODE, int G>
+static void launch_compute_batch(const ibm_batch_t* b, int par, int planes, hipStream_t st) {
+  const IbmArgs* a = (const IbmArgs*)b->args;
+  if (par)
+    hipLaunchKernelGGL((k_compute_force_batch<S, 2, SMODE, G>), dim3(b->blocks), dim3(64 * G), 0,
+                       st, a, b->inst, b->first, b->counter, planes);
+  else
+    hipLaunchKernelGGL((k_compute_force_batch<S, 1, SMODE, G>), dim3(b->blocks), dim3(64 * G), 0,
+                       st, a, b->inst, b->first, b->counter, planes);
+}
+
+template <int S, int SMODE>
+static void launch_batch(const ibm_batch_t* b, int par, int planes, hipStream_t st) {
+  if (b->group == 4)
+    launch_compute_batch<S, SMODE, 4>(b, par, planes, st);
+  else
+    launch_compute_batch<S, SMODE, 8>(b, par, planes, st);
+  hipLaunchKernelGGL(k_clip_batch<SMODE>, dim3(b->clip_blocks * b->n), dim3(IBLK), 0, st,
+                     (const ClipArgs*)b->clip_args, b->clip_blocks, b->counter);
 }
 
 #define CLBM_STORAGE_SWITCH(storage, CALL)                   \
@@ -875,14 +1035,114 @@ int ibm_clip_forces(const clbm_grid_t* g, const ibm_params_t* ip, void* acc_x, v
   if (!acc_x || !acc_y || !box) return fail(-1, "ibm_clip_forces: NULL buffer");
   if (gen == 0) return fail(-1, "ibm_clip_forces: generation must be >= 1");
   const Geo geo = make_geo(*g);
-  const float inv = (float)std::ldexp(1.0, -fixed_shift(ip));
+  const ClipArgs a{geo, ip->f_max, (float)std::ldexp(1.0, -fixed_shift(ip)), acc_x, acc_y, fx, fy,
+                   box, 0u};
   if (ip->spread_mode == IBM_SPREAD_FIXED)
     hipLaunchKernelGGL(k_clip<IBM_SPREAD_FIXED>, dim3(box_blocks(geo)), dim3(IBLK), 0,
-                       (hipStream_t)stream, geo, ip->f_max, inv, acc_x, acc_y, fx, fy, box, gen);
+                       (hipStream_t)stream, a, gen);
   else
     hipLaunchKernelGGL(k_clip<IBM_SPREAD_ATOMIC_F32>, dim3(box_blocks(geo)), dim3(IBLK), 0,
-                       (hipStream_t)stream, geo, ip->f_max, inv, acc_x, acc_y, fx, fy, box, gen);
+                       (hipStream_t)stream, a, gen);
   return check_launch("ibm_clip_forces");
+}
+
+size_t ibm_batch_bytes(int n, const ibm_buffers_t* bufs, int group) {
+  IbmBatchLayout L;
+  if (ibm_batch_layout(n, bufs, group, L)) return 0;
+  return L.total;
+}
+
+int ibm_batch_build(int n, const clbm_grid_t* grids, const clbm_params_t* params, void* const* f,
+                    const ibm_params_t* ip, const ibm_buffers_t* bufs, void* work,
+                    size_t work_bytes, ibm_batch_t* out) {
+  if (int e = check_ibm(ip)) return e;
+  if (ip->check_region) return fail(-1, "ibm_batch_build: batches take no region check");
+  const int group = ip->group == 0 ? 8 : ip->group;
+  IbmBatchLayout L;
+  if (int e = ibm_batch_layout(n, bufs, group, L)) return e;
+  if (!grids || !params || !f || !work || !out) return fail(-1, "ibm_batch_build: NULL argument");
+  if (work_bytes < L.total) return fail(-1, "ibm_batch_build: work buffer too small");
+  std::vector<unsigned char> host(L.total, 0);
+  IbmArgs* args = (IbmArgs*)host.data();
+  ClipArgs* clip = (ClipArgs*)(host.data() + L.clip);
+  uint16_t* inst = (uint16_t*)(host.data() + L.inst);
+  int32_t* first = (int32_t*)(host.data() + L.first);
+  const float fscale = (float)std::ldexp(1.0, fixed_shift(ip));
+  const float inv = (float)std::ldexp(1.0, -fixed_shift(ip));
+  int blk = 0;
+  for (int i = 0; i < n; ++i) {
+    const ibm_buffers_t& m = bufs[i];
+    const clbm_params_t& p = params[i];
+    if (int e = check_grid(&grids[i], p.inlet != 0)) return e;
+    if (grids[i].storage != grids[0].storage) return fail(-1, "ibm_batch_build: storage formats differ");
+    if (!f[i] || !m.X0 || !m.Y0 || !m.Z0 || !m.X || !m.Y || !m.Z || !m.VX || !m.VY || !m.U || !m.V ||
+        !m.FX || !m.FY || !m.acc_x || !m.acc_y || !m.box || !m.kin)
+      return fail(-1, "ibm_batch_build: NULL buffer");
+    IbmArgs a{};
+    a.g = make_geo(grids[i]);
+    a.pl.on = p.inlet ? 1 : 0;
+    a.pl.ux_in = p.ux_in;
+    a.pl.uy_in = p.uy_in;
+    a.quirk = ip->wrap_weight_quirk ? 1 : 0;
+    a.f_max = ip->f_max;
+    a.fscale = fscale;
+    a.f = f[i];
+    a.n = m.n;
+    a.order = m.order;
+    a.X0 = m.X0;
+    a.Y0 = m.Y0;
+    a.Z0 = m.Z0;
+    a.X = m.X;
+    a.Y = m.Y;
+    a.Z = m.Z;
+    a.VX = m.VX;
+    a.VY = m.VY;
+    a.U = m.U;
+    a.V = m.V;
+    a.FX = m.FX;
+    a.FY = m.FY;
+    a.accx = m.acc_x;
+    a.accy = m.acc_y;
+    a.box = m.box;
+    a.kin = m.kin;
+    a.gen0 = m.gen0;
+    args[i] = a;
+    clip[i] = ClipArgs{a.g, ip->f_max, inv, m.acc_x, m.acc_y, m.fx, m.fy, m.box, m.gen0};
+    first[i] = blk;
+    const int nb = (m.n + group - 1) / group;
+    for (int j = 0; j < nb; ++j) inst[blk + j] = (uint16_t)i;
+    blk += nb;
+  }
+  hipError_t e = hipMemcpy(work, host.data(), L.total, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return fail((int)e, std::string("ibm_batch_build: ") + hipGetErrorString(e));
+  unsigned char* w = (unsigned char*)work;
+  *out = ibm_batch_t{n, grids[0].storage, ip->spread_mode, group, L.blocks, CLIP_BPI, w + L.args,
+                     w + L.clip, (uint16_t*)(w + L.inst), (int32_t*)(w + L.first),
+                     (int32_t*)(w + L.counter)};
+  return 0;
+}
+
+int ibm_batch_compute(const ibm_batch_t* b, int state, void* stream) {
+  if (!b || !b->args || !b->clip_args || !b->inst || !b->first || !b->counter || b->blocks <= 0)
+    return fail(-1, "ibm_batch_compute: batch is not built");
+  if (state & ~(CLBM_STATE_PARITY | CLBM_STATE_PLANES))
+    return fail(-1, "ibm_batch_compute: state has unknown bits");
+  if (b->group != 4 && b->group != 8) return fail(-1, "ibm_batch_compute: bad group");
+  const int par = state & CLBM_STATE_PARITY;
+  const int planes = (state & CLBM_STATE_PLANES) ? 1 : 0;
+  hipStream_t st = (hipStream_t)stream;
+  if (b->spread_mode == IBM_SPREAD_FIXED) {
+    CLBM_STORAGE_SWITCH(b->storage, (launch_batch<SS, IBM_SPREAD_FIXED>(b, par, planes, st)));
+  } else {
+    CLBM_STORAGE_SWITCH(b->storage, (launch_batch<SS, IBM_SPREAD_ATOMIC_F32>(b, par, planes, st)));
+  }
+  return check_launch("ibm_batch_compute");
+}
+
+int ibm_batch_advance(const ibm_batch_t* b, void* stream) {
+  if (!b || !b->counter) return fail(-1, "ibm_batch_advance: batch is not built");
+  hipLaunchKernelGGL(k_counter_add, dim3(1), dim3(64), 0, (hipStream_t)stream, b->counter, 1);
+  return check_launch("ibm_batch_advance");
 }
 
 int ibm_force_sum_f64(int n, const float* FX, const float* FY, const float* dV, double* out2,

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <vector>
 
 namespace clbm {
 
@@ -68,11 +69,12 @@ __global__ __launch_bounds__(BLK) void k_init_feq(Geo g, typename Store<S>::T* _
   const R cs2 = R(1) / R(3);
   const R u = R(ux0), v = R(uy0), w = R(uz0), r = R(rho0);
   const R u2 = u * u + v * v + w * w;
+  const DevBase<S> dv(g);
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const R eu = R(ex(q)) * u + R(ey(q)) * v + R(ez(q)) * w;
     const R fe = wq<R>(q) * r * (R(1) + eu / cs2 + R(0.5) * eu * eu / (cs2 * cs2) - R(0.5) * u2 / cs2);
-    Store<S>::st(f + (size_t)q * N + c, q, fe);
+    Store<S>::st(f + (size_t)q * N + c, fe, dv[q]);
   }
 }
 
@@ -98,19 +100,27 @@ __device__ __forceinline__ void force_at(int c, int x, int y, int z, const float
   if (fz) Fz = R(fz[c]);
 }
 
-// REG selects the cells of the launch: 0 every cell; 1 the cells outside
+// Everything one launch of the fused step needs for one lattice.
+struct StepArgs {
+  Geo g;
+  double s_v, s_e;
+  Planes pl;
+  void* f;
+  const float *fx, *fy, *fz;
+  const int32_t* box;
+  Region rg;
+};
+
+// One thread of the fused step: t is the thread's index in the launch over
+// this lattice.  REG selects the cells: 0 every cell; 1 the cells outside
 // region rg (one thread per cell of the grid); 2 the cells inside it (one
 // thread per cell of the region).
 template <int S, int PAR, bool FORCE, bool BOX, int REG>
-__global__ __launch_bounds__(BLK) void k_step(Geo g, double s_v, double s_e, Planes pl,
-                                              typename Store<S>::T* __restrict__ f,
-                                              const float* __restrict__ fx,
-                                              const float* __restrict__ fy,
-                                              const float* __restrict__ fz,
-                                              const int32_t* __restrict__ box, Region rg) {
+__device__ __forceinline__ void step_cell(const StepArgs& a, int t) {
   using R = typename Store<S>::R;
-  const int lb = xcd_block(blockIdx.x, gridDim.x);
-  const int t = lb * BLK + threadIdx.x;
+  const Geo& g = a.g;
+  const Region& rg = a.rg;
+  const Planes& pl = a.pl;
   int c, x, y, z;
   if (REG == 2) {
     if (t >= rg.vol) return;
@@ -126,7 +136,7 @@ __global__ __launch_bounds__(BLK) void k_step(Geo g, double s_v, double s_e, Pla
     cell_xyz(g, c, x, y, z);
     if (REG == 1 && in_region(rg, x, y, z)) return;
   }
-  const Pop<S> pop{f, (size_t)g.ncell};
+  const Pop<S> pop((typename Store<S>::T*)a.f, g);
   // With pending planes, cell x = nx-1 collides the outlet copy of cell nx-2's
   // post-stream populations, but those slots are overwritten by the thread of
   // x = nx-2 during this pass, so that thread handles both cells (cell nx-1's
@@ -140,25 +150,45 @@ __global__ __launch_bounds__(BLK) void k_step(Geo g, double s_v, double s_e, Pla
   R fq[Q], Fx, Fy, Fz, Gx, Gy, Gz;
   const Nbr nb = make_nbr(g, x, y, z);
   pop_load<PAR>(pop, c, nb, fq);
-  force_at<R, FORCE, BOX>(c, x, y, z, fx, fy, fz, box, Fx, Fy, Fz);
+  force_at<R, FORCE, BOX>(c, x, y, z, a.fx, a.fy, a.fz, a.box, Fx, Fy, Fz);
   const bool outlet = pl.on && x == g.nx - 2;
   bool twin = true;
   if (outlet) {
-    force_at<R, FORCE, BOX>(c + 1, x + 1, y, z, fx, fy, fz, box, Gx, Gy, Gz);
+    force_at<R, FORCE, BOX>(c + 1, x + 1, y, z, a.fx, a.fy, a.fz, a.box, Gx, Gy, Gz);
     twin = (Gx == Fx) & (Gy == Fy) & (Gz == Fz);
     if (!twin) pop_store_in<PAR>(pop, c + 1, make_nbr(g, x + 1, y, z), fq);
   }
   if (pl.on && x == 0) inlet_rule<R>(fq, R(pl.ux_in), R(pl.uy_in));
-  collide_cell<R>(fq, Fx, Fy, Fz, R(s_v), R(s_e));
+  collide_cell<R>(fq, Fx, Fy, Fz, R(a.s_v), R(a.s_e));
   pop_store<PAR>(pop, c, nb, fq);
   if (outlet) {
     const Nbr nb1 = make_nbr(g, x + 1, y, z);
     if (!twin) {
       pop_load<PAR>(pop, c + 1, nb1, fq);
-      collide_cell<R>(fq, Gx, Gy, Gz, R(s_v), R(s_e));
+      collide_cell<R>(fq, Gx, Gy, Gz, R(a.s_v), R(a.s_e));
     }
     pop_store<PAR>(pop, c + 1, nb1, fq);
   }
+}
+
+template <int S, int PAR, bool FORCE, bool BOX, int REG>
+__global__ __launch_bounds__(BLK) void k_step(StepArgs a) {
+  step_cell<S, PAR, FORCE, BOX, REG>(a, xcd_block(blockIdx.x, gridDim.x) * BLK + threadIdx.x);
+}
+
+// Many lattices in one launch: block b (after the XCD remap) belongs to
+// instance inst[b] and is that instance's block b - first[inst[b]]; every
+// instance reads its force inside its force box.
+template <int S, int PAR, int BK>
+__global__ __launch_bounds__(BK) void k_step_batch(const StepArgs* __restrict__ args,
+                                                   const uint16_t* __restrict__ inst,
+                                                   const int32_t* __restrict__ first,
+                                                   int planes_on) {
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int i = inst[lb];
+  StepArgs a = args[i];
+  a.pl.on &= planes_on;
+  step_cell<S, PAR, true, true, 0>(a, (lb - first[i]) * BK + threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -218,10 +248,11 @@ __global__ __launch_bounds__(BLK) void k_import(Geo g, typename Store<S>::T* __r
                                                 const InT* __restrict__ in) {
   const int c = blockIdx.x * BLK + threadIdx.x;
   if (c >= g.ncell) return;
+  const DevBase<S> dv(g);
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const size_t i = (size_t)q * g.ncell + c;
-    Store<S>::st(f + i, q, (typename Store<S>::R)in[i]);
+    Store<S>::st(f + i, (typename Store<S>::R)in[i], dv[q]);
   }
 }
 
@@ -320,11 +351,10 @@ template <int S> struct StepRun {
   static void launch(const Geo& g, const clbm_params_t* p, const Planes& pl, void* f,
                      const float* fx, const float* fy, const float* fz, const int32_t* box,
                      const Region& rg, hipStream_t st) {
-    auto fp = (typename Store<S>::T*)f;
     const int n = REG == 2 ? rg.vol : g.ncell;
     const unsigned lds = REG == 1 ? (unsigned)p->outside_lds : 0u;
-    hipLaunchKernelGGL((k_step<S, PAR, FORCE, BOX, REG>), dim3(nblocks(n)), dim3(BLK), lds, st, g,
-                       p->s_v, p->s_e, pl, fp, fx, fy, fz, box, rg);
+    const StepArgs a{g, p->s_v, p->s_e, pl, f, fx, fy, fz, box, rg};
+    hipLaunchKernelGGL((k_step<S, PAR, FORCE, BOX, REG>), dim3(nblocks(n)), dim3(BLK), lds, st, a);
   }
   template <int PAR, int REG>
   static void launch_force(const Geo& g, const clbm_params_t* p, const Planes& pl, void* f,
@@ -410,11 +440,116 @@ template <int S> struct HealthRun {
   }
 };
 
+// ---------------------------------------------------------------------------
+// batches
+// ---------------------------------------------------------------------------
+static inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+struct BatchLayout {
+  size_t args, inst, first, total;
+  int blocks;
+};
+static int batch_layout(int n, const clbm_grid_t* grids, int block, BatchLayout& L) {
+  if (n <= 0 || n > 65535) return fail(-1, "batch: n must be in [1, 65535]");
+  if (!grids) return fail(-1, "batch: grids is NULL");
+  if (block != 128 && block != 256) return fail(-1, "batch: block must be 128 or 256");
+  long long blocks = 0;
+  for (int i = 0; i < n; ++i) {
+    if (int e = check_grid(&grids[i], false)) return e;
+    if (grids[i].storage != grids[0].storage) return fail(-1, "batch: storage formats differ");
+    blocks += ((long long)grids[i].nx * grids[i].ny * grids[i].nz + block - 1) / block;
+  }
+  if (blocks >= (1ll << 31)) return fail(-1, "batch: too many blocks");
+  L.blocks = (int)blocks;
+  L.args = 0;
+  L.inst = align256(sizeof(StepArgs) * n);
+  L.first = L.inst + align256(sizeof(uint16_t) * (size_t)blocks);
+  L.total = L.first + align256(sizeof(int32_t) * n);
+  return 0;
+}
+
+template <int S> struct BatchRun {
+  template <int PAR, int BK>
+  static void go(const clbm_batch_t* b, int planes, hipStream_t st) {
+    hipLaunchKernelGGL((k_step_batch<S, PAR, BK>), dim3(b->blocks), dim3(BK), 0, st,
+                       (const StepArgs*)b->args, b->inst, b->first, planes);
+  }
+  static int run(const clbm_batch_t* b, int state, hipStream_t st) {
+    const int planes = (state & CLBM_STATE_PLANES) ? 1 : 0;
+    const bool odd = state & CLBM_STATE_PARITY;
+    if (b->block == 128) {
+      if (odd) go<1, 128>(b, planes, st); else go<0, 128>(b, planes, st);
+    } else {
+      if (odd) go<1, 256>(b, planes, st); else go<0, 256>(b, planes, st);
+    }
+    return check_launch("clbm_batch_step");
+  }
+};
+
 }  // namespace clbm
 
 using namespace clbm;
 
 extern "C" {
+
+size_t clbm_batch_bytes(int n, const clbm_grid_t* grids, int block) {
+  BatchLayout L;
+  if (batch_layout(n, grids, block, L)) return 0;
+  return L.total;
+}
+
+int clbm_batch_build(int n, const clbm_grid_t* grids, const clbm_params_t* params, void* const* f,
+                     float* const* fx, float* const* fy, uint64_t* const* boxes, int block,
+                     void* work, size_t work_bytes, clbm_batch_t* out) {
+  BatchLayout L;
+  if (int e = batch_layout(n, grids, block, L)) return e;
+  if (!params || !f || !fx || !fy || !boxes || !work || !out)
+    return fail(-1, "clbm_batch_build: NULL argument");
+  if (work_bytes < L.total) return fail(-1, "clbm_batch_build: work buffer too small");
+  std::vector<unsigned char> host(L.total, 0);
+  StepArgs* args = (StepArgs*)host.data();
+  uint16_t* inst = (uint16_t*)(host.data() + L.inst);
+  int32_t* first = (int32_t*)(host.data() + L.first);
+  int blk = 0;
+  for (int i = 0; i < n; ++i) {
+    const clbm_params_t& p = params[i];
+    if (int e = check_grid(&grids[i], p.inlet != 0)) return e;
+    if (!f[i] || !fx[i] || !fy[i] || !boxes[i]) return fail(-1, "clbm_batch_build: NULL buffer");
+    if (p.inlet && !(p.ux_in < 1.0)) return fail(-1, "clbm_batch_build: inlet needs ux_in < 1");
+    StepArgs a{};
+    a.g = make_geo(grids[i]);
+    a.s_v = p.s_v;
+    a.s_e = p.s_e;
+    a.pl.on = p.inlet ? 1 : 0;   // the planes of a step are those of the state AND this flag
+    a.pl.ux_in = p.ux_in;
+    a.pl.uy_in = p.uy_in;
+    a.f = f[i];
+    a.fx = fx[i];
+    a.fy = fy[i];
+    a.fz = nullptr;
+    a.box = IBM_FORCE_BOX(boxes[i]);
+    args[i] = a;
+    first[i] = blk;
+    const int nb = (a.g.ncell + block - 1) / block;
+    for (int j = 0; j < nb; ++j) inst[blk + j] = (uint16_t)i;
+    blk += nb;
+  }
+  hipError_t e = hipMemcpy(work, host.data(), L.total, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return fail((int)e, std::string("clbm_batch_build: ") + hipGetErrorString(e));
+  unsigned char* w = (unsigned char*)work;
+  *out = clbm_batch_t{n, grids[0].storage, block, L.blocks, w, (uint16_t*)(w + L.inst),
+                      (int32_t*)(w + L.first)};
+  return 0;
+}
+
+int clbm_batch_step(const clbm_batch_t* b, int state, void* stream) {
+  if (!b || !b->args || !b->inst || !b->first || b->blocks <= 0)
+    return fail(-1, "clbm_batch_step: batch is not built");
+  if (state & ~(CLBM_STATE_PARITY | CLBM_STATE_PLANES))
+    return fail(-1, "clbm_batch_step: state has unknown bits");
+  if (b->block != 128 && b->block != 256) return fail(-1, "clbm_batch_step: bad block size");
+  return dispatch_storage<BatchRun>(b->storage, b, state, (hipStream_t)stream);
+}
 
 int clbm_abi_version(void) { return CLBM_ABI_VERSION; }
 

@@ -23,7 +23,8 @@
  *                cell Nx-2's populations for x = Nx-1 as it loads (this is how
  *                K5/K6 run without a pass of their own).  Readers of such a
  *                state need the params (inlet velocity).
- *                16-bit storage holds f_q - w_q (deviation from the lattice weight).
+ *                16-bit storage holds f_q - feq_q(1, dev velocity of the grid), the
+ *                deviation from the lattice weight w_q when that velocity is zero.
  *   scalar fields  [z][y][x] float32, x fastest.
  *   force box    uint64[IBM_BOX_WORDS], zero-initialised, owned by the caller.
  *                Every IBM call carries a generation number gen >= 1 (one per
@@ -48,7 +49,7 @@
 extern "C" {
 #endif
 
-#define CLBM_ABI_VERSION 2
+#define CLBM_ABI_VERSION 3
 
 /* population storage formats */
 #define CLBM_STORE_F32 0  /* float32 populations, float32 arithmetic (the drop-in) */
@@ -74,6 +75,9 @@ extern "C" {
 typedef struct {
   int32_t nx, ny, nz; /* lattice size, Nx >= 3 */
   int32_t storage;    /* CLBM_STORE_* */
+  float dev_ux, dev_uy; /* 16-bit formats store f_q - feq_q(rho = 1, u = (dev_ux, dev_uy, 0));
+                           0, 0 stores f_q - w_q.  Part of the buffer's format: every call
+                           on one buffer must pass the same values.  Wide formats ignore it. */
 } clbm_grid_t;
 
 typedef struct {
@@ -240,6 +244,74 @@ int ibm_clip_forces(const clbm_grid_t* g, const ibm_params_t* ip, void* acc_x, v
  * (dV nullable = 1). */
 int ibm_force_sum_f64(int n, const float* FX, const float* FY, const float* dV, double* out2,
                       void* stream);
+
+/* ---- batched runs: many independent lattices and bodies per launch ------
+ * No reference counterpart (the reference runs one case per process).  A
+ * batch advances n instances, each a lattice with its own grid, rates and
+ * inflow (all in one storage format) and one body, in lockstep: per step one
+ * ibm_batch_compute (marker kernel + clamp of every body), one clbm_batch_step
+ * (fused step of every lattice) and one ibm_batch_advance, so that a step of
+ * the whole batch is a fixed sequence of four launches that a caller can
+ * capture in a graph and replay.  Kinematics come from per-body device tables
+ * indexed by the batch's device step counter, never from host arguments.
+ * Every instance computes what its own single-instance calls would, bit for
+ * bit (fixed-point spreading).  Descriptor tables live in caller-owned device
+ * work buffers of the sizes the *_bytes functions return; building copies
+ * them there synchronously. */
+typedef struct {
+  int32_t n, storage, block, blocks;
+  void* args;         /* device: per-instance step arguments */
+  uint16_t* inst;     /* device: instance of each block */
+  int32_t* first;     /* device: first block of each instance */
+} clbm_batch_t;
+
+/* Device bytes of the lattice tables for n grids, `block` threads per
+ * workgroup (128 or 256). */
+size_t clbm_batch_bytes(int n, const clbm_grid_t* grids, int block);
+/* Lattice i: populations f[i], the body's clamped force fields fx[i], fy[i]
+ * and its force-box record boxes[i] (read inside IBM_FORCE_BOX, as clbm_step
+ * reads its force_box). */
+int clbm_batch_build(int n, const clbm_grid_t* grids, const clbm_params_t* params, void* const* f,
+                     float* const* fx, float* const* fy, uint64_t* const* boxes, int block,
+                     void* work, size_t work_bytes, clbm_batch_t* out);
+/* clbm_step of every lattice from state `state` (all lattices share it). */
+int clbm_batch_step(const clbm_batch_t* b, int state, void* stream);
+
+/* The buffers of one body (as the arguments of ibm_compute_force + ibm_clip_forces). */
+typedef struct {
+  int32_t n;
+  const int32_t* order;
+  const float *X0, *Y0, *Z0;
+  float *X, *Y, *Z, *VX, *VY, *U, *V, *FX, *FY;
+  void *acc_x, *acc_y;
+  uint64_t* box;
+  float *fx, *fy;
+  const float* kin; /* device: for each step s of the counter, 7 floats
+                       xc, yc, cos(theta), sin(theta), vx, vy, omega */
+  uint32_t gen0;    /* the body's generation is gen0 + 1 + counter */
+} ibm_buffers_t;
+
+typedef struct {
+  int32_t n, storage, spread_mode, group, blocks, clip_blocks;
+  void* args;         /* device: per-body marker-kernel arguments */
+  void* clip_args;    /* device: per-body clamp arguments */
+  uint16_t* inst;
+  int32_t* first;
+  int32_t* counter;   /* device: steps taken by the batch */
+} ibm_batch_t;
+
+/* Device bytes of the body tables (the step counter included). */
+size_t ibm_batch_bytes(int n, const ibm_buffers_t* bufs, int group);
+/* Body i moves on lattice i (grids[i], params[i], populations f[i]); ip
+ * (spread mode, clamp, quirk, group 4 or 8) is shared.  Sets the counter to 0. */
+int ibm_batch_build(int n, const clbm_grid_t* grids, const clbm_params_t* params, void* const* f,
+                    const ibm_params_t* ip, const ibm_buffers_t* bufs, void* work,
+                    size_t work_bytes, ibm_batch_t* out);
+/* ibm_compute_force + ibm_clip_forces of every body at the counter's step,
+ * velocity from each lattice's populations in state `state`. */
+int ibm_batch_compute(const ibm_batch_t* b, int state, void* stream);
+/* counter += 1 (after the lattices have stepped). */
+int ibm_batch_advance(const ibm_batch_t* b, void* stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,114 @@
+"""Batched sweeps: many instances stepped by one launch per kernel type.
+
+CPU: the seeded draws and the device kinematics table (what
+ibm_compute_force receives per step: float32 arguments, cos/sin of the
+float32 angle rounded once from float64).  GPU: every instance of a batch
+equals the same instance run on its own, bit for bit (fixed-point
+spreading); graph replays equal eager steps; tuning restores the initial
+state; malformed batches are rejected.
+"""
+
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+from conftest import load_sub
+
+
+def test_draws_are_seeded_and_in_range(pkg):
+    bm = load_sub("batch")
+    a = bm.draw_instances(12, seed=3)
+    b = bm.draw_instances(12, seed=3)
+    assert [c.name for c in a] == [c.name for c in b]
+    for c in a:
+        assert (c.Nx, c.Ny, c.Nz) in bm.GRIDS
+        assert c.chord == pytest.approx(c.Ny / 2.5)
+        assert 100 <= c.Re <= 400
+        assert 0.1 * c.chord - 1e-9 <= c.A_heave <= 0.5 * c.chord + 1e-9
+        assert abs(math.degrees(c.A_pitch)) <= 20 + 1e-9
+    assert len({c.Re for c in a}) == 12
+
+
+def test_kinematics_table_matches_the_host_arguments(pkg):
+    bm = load_sub("batch")
+    cfg = pkg.config.naca_case(64, 32, 4, 12.0)
+    t = bm.kinematics_table(cfg, 5, 4)
+    assert t.dtype == np.float32 and t.shape == (4, 7)
+    for j in range(4):
+        xc, yc, th, vx, vy, om = cfg.motion_state(5 + j)
+        a = float(np.float32(th))
+        assert t[j, 0] == np.float32(xc) and t[j, 1] == np.float32(yc)
+        assert t[j, 2] == np.float32(math.cos(a)) and t[j, 3] == np.float32(math.sin(a))
+        assert t[j, 6] == np.float32(om)
+
+
+# ---------------------------------------------------------------------------
+# GPU
+# ---------------------------------------------------------------------------
+def _small(pkg):
+    return [pkg.config.naca_case(48, 24, 4, 9.6, Re=150, name="a"),
+            pkg.config.naca_case(40, 20, 6, 8.0, Re=300, k=0.9, aoa_mean_deg=3.0, name="b"),
+            pkg.config.naca_case(64, 32, 4, 12.8, Re=220, heave_frac=0.4, name="c"),
+            pkg.config.naca_case(48, 24, 4, 9.6, Re=380, phase_deg=10.0, name="d")]
+
+
+def _assert_same(batch_objs, seq_objs):
+    for (bl, bi), (sl, si) in zip(batch_objs, seq_objs):
+        np.testing.assert_array_equal(bl.f.to_numpy(), sl.f.to_numpy())
+        np.testing.assert_array_equal(bi.FX_k.to_numpy(), si.FX_k.to_numpy())
+        np.testing.assert_array_equal(bi.fx.to_numpy(), si.fx.to_numpy())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("block,group", [(256, 8), (128, 4)])
+def test_batch_equals_each_instance_alone(pkg, gpu, block, group):
+    bm = load_sub("batch")
+    cfgs = _small(pkg)
+    sweep = bm.BatchSweep(cfgs, steps=16, block=block, group=group)
+    sweep.step(7)
+    seq, _ = bm.sequential(cfgs, 7)
+    _assert_same(list(zip(sweep.lbms, sweep.ibms)), seq)
+
+
+@pytest.mark.gpu
+def test_graph_replay_equals_eager(pkg, gpu):
+    bm = load_sub("batch")
+    cfgs = _small(pkg)[:3]
+    a = bm.BatchSweep(cfgs, steps=32)
+    b = bm.BatchSweep(cfgs, steps=32)
+    a.run(11, graph=True)
+    a.run(6, graph=True)       # resumes on the other parity: realigns, then replays
+    b.step(17)
+    _assert_same(list(zip(a.lbms, a.ibms)), list(zip(b.lbms, b.ibms)))
+    seq, _ = bm.sequential(cfgs, 17)
+    _assert_same(list(zip(a.lbms, a.ibms)), seq)
+
+
+@pytest.mark.gpu
+def test_autotune_restores_the_initial_state(pkg, gpu, tmp_path):
+    bm = load_sub("batch")
+    cfgs = _small(pkg)[:2]
+    sweep = bm.BatchSweep(cfgs, steps=64)
+    times = sweep.autotune(steps=2, cache=str(tmp_path / "tune.json"))
+    assert set(times) == {f"{b}/{g}" for b, g in bm.CANDIDATES}
+    sweep.step(5)
+    seq, _ = bm.sequential(cfgs, 5)
+    _assert_same(list(zip(sweep.lbms, sweep.ibms)), seq)
+    # the cached choice is reused
+    again = bm.BatchSweep(cfgs, steps=8)
+    assert again.autotune(cache=str(tmp_path / "tune.json")) == times
+
+
+@pytest.mark.gpu
+def test_batch_rejects_malformed_input(pkg, gpu):
+    lib = load_sub("_lib")
+    g = (lib.ClbmGrid * 2)(lib.ClbmGrid(8, 6, 4, lib.STORE_F32), lib.ClbmGrid(8, 6, 4, lib.STORE_F16))
+    assert lib.load().clbm_batch_bytes(2, g, 256) == 0          # mixed storage
+    assert b"storage" in lib.load().clbm_last_error()
+    g1 = (lib.ClbmGrid * 1)(lib.ClbmGrid(8, 6, 4, lib.STORE_F32))
+    assert lib.load().clbm_batch_bytes(1, g1, 96) == 0         # block size
+    out = lib.ClbmBatch()
+    with pytest.raises(lib.ClbmError, match="not built"):
+        lib.call("clbm_batch_step", C.byref(out), 0, None)
