@@ -75,12 +75,55 @@ template <> struct Store<CLBM_STORE_BF16> {
   __device__ static void st(T* p, R v, float b) { *p = __float2bfloat16(v - b); }
 };
 
+// 16-bit float for deviations |x| < 1: sign, 4 exponent bits, 11 mantissa
+// bits.  Exponent field e = 1..15 encodes (1 + m / 2^11) 2^(e - 16); e = 0
+// encodes m 2^-26 (subnormal, gradual underflow); 0x7fff (either sign) is
+// NaN, which every |x| >= 1, infinity or NaN becomes.  Rounding is to
+// nearest, ties to even.  Relative precision 2^-12 for |x| >= 2^-15 (twice
+// IEEE half's), absolute 2^-27 below.
+// Both directions are branch-free (selects), so lanes never diverge on the
+// format's cases.  Below 2^-15 the codes are the float32 bits of 0.125 + |v|
+// minus those of 0.125: that sum's unit in the last place is 2^-26, so the
+// hardware addition does the round-to-nearest-even.
+__device__ __forceinline__ uint16_t f16c_encode(float v) {
+  constexpr uint32_t k8 = 0x3e000000u;  // float32 bits of 0.125
+  const uint32_t b = __float_as_uint(v);
+  const uint32_t ab = b & 0x7fffffffu;
+  // normal: round the float32 mantissa at bit 12 (ties to even), rebias 127 -> 16
+  const uint32_t r = ab + 0x7ffu + ((ab >> 12) & 1u);
+  const uint32_t hn = (r >> 12) - (111u << 11);
+  const uint32_t hs = __float_as_uint(__uint_as_float(ab) + 0.125f) - k8;
+  // |v| >= 1, inf and NaN become 0x7fff; a value rounding up to that code is
+  // NaN by the same code
+  uint32_t h = ab < 0x38000000u ? hs : hn;
+  h = ab >= 0x3f800000u ? 0x7fffu : h;
+  return (uint16_t)(((b >> 16) & 0x8000u) | h);
+}
+__device__ __forceinline__ float f16c_decode(uint16_t hv) {
+  constexpr uint32_t k8 = 0x3e000000u;
+  const uint32_t h = hv;
+  const uint32_t u = h & 0x7fffu;
+  const float an = __uint_as_float((u << 12) + (111u << 23));
+  const float as = __uint_as_float(k8 + u) - 0.125f;  // u < 2^11 here: u 2^-26 exactly
+  float a = u >= 0x800u ? an : as;
+  a = u == 0x7fffu ? __uint_as_float(0x7fc00000u) : a;
+  return __uint_as_float(__float_as_uint(a) | ((h & 0x8000u) << 16));
+}
+template <> struct Store<CLBM_STORE_F16C> {
+  using T = uint16_t;
+  using R = float;
+  static constexpr bool dev = true;
+  __device__ static R ld(const T* p, float b) { return b + f16c_decode(*p); }
+  __device__ static void st(T* p, R v, float b) { *p = f16c_encode(v - b); }
+};
+
 inline size_t elem_bytes(int storage) {
   switch (storage) {
     case CLBM_STORE_F32: return 4;
     case CLBM_STORE_F64: return 8;
     case CLBM_STORE_F16:
-    case CLBM_STORE_BF16: return 2;
+    case CLBM_STORE_BF16:
+    case CLBM_STORE_F16C: return 2;
     default: return 0;
   }
 }

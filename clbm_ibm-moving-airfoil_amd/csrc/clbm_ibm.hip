@@ -892,6 +892,7 @@ static void launch_batch(const ibm_batch_t* b, int par, int planes, hipStream_t 
     case CLBM_STORE_F16: { constexpr int SS = CLBM_STORE_F16; CALL; } break;   \
     case CLBM_STORE_BF16: { constexpr int SS = CLBM_STORE_BF16; CALL; } break; \
     case CLBM_STORE_F64: { constexpr int SS = CLBM_STORE_F64; CALL; } break;   \
+    case CLBM_STORE_F16C: { constexpr int SS = CLBM_STORE_F16C; CALL; } break; \
     default: return fail(-1, "unknown storage format");      \
   }
 

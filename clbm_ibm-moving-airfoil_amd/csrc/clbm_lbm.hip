@@ -332,6 +332,7 @@ static int dispatch_storage(int storage, A&&... a) {
     case CLBM_STORE_F16: return Fn<CLBM_STORE_F16>::run(a...);
     case CLBM_STORE_BF16: return Fn<CLBM_STORE_BF16>::run(a...);
     case CLBM_STORE_F64: return Fn<CLBM_STORE_F64>::run(a...);
+    case CLBM_STORE_F16C: return Fn<CLBM_STORE_F16C>::run(a...);
   }
   return fail(-1, "unknown storage format");
 }

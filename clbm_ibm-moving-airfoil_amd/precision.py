@@ -95,7 +95,7 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--skip", type=int, default=2000, help="steps left out of the window")
     ap.add_argument("--json", default=None)
-    ap.add_argument("--storages", default="f64,f32,f16,bf16,f16-inflow,bf16-inflow",
+    ap.add_argument("--storages", default="f64,f32,f16,bf16,f16-inflow,f16c,f16c-inflow",
                     help="comma list; f64 is the reference and must be first")
     a = ap.parse_args(argv)
     res = study(CASES[a.case](), a.steps, a.skip, tuple(a.storages.split(",")))

@@ -56,6 +56,8 @@ extern "C" {
 #define CLBM_STORE_F16 1  /* IEEE half deviations f - w, float32 arithmetic */
 #define CLBM_STORE_BF16 2 /* bfloat16 deviations f - w, float32 arithmetic */
 #define CLBM_STORE_F64 3  /* float64 populations, float64 arithmetic (error reference) */
+#define CLBM_STORE_F16C 4 /* 16-bit deviations in a 1-4-11 float (|x| < 1, twice IEEE half's
+                             precision), float32 arithmetic */
 
 /* force spreading modes */
 #define IBM_SPREAD_FIXED 0 /* order-independent 64-bit fixed-point sums (deterministic) */

@@ -131,13 +131,18 @@ def test_init_state_is_fixed_point_of_rest(pkg, oracle, gpu):
     np.testing.assert_allclose(f0[0, 0, 0], oracle.W, rtol=0, atol=1e-16)
 
 
-@pytest.mark.parametrize("storage", ["f16", "bf16"])
-def test_16bit_storage_tracks_fp64(pkg, oracle, gpu, storage):
-    """Deviation storage: one rounding of f - w per step; after 6 steps the
-    populations stay within the format's resolution of the FP64 result."""
+MANTISSA = {"f16": 10, "bf16": 7, "f16c": 11}
+
+
+@pytest.mark.parametrize("deviation", ["weights", "inflow"])
+@pytest.mark.parametrize("storage", ["f16", "bf16", "f16c"])
+def test_16bit_storage_tracks_fp64(pkg, oracle, gpu, storage, deviation):
+    """Deviation storage: one rounding of f - base per step; after 6 steps the
+    populations stay within the format's resolution (4 roundings of the
+    largest deviation) of the FP64 result."""
     shape = (12, 8, 6)
     cfg = _case(pkg, *shape)
-    s = pkg.CLBMSolver(cfg, storage=storage)
+    s = pkg.CLBMSolver(cfg, storage=storage, deviation=deviation)
     f = _rand_state(oracle, shape, 3, amp=0.005)
     F = _rand_force(shape, 3, amp=2e-4)
     s.f.from_numpy(f)
@@ -145,9 +150,33 @@ def test_16bit_storage_tracks_fp64(pkg, oracle, gpu, storage):
     for _ in range(6):
         s.step(*tF)
         f, *_ = oracle.lbm_step(f, *F, cfg.s_v, cfg.s_e, cfg.ux_in, cfg.uy_in)
-    dev = np.abs(f - oracle.W).max()
-    tol = dev * (2.0 ** -10 if storage == "f16" else 2.0 ** -7) * 4
+    base = oracle.W if deviation == "weights" else oracle.init_feq(1, 1, 1, cfg.ux_in, cfg.uy_in)[0][0, 0, 0]
+    dev = np.abs(f - base).max()
+    tol = dev * 2.0 ** -MANTISSA[storage] * 4
     np.testing.assert_allclose(s.f.to_numpy(), f, rtol=0, atol=tol)
+
+
+@pytest.mark.parametrize("deviation", ["weights", "inflow"])
+def test_f16c_import_export_round_trip_is_the_codec(pkg, oracle, gpu, deviation):
+    """clbm_import_state + clbm_export_state of the 1-4-11 format equal the
+    oracle's codec on f - base, bit for bit (base: float32 equilibrium at
+    the deviation velocity, rounded once from float64)."""
+    shape = (10, 6, 4)
+    cfg = _case(pkg, *shape)
+    s = pkg.CLBMSolver(cfg, storage="f16c", deviation=deviation)
+    f = _rand_state(oracle, shape, 9, amp=0.05).astype(np.float32)
+    f[0, 0, 0, 3] = np.float32(np.nan)
+    s.f.from_numpy(f)
+    # the base as the library forms it: float32 grid velocity, second-order
+    # equilibrium at rho = 1 in float64, rounded once
+    u, v = ((float(np.float32(cfg.ux_in)), float(np.float32(cfg.uy_in))) if deviation == "inflow"
+            else (0.0, 0.0))
+    eu = oracle.EX * u + oracle.EY * v
+    base = (oracle.W * (1.0 + 3.0 * eu + 4.5 * eu * eu - 1.5 * (u * u + v * v))).astype(np.float32)
+    want = base + oracle.f16c_decode(oracle.f16c_encode(f - base))
+    got = s.f.to_numpy()
+    np.testing.assert_array_equal(got, want)
+    assert np.isnan(got[0, 0, 0, 3])
 
 
 def test_health_reports_min_and_nonfinite(pkg, oracle, gpu):
