@@ -103,6 +103,14 @@ _SIGNATURES = {
                              C.POINTER(IbmBatch)]),
     "ibm_batch_compute": (_I, [C.POINTER(IbmBatch), _I, _P]),
     "ibm_batch_advance": (_I, [C.POINTER(IbmBatch), _P]),
+    "closure_param_count": (_I, []),
+    "closure_packed_bytes": (C.c_size_t, []),
+    "closure_pack": (_I, [_P, _P, _P]),
+    "closure_infer": (_I, [_P, _P, _I, _P, _P, _P]),
+    "clbm_step_closure": (_I, [_G, _LP, _P, _I, _P, _P, _P, _P, _P, _P, _F, _F, _P]),
+    "closure_partial_bytes": (C.c_size_t, [_I]),
+    "closure_grad": (_I, [_P, _P, _I, _P, _P, _P, _F, _P, _I, _P, _P]),
+    "closure_adam": (_I, [_P, _P, _P, _P, _F, _F, _F, _F, _I, _P]),
 }
 STATE_PARITY, STATE_PLANES = 1, 2
 

@@ -11,7 +11,7 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SOURCES = ["csrc/clbm_lbm.hip", "csrc/clbm_ibm.hip"]
+SOURCES = ["csrc/clbm_lbm.hip", "csrc/clbm_ibm.hip", "csrc/clbm_closure.hip"]
 LIB_NAME = "libclbm_amd.so"
 
 # Packed-FP32 instructions are kept out of the device code on purpose.

@@ -140,6 +140,8 @@ class CLBMSolver:
         # next step (the whole step, or its outside part); cells that kernel updated
         self.kernel_events = None
         self.last_kernel_cells = 0
+        # a learned closure (closure.ClosureNet) replaces the plain cascade when set
+        self.closure = None
         self._init_feq()
 
     # -- reference surface ------------------------------------------------
@@ -154,7 +156,7 @@ class CLBMSolver:
     def step(self, fx=None, fy=None, fz=None) -> None:
         """Advance one step with body force (fx, fy, fz) (clbm_moving.py:326-343)."""
         import torch
-        split = self._split_source(fx, fy, fz)
+        split = self._split_source(fx, fy, fz) if self.closure is None else None
         ev = self.kernel_events
         self.kernel_events = None
         main = torch.cuda.current_stream(self.device)
@@ -191,9 +193,16 @@ class CLBMSolver:
             fxp, fyp = as_device_ptr(fx, shape), as_device_ptr(fy, shape)
             if ev is not None:
                 ev[0].record(main)
-            _lib.call("clbm_step", C.byref(self.grid), C.byref(self.params), state,
-                      self.state_word, C.c_void_p(fxp), C.c_void_p(fyp), C.c_void_p(fzp),
-                      C.c_void_p(box), st)
+            if self.closure is None:
+                _lib.call("clbm_step", C.byref(self.grid), C.byref(self.params), state,
+                          self.state_word, C.c_void_p(fxp), C.c_void_p(fyp), C.c_void_p(fzp),
+                          C.c_void_p(box), st)
+            else:
+                net = self.closure
+                _lib.call("clbm_step_closure", C.byref(self.grid), C.byref(self.params), state,
+                          self.state_word, C.c_void_p(fxp), C.c_void_p(fyp), C.c_void_p(fzp),
+                          C.c_void_p(box), C.c_void_p(net.packed.data_ptr()),
+                          C.c_void_p(net.params.data_ptr()), net.lo, net.hi, st)
             if ev is not None:
                 ev[1].record(main)
             self.last_kernel_cells = self.ncell

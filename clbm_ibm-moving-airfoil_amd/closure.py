@@ -1,0 +1,280 @@
+"""Learned closure: a per-cell network that corrects the higher-order rates.
+
+No reference counterpart.  The reference's cascade relaxes third-order
+central moments to 0 and fourth-order ones to rho/9 in every cell
+(s_q = s_pi = 1, clbm_moving.py:188-196).  On under-resolved high-Re grids
+this closure models a network 19 -> 32 -> 32 -> 4 (GELU) that maps each
+cell's 19 normalised central moments to corrections y of four rates,
+s = 1 + y, for (k210, k120), (k201, k021, k102, k012), (k220) and
+(k202, k022).  A cell whose rates leave [0, 2] falls back to the plain
+cascade.
+
+Everything runs on the GPU through the library's kernels:
+
+* ``clbm_step_closure``: the fused step with the network evaluated per
+  wavefront on bf16 MFMA;
+* ``closure_grad`` and ``closure_adam``: training, forward and backward on
+  bf16 MFMA with float32 accumulation and a fixed-order gradient reduction.
+
+The training data come from the method itself.  A 2x-refined run of the
+same case (same Re, so twice the chord, and two fine steps per coarse step)
+is box-filtered onto the coarse lattice.  A cell's target rates are those
+that turn its filtered populations at one coarse step into the filtered
+populations its neighbours hold one coarse step later
+(f*_q(x) = f_q(x + e_q, t + 1)), by least squares over each rate's moments.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import _lib
+
+NP = 1828
+_SHAPES = [(19, 32), (32,), (32, 32), (32,), (32, 4), (4,)]
+EX = np.array([0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0])
+EY = np.array([0, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, 1, -1])
+EZ = np.array([0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1])
+PQR = [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), (1, 1, 0), (1, 0, 1), (0, 1, 1), (2, 0, 0),
+       (0, 2, 0), (0, 0, 2), (2, 1, 0), (1, 2, 0), (2, 0, 1), (0, 2, 1), (1, 0, 2), (0, 1, 2),
+       (2, 2, 0), (2, 0, 2), (0, 2, 2)]
+FEATURE_EQ = np.array([1.0, 0, 0, 0, 0, 0, 0, 1 / 3, 1 / 3, 1 / 3, 0, 0, 0, 0, 0, 0, 1 / 9, 1 / 9,
+                       1 / 9])
+GROUPS = [[10, 11], [12, 13, 14, 15], [16], [17, 18]]   # feature indices of each rate
+
+
+def split_params(p):
+    out, o = [], 0
+    for s in _SHAPES:
+        n = int(np.prod(s))
+        out.append(np.asarray(p[o:o + n]).reshape(s))
+        o += n
+    return out
+
+
+def join_params(parts):
+    return np.concatenate([np.asarray(a, dtype=np.float64).ravel() for a in parts])
+
+
+def init_params(seed: int = 0, out_scale: float = 0.0) -> np.ndarray:
+    """He-style random weights; the last layer starts at ``out_scale`` (0 gives
+    the plain cascade exactly: y = 0, s = 1)."""
+    rng = np.random.default_rng(seed)
+    W1 = rng.standard_normal((19, 32)) * math.sqrt(2.0 / 19)
+    W2 = rng.standard_normal((32, 32)) * math.sqrt(2.0 / 32)
+    W3 = rng.standard_normal((32, 4)) * math.sqrt(2.0 / 32) * out_scale
+    return join_params([W1, np.zeros(32), W2, np.zeros(32), W3, np.zeros(4)])
+
+
+def fold_standardization(params, mean, std) -> np.ndarray:
+    """Parameters acting on raw features x from parameters trained on
+    (x - mean) / std: W1' = W1 / std, b1' = b1 - (mean / std) W1."""
+    W1, b1, W2, b2, W3, b3 = split_params(np.asarray(params, dtype=np.float64))
+    s = np.asarray(std, dtype=np.float64)
+    m = np.asarray(mean, dtype=np.float64)
+    W1f = W1 / s[:, None]
+    b1f = b1 - (m / s) @ W1
+    return join_params([W1f, b1f, W2, b2, W3, b3])
+
+
+class ClosureNet:
+    """A network on the device: float32 parameters and their packed bf16 operands."""
+
+    def __init__(self, params=None, device=None, lo: float = 0.0, hi: float = 2.0):
+        import torch
+        self.device = torch.device(device if device is not None else "cuda")
+        lib = _lib.load()
+        if lib.closure_param_count() != NP:
+            raise _lib.ClbmError("closure parameter count mismatch")
+        p = init_params() if params is None else np.asarray(params, dtype=np.float32)
+        self.params = torch.from_numpy(np.ascontiguousarray(p, dtype=np.float32)).to(self.device)
+        self.packed = torch.empty(lib.closure_packed_bytes(), dtype=torch.uint8, device=self.device)
+        self.lo, self.hi = float(lo), float(hi)
+        self.repack()
+
+    def _st(self):
+        import torch
+        return C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def repack(self) -> None:
+        _lib.call("closure_pack", C.c_void_p(self.params.data_ptr()),
+                  C.c_void_p(self.packed.data_ptr()), self._st())
+
+    def numpy(self) -> np.ndarray:
+        return self.params.cpu().numpy().astype(np.float64)
+
+    def infer(self, X):
+        """(n, 19) features -> (n, 4) outputs y on the device."""
+        import torch
+        X = torch.as_tensor(np.ascontiguousarray(X, dtype=np.float32)).to(self.device) \
+            if not isinstance(X, torch.Tensor) else X.contiguous().float()
+        Y = torch.empty((X.shape[0], 4), dtype=torch.float32, device=self.device)
+        _lib.call("closure_infer", C.c_void_p(self.packed.data_ptr()),
+                  C.c_void_p(self.params.data_ptr()), int(X.shape[0]), C.c_void_p(X.data_ptr()),
+                  C.c_void_p(Y.data_ptr()), self._st())
+        return Y
+
+    def save(self, path: str, **meta) -> None:
+        np.savez(path, params=self.numpy(), lo=self.lo, hi=self.hi,
+                 meta=json.dumps(meta))
+
+    @classmethod
+    def load(cls, path: str, device=None):
+        d = np.load(path, allow_pickle=False)
+        return cls(d["params"], device=device, lo=float(d["lo"]), hi=float(d["hi"]))
+
+
+class Trainer:
+    """Adam on the weighted mean squared rate error, all on the device."""
+
+    def __init__(self, X, T, W, device=None, blocks: int = 256, lr: float = 3e-3,
+                 params=None, seed: int = 0):
+        import torch
+        self.device = torch.device(device if device is not None else "cuda")
+        X = np.asarray(X, dtype=np.float64)
+        self.mean = X.mean(0)
+        self.std = X.std(0)
+        self.std[self.std < 1e-12] = 1.0
+        Xs = (X - self.mean) / self.std
+        dev = self.device
+        self.X = torch.from_numpy(np.ascontiguousarray(Xs, dtype=np.float32)).to(dev)
+        self.T = torch.from_numpy(np.ascontiguousarray(T, dtype=np.float32)).to(dev)
+        self.W = torch.from_numpy(np.ascontiguousarray(W, dtype=np.float32)).to(dev)
+        self.wsum = float(np.sum(W))
+        self.net = ClosureNet(init_params(seed) if params is None else params, device=dev)
+        self.m = torch.zeros(NP, dtype=torch.float32, device=dev)
+        self.v = torch.zeros(NP, dtype=torch.float32, device=dev)
+        self.blocks = int(blocks)
+        lib = _lib.load()
+        self.partial = torch.empty(lib.closure_partial_bytes(self.blocks) // 4, dtype=torch.float32,
+                                   device=dev)
+        self.grad = torch.empty(NP + 1, dtype=torch.float32, device=dev)
+        self.lr = lr
+        self.t = 0
+
+    def gradient(self, lo: int = 0, hi=None):
+        """(grad, loss) over samples lo..hi with the current parameters."""
+        hi = self.X.shape[0] if hi is None else hi
+        n = hi - lo
+        wsum = float(self.W[lo:hi].sum().item())
+        _lib.call("closure_grad", C.c_void_p(self.net.packed.data_ptr()),
+                  C.c_void_p(self.net.params.data_ptr()), n,
+                  C.c_void_p(self.X[lo:hi].data_ptr()), C.c_void_p(self.T[lo:hi].data_ptr()),
+                  C.c_void_p(self.W[lo:hi].data_ptr()), C.c_float(wsum),
+                  C.c_void_p(self.partial.data_ptr()), self.blocks,
+                  C.c_void_p(self.grad.data_ptr()), self.net._st())
+        return self.grad
+
+    def step(self, lo: int = 0, hi=None) -> None:
+        self.gradient(lo, hi)
+        self.t += 1
+        _lib.call("closure_adam", C.c_void_p(self.net.params.data_ptr()),
+                  C.c_void_p(self.m.data_ptr()), C.c_void_p(self.v.data_ptr()),
+                  C.c_void_p(self.grad.data_ptr()), C.c_float(self.lr), C.c_float(0.9),
+                  C.c_float(0.999), C.c_float(1e-8), self.t, self.net._st())
+        self.net.repack()
+
+    def loss(self) -> float:
+        return float(self.gradient()[NP].item())
+
+    def fit(self, epochs: int = 50, batch: int = 16384, verbose: bool = False):
+        """Minibatch epochs over a fixed shuffled order (deterministic)."""
+        import torch
+        n = self.X.shape[0]
+        order = torch.from_numpy(np.random.default_rng(0).permutation(n)).to(self.device)
+        self.X, self.T, self.W = self.X[order].contiguous(), self.T[order].contiguous(), \
+            self.W[order].contiguous()
+        hist = []
+        for e in range(epochs):
+            for lo in range(0, n, batch):
+                self.step(lo, min(n, lo + batch))
+            hist.append(self.loss())
+            if verbose:
+                print(f"[closure] epoch {e + 1}: loss {hist[-1]:.4e}", file=sys.stderr, flush=True)
+        return hist
+
+    def export(self) -> ClosureNet:
+        """The trained network on raw features (standardisation folded in)."""
+        p = fold_standardization(self.net.numpy(), self.mean, self.std)
+        return ClosureNet(p, device=self.device)
+
+
+# ---------------------------------------------------------------------------
+# training data from the method's own refined runs (host-side preprocessing)
+# ---------------------------------------------------------------------------
+def box_filter(f_fine: np.ndarray) -> np.ndarray:
+    """(2X, 2Y, 2Z, 19) -> (X, Y, Z, 19): mean over each 2x2x2 block."""
+    a = np.asarray(f_fine, dtype=np.float64)
+    X, Y, Z = a.shape[0] // 2, a.shape[1] // 2, a.shape[2] // 2
+    a = a[:2 * X, :2 * Y, :2 * Z].reshape(X, 2, Y, 2, Z, 2, 19)
+    return a.mean(axis=(1, 3, 5))
+
+
+def _moments(f):
+    """Central moments about u = m / rho (no force) of f (..., 19), raw-moment order."""
+    rho = f.sum(-1)
+    u = (f * EX).sum(-1) / rho
+    v = (f * EY).sum(-1) / rho
+    w = (f * EZ).sum(-1) / rho
+    return rho, u, v, w
+
+
+def features_and_targets(f_now: np.ndarray, f_next: np.ndarray, mask=None):
+    """Features (n, 19), target outputs y = s - 1 (n, 4) and weights (n,) from
+    two consecutive filtered states on the coarse lattice (x, y, z periodic
+    neighbours; cells listed by ``mask``).  Force is taken as zero, so the
+    cells should lie away from the body."""
+    f = np.asarray(f_now, dtype=np.float64)
+    fn = np.asarray(f_next, dtype=np.float64)
+    # ideal post-collision populations: f*_q(x) = f_next_q(x + e_q)
+    fs = np.empty_like(fn)
+    for q in range(19):
+        fs[..., q] = np.roll(fn[..., q], shift=(-EX[q], -EY[q], -EZ[q]), axis=(0, 1, 2))
+    if mask is None:
+        mask = np.ones(f.shape[:3], dtype=bool)
+    f, fs = f[mask], fs[mask]
+    rho, u, v, w = _moments(f)
+    xi = [EX - u[:, None], EY - v[:, None], EZ - w[:, None]]
+    pre = np.empty((len(f), 19))
+    post = np.empty((len(f), 19))
+    for i, (p, q, r) in enumerate(PQR):
+        k = xi[0] ** p * xi[1] ** q * xi[2] ** r
+        pre[:, i] = (f * k).sum(-1) / rho
+        post[:, i] = (fs * k).sum(-1) / rho
+    X = pre - FEATURE_EQ
+    X[:, 0] = rho - 1.0
+    Dpre = pre - FEATURE_EQ
+    Dpost = post - FEATURE_EQ
+    T = np.zeros((len(f), 4))
+    W = np.zeros(len(f))
+    for j, idx in enumerate(GROUPS):
+        num = (Dpre[:, idx] * Dpost[:, idx]).sum(1)
+        den = (Dpre[:, idx] ** 2).sum(1)
+        s = 1.0 - num / np.maximum(den, 1e-300)
+        T[:, j] = np.clip(s, 0.0, 2.0) - 1.0
+        W += den
+    return X, T, W / max(W.mean(), 1e-300)
+
+
+def one_step_error(cfg, f_now, f_next, net=None, mask=None, device=None):
+    """A-priori check: relative L2 error, over ``mask``, of one coarse step
+    (plain, or with ``net``) from the filtered state f_now against the filtered
+    state one coarse step later.  x is periodic here (no planes)."""
+    import torch
+    from .clbm_moving import CLBMSolver
+    s = CLBMSolver(cfg, device=device, inlet=False)
+    s.f.from_numpy(f_now)
+    if net is not None:
+        s.closure = net
+    s.step()
+    got = s.f.to_numpy().astype(np.float64)
+    ref = np.asarray(f_next, dtype=np.float64)
+    m = np.ones(ref.shape[:3], bool) if mask is None else mask
+    return float(np.sqrt(((got - ref)[m] ** 2).sum() / ((ref - 0)[m] ** 2).sum()))

@@ -8,7 +8,7 @@
 // reads them as post-stream values (AA pattern: in place, one buffer).  The
 // macroscopic fields (K2, K7) live in registers; K7 is materialised only when
 // someone reads rho/u.
-#include "clbm_collide.h"
+#include "clbm_step.h"
 
 #include <algorithm>
 #include <cmath>
@@ -83,34 +83,6 @@ __global__ __launch_bounds__(BLK) void k_init_feq(Geo g, typename Store<S>::T* _
 // PAR: parity of the incoming state.  FORCE: force fields present.  BOX: the
 // force is read only inside box[0..5].
 // ---------------------------------------------------------------------------
-template <typename R, bool FORCE, bool BOX>
-__device__ __forceinline__ void force_at(int c, int x, int y, int z, const float* __restrict__ fx,
-                                         const float* __restrict__ fy,
-                                         const float* __restrict__ fz,
-                                         const int32_t* __restrict__ box, R& Fx, R& Fy, R& Fz) {
-  Fx = Fy = Fz = R(0);
-  if (!FORCE) return;
-  if (BOX) {
-    const bool in = (x >= box[0]) & (y >= box[1]) & (z >= box[2]) & (x <= box[3]) &
-                    (y <= box[4]) & (z <= box[5]);
-    if (!in) return;
-  }
-  if (fx) Fx = R(fx[c]);
-  if (fy) Fy = R(fy[c]);
-  if (fz) Fz = R(fz[c]);
-}
-
-// Everything one launch of the fused step needs for one lattice.
-struct StepArgs {
-  Geo g;
-  double s_v, s_e;
-  Planes pl;
-  void* f;
-  const float *fx, *fy, *fz;
-  const int32_t* box;
-  Region rg;
-};
-
 // One thread of the fused step: t is the thread's index in the launch over
 // this lattice.  REG selects the cells: 0 every cell; 1 the cells outside
 // region rg (one thread per cell of the grid); 2 the cells inside it (one

@@ -315,6 +315,42 @@ int ibm_batch_compute(const ibm_batch_t* b, int state, void* stream);
 /* counter += 1 (after the lattices have stepped). */
 int ibm_batch_advance(const ibm_batch_t* b, void* stream);
 
+/* ---- learned closure (no reference counterpart: the reference fixes the
+ * third- and fourth-order rates at 1, clbm_moving.py:188-196) -------------
+ * A per-cell network 19 -> 32 -> 32 -> 4 (GELU in tanh form) maps the cell's
+ * 19 features (central moments about the half-force velocity divided by
+ * rho, minus their equilibrium values, in the raw-moment order of
+ * config_moving.py:62-68) to y; the rates are s = 1 + y for (k210, k120),
+ * (k201, k021, k102, k012), (k220), (k202, k022): third-order moments relax as
+ * (1 - s) k, fourth-order ones as rho/9 + (1 - s)(k - rho/9).  A cell whose
+ * rates leave [lo, hi] takes the plain cascade.  Parameters: one float32
+ * vector of closure_param_count() = 1828 entries, W1 (19 x 32, [in][out]),
+ * b1, W2 (32 x 32), b2, W3 (32 x 4), b3.  Products run on bf16 MFMA (weights
+ * and activations rounded to bf16, float32 accumulation and biases); the
+ * kernels read the weights through a packed copy (closure_pack). */
+int closure_param_count(void);
+size_t closure_packed_bytes(void);
+/* params (device, float32) -> packed (device, closure_packed_bytes()). */
+int closure_pack(const float* params, void* packed, void* stream);
+/* Y[n][4] = network(X[n][19]) (device). */
+int closure_infer(const void* packed, const float* params, int count, const float* X, float* Y,
+                  void* stream);
+/* clbm_step with the learned rates (storage f32, f16 or f16c). */
+int clbm_step_closure(const clbm_grid_t* g, const clbm_params_t* p, void* f, int state,
+                      const float* fx, const float* fy, const float* fz, const int32_t* force_box,
+                      const void* packed, const float* params, float lo, float hi, void* stream);
+/* Bytes of the partial-gradient buffer of `blocks` workgroups. */
+size_t closure_partial_bytes(int blocks);
+/* grad[0..1827] = d/dparams of L = sum_n W[n] |network(X[n]) - T[n]|^2 / wsum,
+ * grad[1828] = L, over count samples with `blocks` workgroups (fixed
+ * summation order: the same inputs give the same bits). */
+int closure_grad(const void* packed, const float* params, int count, const float* X,
+                 const float* T, const float* W, float wsum, float* partial, int blocks,
+                 float* grad, void* stream);
+/* One Adam step (bias-corrected, step number t >= 1) on params with moments m, v. */
+int closure_adam(float* params, float* m, float* v, const float* grad, float lr, float beta1,
+                 float beta2, float eps, int t, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -139,7 +139,7 @@ def macro_post(f):
     return r, mx / r, my / r, mz / r
 
 
-def collide(f, rho, ux, uy, uz, fx, fy, fz, s_v: float, s_e: float = 1.0):
+def collide(f, rho, ux, uy, uz, fx, fy, fz, s_v: float, s_e: float = 1.0, rates=None):
     """K3 ``_collide`` (clbm_moving.py:120-249).
 
     Step 1 (:138-168): 15 central moments about u.  Step 2 (:170-196): shear
@@ -148,6 +148,12 @@ def collide(f, rho, ux, uy, uz, fx, fy, fz, s_v: float, s_e: float = 1.0):
     -> rho/9 (s_q = s_pi = 1 are hard-coded there).  Step 3 (:198-231): central
     -> raw moments T.  Step 4 (:233-238): f* = M_inv T with the float32 M_inv.
     Step 5 (:240-249): Guo forcing with coefficient (1 - s_v/2).
+
+    ``rates`` (the learned-closure extension; None = the reference): a tuple
+    (s3xy, s3z, s4xy, s4z) of scalars or per-cell arrays.  Third-order central
+    moments relax as k3* = (1 - s3) k3 (s3xy for k210, k120; s3z for k201,
+    k021, k102, k012) and fourth-order ones as k4* = rho/9 + (1 - s4)(k4 -
+    rho/9) (s4xy for k220; s4z for k202, k022).  All ones is the reference.
     """
     dt = f.dtype.type
     u, v, w, r = ux, uy, uz, rho
@@ -196,10 +202,19 @@ def collide(f, rho, ux, uy, uz, fx, fy, fz, s_v: float, s_e: float = 1.0):
     k200s = (kAs + kDs + kEs) / three
     k020s = (kAs - two * kDs + kEs) / three
     k002s = (kAs + kDs - two * kEs) / three
-    k210s = k120s = k201s = k021s = k102s = k012s = z
-    k220s = r / nine
-    k202s = r / nine
-    k022s = r / nine
+    if rates is None:
+        k210s = k120s = k201s = k021s = k102s = k012s = z
+        k220s = r / nine
+        k202s = r / nine
+        k022s = r / nine
+    else:
+        s3xy, s3z, s4xy, s4z = (np.asarray(a, dtype=dt) for a in rates)
+        k210s, k120s = (one - s3xy) * k210, (one - s3xy) * k120
+        k201s, k021s = (one - s3z) * k201, (one - s3z) * k021
+        k102s, k012s = (one - s3z) * k102, (one - s3z) * k012
+        k220s = r / nine + (one - s4xy) * (k220 - r / nine)
+        k202s = r / nine + (one - s4z) * (k202 - r / nine)
+        k022s = r / nine + (one - s4z) * (k022 - r / nine)
     # central -> raw (clbm_moving.py:198-223)
     T = [
         r, u * r, v * r, w * r,
@@ -275,14 +290,18 @@ def bc_outlet(f):
     return f
 
 
-def lbm_step(f, fx, fy, fz, s_v: float, s_e: float, ux_in: float, uy_in: float):
+def lbm_step(f, fx, fy, fz, s_v: float, s_e: float, ux_in: float, uy_in: float, rates=None,
+             inlet: bool = True):
     """``CLBMSolver.step`` (clbm_moving.py:326-343): K2 -> K3 -> K4 -> K5 -> K6 -> K7.
-    Returns (f_new, rho, ux, uy, uz) with the K7 (force-free) macroscopic fields."""
+    Returns (f_new, rho, ux, uy, uz) with the K7 (force-free) macroscopic fields.
+    ``rates``: per-cell higher-order rates (see ``collide``); ``inlet`` False
+    leaves x periodic."""
     rho, ux, uy, uz = macro_with_force(f, fx, fy, fz)
-    f_post = collide(f, rho, ux, uy, uz, fx, fy, fz, s_v, s_e)
+    f_post = collide(f, rho, ux, uy, uz, fx, fy, fz, s_v, s_e, rates=rates)
     f_new = stream(f_post)
-    bc_inlet(f_new, ux_in, uy_in)
-    bc_outlet(f_new)
+    if inlet:
+        bc_inlet(f_new, ux_in, uy_in)
+        bc_outlet(f_new)
     rho, ux, uy, uz = macro_post(f_new)
     return f_new, rho, ux, uy, uz
 
@@ -527,3 +546,89 @@ def run_coupled(f, X0, Y0, Z0, steps, motion, s_v, s_e, ux_in, uy_in,
         f, rho, ux, uy, uz = lbm_step(f, fx, fy, fz, s_v, s_e, ux_in, uy_in)
         hist.append(force_sum_f64(out["FX"], out["FY"], dV))
     return f, np.array(hist, dtype=np.float64)
+
+
+# --------------------------------------------------------------------------
+# Learned closure (the build's own extension; no reference counterpart).
+# --------------------------------------------------------------------------
+# The 19 features of a cell: its moments about the half-force velocity u,
+# divided by rho, minus their equilibrium values, in the raw-moment order of
+# PQR: rho - 1, three first central moments (= -F/2/rho), k110, k101, k011,
+# k200 - 1/3, k020 - 1/3, k002 - 1/3, the six third-order moments, and
+# k220 - 1/9, k202 - 1/9, k022 - 1/9.
+FEATURE_EQ = np.array([1.0, 0, 0, 0, 0, 0, 0, 1 / 3, 1 / 3, 1 / 3, 0, 0, 0, 0, 0, 0, 1 / 9, 1 / 9,
+                       1 / 9])
+
+
+def central_features(f, fx, fy, fz):
+    """(..., 19) float64 features of the populations f (..., 19) with force
+    (fx, fy, fz): central moments about u = (sum e f + F/2) / rho."""
+    f = np.asarray(f, dtype=np.float64)
+    rho = f.sum(-1)
+    u = ((f * EX).sum(-1) + 0.5 * np.asarray(fx, np.float64)) / rho
+    v = ((f * EY).sum(-1) + 0.5 * np.asarray(fy, np.float64)) / rho
+    w = ((f * EZ).sum(-1) + 0.5 * np.asarray(fz, np.float64)) / rho
+    xi = [EX - u[..., None], EY - v[..., None], EZ - w[..., None]]
+    out = np.empty(f.shape[:-1] + (Q,))
+    for i, (p, q, r) in enumerate(PQR):
+        out[..., i] = (f * xi[0] ** p * xi[1] ** q * xi[2] ** r).sum(-1) / rho
+    out[..., 0] = rho
+    return out - FEATURE_EQ
+
+
+def gelu(x):
+    """GELU, tanh form: x/2 (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3)))."""
+    return 0.5 * x * (1.0 + np.tanh(0.7978845608028654 * (x + 0.044715 * x ** 3)))
+
+
+def gelu_grad(x):
+    t = np.tanh(0.7978845608028654 * (x + 0.044715 * x ** 3))
+    return 0.5 * (1.0 + t) + 0.5 * x * (1.0 - t * t) * 0.7978845608028654 * (1.0 + 3 * 0.044715 * x * x)
+
+
+def mlp_forward(params, X):
+    """19 -> 32 -> 32 -> 4 with GELU: params (W1 (19,32), b1, W2 (32,32), b2,
+    W3 (32,4), b3); returns (Y, cache)."""
+    W1, b1, W2, b2, W3, b3 = params
+    z1 = X @ W1 + b1
+    h1 = gelu(z1)
+    z2 = h1 @ W2 + b2
+    h2 = gelu(z2)
+    Y = h2 @ W3 + b3
+    return Y, (X, z1, h1, z2, h2)
+
+
+def mlp_backward(params, cache, dY):
+    """Gradients of sum(dY * Y) with respect to the parameters."""
+    W1, b1, W2, b2, W3, b3 = params
+    X, z1, h1, z2, h2 = cache
+    gW3 = h2.T @ dY
+    gb3 = dY.sum(0)
+    dh2 = dY @ W3.T
+    dz2 = dh2 * gelu_grad(z2)
+    gW2 = h1.T @ dz2
+    gb2 = dz2.sum(0)
+    dh1 = dz2 @ W2.T
+    dz1 = dh1 * gelu_grad(z1)
+    gW1 = X.T @ dz1
+    gb1 = dz1.sum(0)
+    return gW1, gb1, gW2, gb2, gW3, gb3
+
+
+def closure_loss(params, X, T, wgt):
+    """Weighted mean squared rate error sum_n w_n |Y_n - T_n|^2 / sum_n w_n and its gradient."""
+    Y, cache = mlp_forward(params, X)
+    d = Y - T
+    W = wgt.sum()
+    loss = float((wgt[:, None] * d * d).sum() / W)
+    grads = mlp_backward(params, cache, 2.0 * wgt[:, None] * d / W)
+    return loss, grads
+
+
+def closure_rates(Y, fallback_lo=0.0, fallback_hi=2.0):
+    """Rates s = 1 + Y; a cell whose rates leave [lo, hi] takes the plain
+    cascade (all ones)."""
+    s = 1.0 + np.asarray(Y, dtype=np.float64)
+    bad = np.any((s < fallback_lo) | (s > fallback_hi), axis=-1)
+    s[bad] = 1.0
+    return s

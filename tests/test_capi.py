@@ -17,7 +17,7 @@ def declared_functions():
     text = open(HEADER).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     names = re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s*([a-z_0-9]+)\s*\(", text, flags=re.M)
-    return sorted(set(n for n in names if n.startswith(("clbm_", "ibm_"))))
+    return sorted(set(n for n in names if n.startswith(("clbm_", "ibm_", "closure_"))))
 
 
 def test_header_declares_the_expected_surface():
