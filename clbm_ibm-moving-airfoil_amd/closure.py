@@ -260,7 +260,9 @@ def features_and_targets(f_now: np.ndarray, f_next: np.ndarray, mask=None):
         s = 1.0 - num / np.maximum(den, 1e-300)
         T[:, j] = np.clip(s, 0.0, 2.0) - 1.0
         W += den
-    return X, T, W / max(W.mean(), 1e-300)
+    ok = np.all(np.isfinite(X), 1) & np.all(np.isfinite(T), 1) & np.isfinite(W)
+    X, T, W = X[ok], T[ok], W[ok]
+    return X, T, W / max(W.mean(), 1e-300) if len(W) else W
 
 
 def one_step_error(cfg, f_now, f_next, net=None, mask=None, device=None):
@@ -278,3 +280,190 @@ def one_step_error(cfg, f_now, f_next, net=None, mask=None, device=None):
     ref = np.asarray(f_next, dtype=np.float64)
     m = np.ones(ref.shape[:3], bool) if mask is None else mask
     return float(np.sqrt(((got - ref)[m] ** 2).sum() / ((ref - 0)[m] ** 2).sum()))
+
+
+# ---------------------------------------------------------------------------
+# the study: refined runs -> data -> training -> evaluation
+# ---------------------------------------------------------------------------
+def case_for(seed: int, coarse, chord: float, Re: float, refine: int = 1):
+    """Seeded kinematics (as the batched sweeps draw them) on the coarse grid
+    ``coarse`` (refine = 1) or its 2x refinement (refine = 2, twice the chord,
+    same Re, kinematics at the same physical times)."""
+    from . import config as cfgmod
+    rng = np.random.default_rng(7919 * seed + 17)
+    kw = dict(k=float(rng.uniform(0.25, 1.0)), heave_frac=float(rng.uniform(0.1, 0.5)),
+              aoa_mean_deg=float(rng.uniform(0, 12)), pitch_amp_deg=float(rng.uniform(0, 20)),
+              phase_deg=float(rng.uniform(0, 180)))
+    nx, ny, nz = coarse
+    return cfgmod.naca_case(refine * nx, refine * ny, refine * nz, refine * chord, Re=Re,
+                            le_x=refine * nx / 4.0, name=f"seed{seed}-x{refine}", **kw)
+
+
+def _far_mask(cfg_coarse, ibm_box, margin: int = 3):
+    nx, ny, nz = cfg_coarse.Nx, cfg_coarse.Ny, cfg_coarse.Nz
+    m = np.ones((nx, ny, nz), dtype=bool)
+    m[:4] = False
+    m[-5:] = False
+    x0, y0, _, x1, y1, _ = ibm_box
+    m[max(0, x0 - margin):x1 + margin + 1, max(0, y0 - margin):y1 + margin + 1, :] = False
+    return m
+
+
+def fine_pairs(seed: int, coarse, chord: float, Re: float, warm: int, pairs: int, gap: int,
+               storage: str = "f32", device=None, log=None):
+    """Run the 2x case for ``warm`` fine steps, then take ``pairs`` filtered
+    (now, one coarse step later) pairs every ``gap`` fine steps.  Returns the
+    pairs and a mask of coarse cells away from the body and the x planes."""
+    from .airfoil_moving import DroneMovingIBM
+    from .clbm_moving import CLBMSolver
+    cf = case_for(seed, coarse, chord, Re, refine=2)
+    cc = case_for(seed, coarse, chord, Re, refine=1)
+    lbm = CLBMSolver(cf, storage=storage, device=device)
+    ibm = DroneMovingIBM(cf, device=device)
+    out, n = [], 0
+
+    def advance(k):
+        nonlocal n
+        for _ in range(k):
+            n += 1
+            fx, fy, fz = ibm.compute_force(lbm.ux, lbm.uy, *cf.motion_state(n))
+            lbm.step(fx, fy, fz)
+
+    advance(warm)
+    masks = []
+    for _ in range(pairs):
+        advance(gap)
+        a = box_filter(lbm.f.to_numpy())
+        box = ibm.force_box()
+        advance(2)
+        b = box_filter(lbm.f.to_numpy())
+        out.append((a, b))
+        coarse_box = tuple(int(v) // 2 for v in box)
+        masks.append(_far_mask(cc, coarse_box))
+        if log:
+            log(f"seed {seed}: pair at fine step {n}, finite {lbm.health()[1]}")
+    return out, masks
+
+
+def run_forces(cfg, steps: int, net=None, storage: str = "f32", device=None, skip: int = 0):
+    """Coupled run; returns (mean C_D, mean C_L over steps after ``skip``,
+    seconds, finite)."""
+    import torch
+    from .airfoil_moving import DroneMovingIBM
+    from .clbm_moving import CLBMSolver
+    lbm = CLBMSolver(cfg, storage=storage, device=device)
+    ibm = DroneMovingIBM(cfg, device=device)
+    lbm.closure = net
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for n in range(1, steps + 1):
+        fx, fy, fz = ibm.compute_force(lbm.ux, lbm.uy, *cfg.motion_state(n))
+        lbm.step(fx, fy, fz)
+        ibm.record_force()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    cd, cl = ibm.coefficients(ibm.force_history())
+    w = slice(skip, None)
+    return float(np.mean(cd[w])), float(np.mean(cl[w])), dt, bool(lbm.health()[1])
+
+
+def study(coarse=(300, 100, 24), chord=None, Re=1000.0, train_seeds=range(8), test_seeds=range(100, 104),
+          warm=4000, pairs=3, gap=400, samples=600000, epochs=40, eval_steps=4000, device=None,
+          log=None):
+    """The full pipeline at a given scale; returns a dict of results."""
+    chord = coarse[1] / 2.5 if chord is None else chord
+    log = log or (lambda s: print(f"[closure] {s}", file=sys.stderr, flush=True))
+    res = {"coarse": list(coarse), "fine": [2 * v for v in coarse], "chord": chord, "Re": Re}
+    Xs, Ts, Ws = [], [], []
+    rng = np.random.default_rng(0)
+    for s in train_seeds:
+        pr, masks = fine_pairs(s, coarse, chord, Re, warm, pairs, gap, device=device, log=log)
+        for (a, b), m in zip(pr, masks):
+            X, T, W = features_and_targets(a, b, m)
+            Xs.append(X)
+            Ts.append(T)
+            Ws.append(W)
+    X, T, W = np.concatenate(Xs), np.concatenate(Ts), np.concatenate(Ws)
+    if len(X) > samples:      # keep the cells with the most non-equilibrium signal
+        p = W / W.sum()
+        idx = rng.choice(len(X), samples, replace=False, p=p)
+        X, T, W = X[idx], T[idx], np.ones(samples)
+    res["train_samples"] = int(len(X))
+    log(f"training on {len(X)} samples")
+    tr = Trainer(X, T, W, device=device, lr=3e-3)
+    hist = tr.fit(epochs=epochs, batch=16384, verbose=False)
+    res["loss_first_last"] = [float(hist[0]), float(hist[-1])]
+    res["loss_plain"] = float(np.average((T ** 2).sum(1), weights=W))
+    net = tr.export()
+    # a-priori on held-out kinematics: the rate loss of the targets and one
+    # coarse step from filtered states
+    errs = {"plain": [], "closure": []}
+    rate_loss = {"plain": [], "closure": []}
+    for s in test_seeds:
+        pr, masks = fine_pairs(s, coarse, chord, Re, warm, 1, gap, device=device, log=log)
+        cfg = case_for(s, coarse, chord, Re)
+        for (a, b), m in zip(pr, masks):
+            errs["plain"].append(one_step_error(cfg, a, b, None, m, device))
+            errs["closure"].append(one_step_error(cfg, a, b, net, m, device))
+            Xh, Th, Wh = features_and_targets(a, b, m)
+            Yh = net.infer(Xh).cpu().numpy().astype(np.float64)
+            rate_loss["plain"].append(float(np.average((Th ** 2).sum(1), weights=Wh)))
+            rate_loss["closure"].append(float(np.average(((Yh - Th) ** 2).sum(1), weights=Wh)))
+    res["one_step_rel_error"] = {k: float(np.mean(v)) for k, v in errs.items()}
+    res["heldout_rate_loss"] = {k: float(np.mean(v)) for k, v in rate_loss.items()}
+    log(f"one-step error: {res['one_step_rel_error']}")
+    # a-posteriori: coarse with / without the network against the fine run
+    post = []
+    for s in test_seeds:
+        cc = case_for(s, coarse, chord, Re)
+        cf = case_for(s, coarse, chord, Re, refine=2)
+        fd, fl, ft, fok = run_forces(cf, 2 * eval_steps, device=device, skip=eval_steps)
+        pd, pl_, pt, pok = run_forces(cc, eval_steps, device=device, skip=eval_steps // 2)
+        nd, nl, nt, nok = run_forces(cc, eval_steps, net=net, device=device, skip=eval_steps // 2)
+        post.append(dict(seed=s, fine=[fd, fl, ft, fok], plain=[pd, pl_, pt, pok],
+                         closure=[nd, nl, nt, nok]))
+        log(f"seed {s}: fine CD {fd:.4f} CL {fl:.4f} ({ft:.1f} s) | plain {pd:.4f} {pl_:.4f} "
+            f"| closure {nd:.4f} {nl:.4f} ({nt:.1f} s)")
+    res["a_posteriori"] = post
+
+    def relerr(key, i):
+        return float(np.mean([abs(r[key][i] - r["fine"][i]) / max(abs(r["fine"][i]), 1e-12)
+                              for r in post]))
+    res["mean_rel_error_vs_fine"] = {k: {"CD": relerr(k, 0), "CL": relerr(k, 1)}
+                                    for k in ("plain", "closure")}
+    res["wall_time_ratio_fine_over_closure"] = float(
+        np.mean([r["fine"][2] / r["closure"][2] for r in post]))
+    return res, net
+
+
+def main(argv=None) -> int:
+    import argparse
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--coarse", default="300,100,24")
+    ap.add_argument("--re", type=float, default=1000.0)
+    ap.add_argument("--train-seeds", type=int, default=8)
+    ap.add_argument("--test-seeds", type=int, default=4)
+    ap.add_argument("--warm", type=int, default=4000)
+    ap.add_argument("--pairs", type=int, default=3)
+    ap.add_argument("--gap", type=int, default=400)
+    ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--eval-steps", type=int, default=4000)
+    ap.add_argument("--json", default=None)
+    ap.add_argument("--save", default=None, help="write the trained network (.npz)")
+    a = ap.parse_args(argv)
+    coarse = tuple(int(v) for v in a.coarse.split(","))
+    res, net = study(coarse, Re=a.re, train_seeds=range(a.train_seeds),
+                     test_seeds=range(100, 100 + a.test_seeds), warm=a.warm, pairs=a.pairs,
+                     gap=a.gap, epochs=a.epochs, eval_steps=a.eval_steps)
+    if a.save:
+        net.save(a.save, coarse=list(coarse), Re=a.re)
+    text = json.dumps(res, indent=1)
+    if a.json:
+        with open(a.json, "w") as fh:
+            fh.write(text)
+    print(text, flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

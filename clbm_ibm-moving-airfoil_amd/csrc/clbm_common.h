@@ -75,37 +75,27 @@ template <> struct Store<CLBM_STORE_BF16> {
   __device__ static void st(T* p, R v, float b) { *p = __float2bfloat16(v - b); }
 };
 
-// 16-bit float for deviations |x| < 1: sign, 4 exponent bits, 11 mantissa
-// bits.  Exponent field e = 1..15 encodes (1 + m / 2^11) 2^(e - 16); e = 0
-// encodes m 2^-26 (subnormal, gradual underflow); 0x7fff (either sign) is
-// NaN, which every |x| >= 1, infinity or NaN becomes.  Rounding is to
-// nearest, ties to even.  Relative precision 2^-12 for |x| >= 2^-15 (twice
-// IEEE half's), absolute 2^-27 below.
-// Both directions are branch-free (selects), so lanes never diverge on the
-// format's cases.  Below 2^-15 the codes are the float32 bits of 0.125 + |v|
-// minus those of 0.125: that sum's unit in the last place is 2^-26, so the
-// hardware addition does the round-to-nearest-even.
+// 16-bit code for deviations |x| < 2: a sign bit and a 15-bit code u for
+// the magnitude.  u is the 12-significant-bit float F = |x| + 2^-15 with
+// F's exponent stored in 4 bits (F in [2^-15, 2)) and 11 mantissa bits:
+// encode rounds fl32(|x| + 2^-15) to 12 significant bits (nearest, ties to
+// even), decode returns F(u) - 2^-15 (exact in float32).  Steps are 2^-26
+// below 2^-15 and at most 2^-11 of the value above it (|x| >= 2^-13: 2^-12,
+// twice IEEE half's precision).  0x7fff (either sign) is NaN, which every
+// value rounding to F >= 2 - 2^-11, infinity or NaN becomes.  Both
+// directions are branch-free.
 __device__ __forceinline__ uint16_t f16c_encode(float v) {
-  constexpr uint32_t k8 = 0x3e000000u;  // float32 bits of 0.125
   const uint32_t b = __float_as_uint(v);
   const uint32_t ab = b & 0x7fffffffu;
-  // normal: round the float32 mantissa at bit 12 (ties to even), rebias 127 -> 16
-  const uint32_t r = ab + 0x7ffu + ((ab >> 12) & 1u);
-  const uint32_t hn = (r >> 12) - (111u << 11);
-  const uint32_t hs = __float_as_uint(__uint_as_float(ab) + 0.125f) - k8;
-  // |v| >= 1, inf and NaN become 0x7fff; a value rounding up to that code is
-  // NaN by the same code
-  uint32_t h = ab < 0x38000000u ? hs : hn;
-  h = ab >= 0x3f800000u ? 0x7fffu : h;
+  const uint32_t t = __float_as_uint(__uint_as_float(ab) + 3.0517578125e-05f);  // + 2^-15
+  const uint32_t r = t + 0x7ffu + ((t >> 12) & 1u);
+  const uint32_t h = min((r - (112u << 23)) >> 12, 0x7fffu);  // too large, inf, NaN -> 0x7fff
   return (uint16_t)(((b >> 16) & 0x8000u) | h);
 }
 __device__ __forceinline__ float f16c_decode(uint16_t hv) {
-  constexpr uint32_t k8 = 0x3e000000u;
   const uint32_t h = hv;
   const uint32_t u = h & 0x7fffu;
-  const float an = __uint_as_float((u << 12) + (111u << 23));
-  const float as = __uint_as_float(k8 + u) - 0.125f;  // u < 2^11 here: u 2^-26 exactly
-  float a = u >= 0x800u ? an : as;
+  float a = __uint_as_float((u << 12) + (112u << 23)) - 3.0517578125e-05f;
   a = u == 0x7fffu ? __uint_as_float(0x7fc00000u) : a;
   return __uint_as_float(__float_as_uint(a) | ((h & 0x8000u) << 16));
 }

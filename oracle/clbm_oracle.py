@@ -28,7 +28,7 @@ Pinning (what checks each function, in ``tests/test_oracle_*.py``):
 * ``marker_forces`` / ``clip_forces``           -- closed form
 * ``force_sum_f64``                             -- exact FP64 sum (math.fsum)
 * ``f16c_encode`` / ``f16c_decode``             -- known codes, full round trip, monotonicity,
-                                                   brute-force nearest rounding with ties to even
+                                                   brute-force nearest rounding of |v| + 2^-15
 * ``motion_state``                              -- closed form derivatives
 
 Taichi itself is not installed here, so bit-level agreement with a Taichi
@@ -311,37 +311,27 @@ def lbm_step(f, fx, fy, fz, s_v: float, s_e: float, ux_in: float, uy_in: float, 
 # reference stores float32).  Restated with numpy integer operations.
 # --------------------------------------------------------------------------
 def f16c_encode(v):
-    """float32 -> uint16: sign, 4-bit exponent e (value (1 + m/2^11) 2^(e-16)
-    for e = 1..15), 11-bit mantissa m; e = 0 holds m 2^-26; 0x7fff is NaN,
-    which |v| >= 1, inf and NaN become.  Round to nearest, ties to even."""
+    """float32 -> uint16: sign bit and 15-bit code u of F = fl32(|v| + 2^-15)
+    rounded to 12 significant bits (nearest, ties to even), F's exponent in 4
+    bits (F in [2^-15, 2)) and 11 mantissa bits; 0x7fff is NaN, which values
+    rounding to F >= 2 - 2^-11, inf and NaN become."""
     v = np.asarray(v, dtype=np.float32)
     b = v.view(np.uint32).astype(np.uint64)
     sign = (b >> 16) & 0x8000
     a = np.abs(v)
-    nan = ~(a < 1.0)
-    sub = (a < 2.0 ** -15) & ~nan
-    nor = ~sub & ~nan
-    out = np.zeros(v.shape, dtype=np.uint64)
-    out[sub] = np.rint(a[sub].astype(np.float64) * 2.0 ** 26).astype(np.uint64)
-    ab = b[nor] & 0x7FFFFFFF
-    r = ab + 0x7FF + ((ab >> 12) & 1)
-    e = (r >> 23).astype(np.int64) - 127 + 16
-    h = (np.clip(e, 0, 15).astype(np.uint64) << 11) | ((r >> 12) & 0x7FF)
-    h[(e > 15) | (h == 0x7FFF)] = 0x7FFF
-    out[nor] = h
-    out[nan] = 0x7FFF
-    return (out | sign).astype(np.uint16)
+    with np.errstate(invalid="ignore", over="ignore"):
+        t = (a + np.float32(2.0 ** -15)).astype(np.float32).view(np.uint32).astype(np.uint64)
+    r = t + 0x7FF + ((t >> 12) & 1)
+    h = np.minimum((r - (112 << 23)) >> 12, 0x7FFF)
+    return (h | sign).astype(np.uint16)
 
 
 def f16c_decode(h):
-    """uint16 -> float32 (inverse of f16c_encode on its codes)."""
+    """uint16 -> float32 (F(u) - 2^-15 with the sign; 0x7fff is NaN)."""
     h = np.asarray(h, dtype=np.uint16).astype(np.uint64)
     u = h & 0x7FFF
-    e = u >> 11
-    m = u & 0x7FF
-    bits = (((e + 127 - 16) << 23) | (m << 12)) & 0xFFFFFFFF
-    norm = bits.astype(np.uint32).view(np.float32)
-    a = np.where(e > 0, norm, m.astype(np.float32) * np.float32(2.0 ** -26))
+    F = (((u << 12) + (112 << 23)) & 0xFFFFFFFF).astype(np.uint32).view(np.float32)
+    a = (F - np.float32(2.0 ** -15)).astype(np.float32)
     a = np.where(u == 0x7FFF, np.float32(np.nan), a).astype(np.float32)
     return np.where(h & 0x8000, -a, a).astype(np.float32)
 

@@ -1,29 +1,31 @@
-"""The 1-4-11 16-bit deviation format of the oracle (CPU).
+"""The 16-bit deviation code of the oracle (CPU).
 
-Pinned by known codes, a round trip over all 65,536 codes, strict
-monotonicity of the decoded values, and brute-force nearest-value rounding
-(ties to even) against the table of all codes.  A planted truncating
-encoder must fail the rounding check.
+The code of |v| is F = fl32(|v| + 2^-15) rounded to 12 significant bits,
+stored as a 4-bit exponent and 11 mantissa bits; decoding gives F - 2^-15.
+Pinned by known codes, a round trip over all 32,767 finite magnitudes and
+their negatives, strict monotonicity, brute-force nearest rounding of F
+(ties to even) against the table of all codes, and the resulting error
+bounds.  A planted truncating encoder must fail the rounding check.
 """
 
 import numpy as np
-import pytest
+
+OFF = np.float32(2.0 ** -15)
 
 
 def test_known_codes(oracle):
-    enc = oracle.f16c_encode
-    vals = np.array([0.0, -0.0, 2.0 ** -16, 2.0 ** -15, 0.5, 0.75, -0.75, 2.0 ** -26,
-                     1.0, -1.0, np.inf, np.nan, 3 * 2.0 ** -27], dtype=np.float32)
-    codes = [0x0000, 0x8000, 0x0400, 0x0800, 0x7800, 0x7C00, 0xFC00, 0x0001,
-             0x7FFF, 0xFFFF, 0x7FFF, 0x7FFF, 0x0002]   # 1.5 * 2^-26 ties to the even 2
-    assert list(enc(vals)) == codes
-    dec = oracle.f16c_decode(np.array([0x0800, 0x7800, 0x7FFE, 0x0001, 0x8001], np.uint16))
-    assert list(dec) == [2.0 ** -15, 0.5, np.float32((1 + 2046 / 2048) * 0.5), 2.0 ** -26,
-                         -(2.0 ** -26)]
+    vals = np.array([0.0, -0.0, 2.0 ** -26, 2.0 ** -15, 0.5, -0.5, 1.0, 3 * 2.0 ** -27,
+                     1.9999, 2.5, np.inf, np.nan], dtype=np.float32)
+    codes = [0x0000, 0x8000, 0x0001, 0x0800, 0x7000, 0xF000, 0x7800, 0x0002,
+             0x7FFF, 0x7FFF, 0x7FFF, 0x7FFF]     # 1.5 * 2^-26 ties to the even code 2
+    assert list(oracle.f16c_encode(vals)) == codes
+    dec = oracle.f16c_decode(np.array([0x0800, 0x7000, 0x0001, 0x8001], np.uint16))
+    assert list(dec) == [2.0 ** -15, np.float32(0.5 - 2.0 ** -15), 2.0 ** -26, -(2.0 ** -26)]
+    assert np.isnan(oracle.f16c_decode(np.array([0x7FFF, 0xFFFF], np.uint16))).all()
 
 
 def _table(oracle):
-    codes = np.arange(0x7FFF, dtype=np.uint16)          # every non-negative finite code
+    codes = np.arange(0x7FFF, dtype=np.uint16)
     return codes, oracle.f16c_decode(codes)
 
 
@@ -35,37 +37,44 @@ def test_round_trip_and_monotone(oracle):
 
 
 def _check_nearest(encode, oracle, x):
+    """The decoded F is the nearest code value to fl32(|x| + 2^-15), ties to
+    the even code."""
     codes, vals = _table(oracle)
-    v = vals.astype(np.float64)
-    a = np.abs(x).astype(np.float64)
+    F = vals.astype(np.float64) + 2.0 ** -15
+    t = (np.abs(x).astype(np.float32) + OFF).astype(np.float64)
     got = oracle.f16c_decode(encode(x)).astype(np.float64)
-    i = np.clip(np.searchsorted(v, a), 1, len(v) - 1)
-    lo, hi = v[i - 1], v[i]
-    dlo, dhi = a - lo, hi - a
+    i = np.clip(np.searchsorted(F, t), 1, len(F) - 1)
+    lo, hi = F[i - 1], F[i]
+    dlo, dhi = t - lo, hi - t
     want = np.where(dlo < dhi, lo, np.where(dhi < dlo, hi, np.where(codes[i - 1] % 2 == 0, lo, hi)))
-    return np.array_equal(np.abs(got), want)
+    return np.array_equal(np.abs(got) + 2.0 ** -15, want)
 
 
 def test_rounding_is_nearest_ties_even(oracle):
     rng = np.random.default_rng(0)
-    x = np.concatenate([rng.uniform(-0.99, 0.99, 200000),
+    x = np.concatenate([rng.uniform(-1.9, 1.9, 200000),
                         rng.uniform(-2e-4, 2e-4, 50000)]).astype(np.float32)
-    # exact midpoints between neighbouring codes
     _, vals = _table(oracle)
-    mid = ((vals[1:4000].astype(np.float64) + vals[:3999]) / 2).astype(np.float32)
-    x = np.concatenate([x, mid[mid.astype(np.float64) == (vals[1:4000].astype(np.float64) + vals[:3999]) / 2]])
+    F = vals[:4000].astype(np.float64) + 2.0 ** -15
+    mid = (F[1:] + F[:-1]) / 2 - 2.0 ** -15          # magnitudes whose F is an exact tie
+    mid32 = mid.astype(np.float32)
+    x = np.concatenate([x, mid32[(mid32.astype(np.float64) + 2.0 ** -15) == (F[1:] + F[:-1]) / 2]])
     assert _check_nearest(oracle.f16c_encode, oracle, x)
-    rel = np.abs(oracle.f16c_decode(oracle.f16c_encode(x)) - x)
-    big = np.abs(x) >= 2.0 ** -15
-    assert np.all(rel[big] <= 2.0 ** -12 * np.abs(x[big]))
-    assert np.all(rel[~big] <= 2.0 ** -27)
+    q = oracle.f16c_decode(oracle.f16c_encode(x)).astype(np.float64)
+    a = np.abs(x).astype(np.float64)
+    Fx = a + 2.0 ** -15
+    ulp12 = 2.0 ** (np.floor(np.log2(Fx)) - 11)
+    assert np.all(np.abs(np.abs(q) - a) <= 0.5 * ulp12 + 2.0 ** -24 * Fx)
+    small = a < 2.0 ** -15
+    assert np.all(np.abs(np.abs(q[small]) - a[small]) <= 2.0 ** -27 + 2.0 ** -39)
 
 
 def test_planted_truncation_is_caught(oracle):
     def trunc(v):
         v = np.asarray(v, np.float32)
-        t = (v.view(np.uint32) & np.uint32(0xFFFFF000)).view(np.float32)
-        return oracle.f16c_encode(t)
+        t = (np.abs(v) + OFF).astype(np.float32)
+        t = (t.view(np.uint32) & np.uint32(0xFFFFF000)).view(np.float32)
+        return oracle.f16c_encode(np.copysign((t - OFF).astype(np.float32), v))
     rng = np.random.default_rng(1)
     x = rng.uniform(-0.9, 0.9, 20000).astype(np.float32)
     assert not _check_nearest(trunc, oracle, x)
