@@ -93,7 +93,8 @@ class CLBMSolver:
     Parameters mirror the reference's import-time constants; all are optional:
     ``config`` (CaseConfig or config module; default: ``sys.modules
     ['config_moving']`` or the reference airplane case), ``storage`` one of
-    f32 (drop-in), f16, bf16 (deviation storage) or f64 (error reference),
+    f32 (drop-in), f16, bf16, f16c (16-bit deviation storage) or f64 (error
+    reference),
     ``inlet`` False for an x-periodic box, ``outside_lds`` the LDS bytes per
     workgroup the outside part of a split step reserves (a tuning choice:
     33 KiB holds that launch to 4 workgroups per CU, which leaves each SIMD

@@ -350,7 +350,7 @@ __global__ __launch_bounds__(CBLK) void k_step_closure(StepArgs a, ClosureArgs n
   using R = typename Store<S>::R;
   const Geo& g = a.g;
   const Planes& pl = a.pl;
-  const int c = xcd_block(blockIdx.x, gridDim.x) * CBLK + threadIdx.x;
+  const int c = sweep_block<PAR>(blockIdx.x, gridDim.x) * CBLK + threadIdx.x;
   int x = 0, y = 0, z = 0;
   bool act = c < g.ncell;
   if (act) cell_xyz(g, c, x, y, z);

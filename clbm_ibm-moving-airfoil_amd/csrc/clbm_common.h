@@ -240,6 +240,26 @@ __device__ __forceinline__ int xcd_block(int bid, int nblk) {
   return (xcd < rem) ? xcd * (per + 1) + loc : rem * (per + 1) + (xcd - rem) * per + loc;
 }
 
+// Block order of a lattice sweep.  Steps from parity 0 sweep the cells
+// forward, steps from parity 1 backward, so each step starts on the cells the
+// previous one finished with.  Their populations are still in the 256 MB
+// last-level cache (over half of a 6 M-cell FP32 lattice, all of a 16-bit
+// one), which cuts the HBM reads at the turn.  Both sweeps keep one front for
+// the whole chip.  The backward sweep hands each XCD (workgroup b runs on XCD
+// b % 8) runs of 32 consecutive tiles, so the rows its x-shifted loads and
+// stores share with the next tile stay in that XCD's L2.
+__device__ __forceinline__ int xcd_runs(int bid, int nblk) {
+  constexpr int K = 32, SG = 8 * K;
+  const int full = (nblk / SG) * SG;
+  if (bid >= full) return bid;
+  const int sg = bid / SG, r = bid - sg * SG;
+  return sg * SG + (r % 8) * K + r / 8;
+}
+template <int PAR>
+__device__ __forceinline__ int sweep_block(int bid, int nblk) {
+  return PAR == 0 ? bid : nblk - 1 - xcd_runs(bid, nblk);
+}
+
 // Address (element index) of post-stream population q of cell c in an AA
 // state of parity PAR: parity 0 keeps it at [q][c], parity 1 at
 // [opp(q)][c - e_q].

@@ -145,10 +145,10 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, int t) {
 
 template <int S, int PAR, bool FORCE, bool BOX, int REG>
 __global__ __launch_bounds__(BLK) void k_step(StepArgs a) {
-  step_cell<S, PAR, FORCE, BOX, REG>(a, xcd_block(blockIdx.x, gridDim.x) * BLK + threadIdx.x);
+  step_cell<S, PAR, FORCE, BOX, REG>(a, sweep_block<PAR>(blockIdx.x, gridDim.x) * BLK + threadIdx.x);
 }
 
-// Many lattices in one launch: block b (after the XCD remap) belongs to
+// Many lattices in one launch: block b (in sweep order) belongs to
 // instance inst[b] and is that instance's block b - first[inst[b]]; every
 // instance reads its force inside its force box.
 template <int S, int PAR, int BK>
@@ -156,7 +156,7 @@ __global__ __launch_bounds__(BK) void k_step_batch(const StepArgs* __restrict__ 
                                                    const uint16_t* __restrict__ inst,
                                                    const int32_t* __restrict__ first,
                                                    int planes_on) {
-  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int lb = sweep_block<PAR>(blockIdx.x, gridDim.x);
   const int i = inst[lb];
   StepArgs a = args[i];
   a.pl.on &= planes_on;
