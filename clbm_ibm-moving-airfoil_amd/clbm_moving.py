@@ -96,15 +96,17 @@ class CLBMSolver:
     f32 (drop-in), f16, bf16, f16c (16-bit deviation storage) or f64 (error
     reference),
     ``inlet`` False for an x-periodic box, ``outside_lds`` the LDS bytes per
-    workgroup the outside part of a split step reserves (a tuning choice:
-    33 KiB holds that launch to 4 workgroups per CU, which leaves each SIMD
-    room for one concurrent marker wavefront), ``deviation`` what 16-bit
+    workgroup the outside part of a split step reserves (a tuning choice that
+    leaves each CU room for a concurrent marker workgroup; by default 33 KiB,
+    4 workgroups per CU, for 4- and 8-byte populations, which are bound by
+    HBM, and 27 KiB, 5 per CU, for the 16-bit formats, whose step kernel also
+    leans on instruction issue), ``deviation`` what 16-bit
     storage subtracts before rounding: "weights" f_q - w_q, or "inflow"
     f_q - feq_q(1, inlet velocity), which is near zero wherever the flow is
     near the free stream."""
 
     def __init__(self, config=None, storage: str = "f32", device=None, inlet: bool = True,
-                 outside_lds: int = 33792, deviation: str = "weights"):
+                 outside_lds=None, deviation: str = "weights"):
         import torch
         cfg: CaseConfig = resolve(config)
         self.cfg = cfg
@@ -119,6 +121,8 @@ class CLBMSolver:
         self.deviation = deviation
         du, dv = (float(cfg.ux_in), float(cfg.uy_in)) if deviation == "inflow" else (0.0, 0.0)
         self.grid = _lib.ClbmGrid(self.Nx, self.Ny, self.Nz, _lib.STORAGE[storage], du, dv)
+        if outside_lds is None:
+            outside_lds = 27648 if _lib.ELEM_BYTES[self.grid.storage] == 2 else 33792
         self.params = _lib.ClbmParams(float(cfg.s_v), float(cfg.s_e), float(cfg.ux_in),
                                       float(cfg.uy_in), 1 if inlet else 0, int(outside_lds))
         nbytes = Q * self.Nx * self.Ny * self.Nz * _lib.ELEM_BYTES[self.grid.storage]

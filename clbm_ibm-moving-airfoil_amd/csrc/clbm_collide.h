@@ -13,14 +13,16 @@
 //  * Guo's term w_q (1 - s_v/2) [3 (e_q - u) + 9 (e_q.u) e_q] . F
 //    (clbm_moving.py:240-249) is added per direction.
 // Results agree with the reference formula to rounding; the order of the
-// floating-point operations differs.
+// floating-point operations differs.  FORCE = false drops the forcing terms
+// of a force-free launch (IEEE rules keep u * 0 and x + 0 in the code
+// otherwise); the result is the same except for the sign of zero sums.
 #pragma once
 
 #include "clbm_common.h"
 
 namespace clbm {
 
-template <typename R>
+template <typename R, bool FORCE = true>
 __device__ __forceinline__ void collide_cell(R* f, R Fx, R Fy, R Fz, R s_v, R s_e) {
   const R half = R(0.5), third = R(1) / R(3), quarter = R(0.25);
   // ---- raw moments of the post-stream populations ----
@@ -40,9 +42,9 @@ __device__ __forceinline__ void collide_cell(R* f, R Fx, R Fy, R Fz, R s_v, R s_
 
   // ---- half-force shifted velocity (clbm_moving.py:113-115) ----
   const R inv = R(1) / rho;
-  const R u = (mx + half * Fx) * inv;
-  const R v = (my + half * Fy) * inv;
-  const R w = (mz + half * Fz) * inv;
+  const R u = (FORCE ? mx + half * Fx : mx) * inv;
+  const R v = (FORCE ? my + half * Fy : my) * inv;
+  const R w = (FORCE ? mz + half * Fz : mz) * inv;
   const R ru = rho * u, rv = rho * v, rw = rho * w;
 
   // ---- second-order central moments about u (first central moments are -F/2) ----
@@ -124,6 +126,7 @@ __device__ __forceinline__ void collide_cell(R* f, R Fx, R Fy, R Fz, R s_v, R s_
   }
 
   // ---- Guo forcing ----
+  if (!FORCE) return;
   const R c = R(1) - half * s_v;
   const R uF = u * Fx + v * Fy + w * Fz;
 #pragma unroll

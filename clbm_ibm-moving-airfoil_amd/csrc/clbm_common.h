@@ -75,15 +75,18 @@ template <> struct Store<CLBM_STORE_BF16> {
   __device__ static void st(T* p, R v, float b) { *p = __float2bfloat16(v - b); }
 };
 
-// 16-bit code for deviations |x| < 2: a sign bit and a 15-bit code u for
+// 16-bit code for deviations |x| < 1: a sign bit and a 15-bit code u for
 // the magnitude.  u is the 12-significant-bit float F = |x| + 2^-15 with
-// F's exponent stored in 4 bits (F in [2^-15, 2)) and 11 mantissa bits:
-// encode rounds fl32(|x| + 2^-15) to 12 significant bits (nearest, ties to
-// even), decode returns F(u) - 2^-15 (exact in float32).  Steps are 2^-26
-// below 2^-15 and at most 2^-11 of the value above it (|x| >= 2^-13: 2^-12,
-// twice IEEE half's precision).  0x7fff (either sign) is NaN, which every
-// value rounding to F >= 2 - 2^-11, infinity or NaN becomes.  Both
-// directions are branch-free.
+// F's exponent stored in 4 bits (F in [2^-15, 1) for exponent codes 0-14)
+// and 11 mantissa bits: encode rounds fl32(|x| + 2^-15) to 12 significant
+// bits (nearest, ties to even), decode returns F(u) - 2^-15 (exact in
+// float32).  Steps are 2^-26 below 2^-15 and at most 2^-11 of the value
+// above it (|x| >= 2^-13: 2^-12, twice IEEE half's precision).  Exponent
+// code 15 is not finite: 0x7800 is infinity, above it NaN; values rounding to
+// F >= 1, infinity and NaN encode there (0x7fff from F >= 2).  Decode ORs the
+// code, shifted by 12, under the bias 0x78000000, whose bits also cover the
+// sign bit, so exponent code 15 lands on float32 exponent 0xff by itself; an
+// ldexp by -128 undoes the bias.  Both directions are branch-free.
 __device__ __forceinline__ uint16_t f16c_encode(float v) {
   const uint32_t b = __float_as_uint(v);
   const uint32_t ab = b & 0x7fffffffu;
@@ -94,10 +97,9 @@ __device__ __forceinline__ uint16_t f16c_encode(float v) {
 }
 __device__ __forceinline__ float f16c_decode(uint16_t hv) {
   const uint32_t h = hv;
-  const uint32_t u = h & 0x7fffu;
-  float a = __uint_as_float((u << 12) + (112u << 23)) - 3.0517578125e-05f;
-  a = u == 0x7fffu ? __uint_as_float(0x7fc00000u) : a;
-  return __uint_as_float(__float_as_uint(a) | ((h & 0x8000u) << 16));
+  const float big = __uint_as_float((h << 12) | 0x78000000u);   // F * 2^128
+  const float a = __builtin_amdgcn_ldexpf(big, -128) - 3.0517578125e-05f;
+  return __uint_as_float(__float_as_uint(a) | ((h << 16) & 0x80000000u));
 }
 template <> struct Store<CLBM_STORE_F16C> {
   using T = uint16_t;

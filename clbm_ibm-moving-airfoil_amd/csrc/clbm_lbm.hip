@@ -131,13 +131,13 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, int t) {
     if (!twin) pop_store_in<PAR>(pop, c + 1, make_nbr(g, x + 1, y, z), fq);
   }
   if (pl.on && x == 0) inlet_rule<R>(fq, R(pl.ux_in), R(pl.uy_in));
-  collide_cell<R>(fq, Fx, Fy, Fz, R(a.s_v), R(a.s_e));
+  collide_cell<R, FORCE>(fq, Fx, Fy, Fz, R(a.s_v), R(a.s_e));
   pop_store<PAR>(pop, c, nb, fq);
   if (outlet) {
     const Nbr nb1 = make_nbr(g, x + 1, y, z);
     if (!twin) {
       pop_load<PAR>(pop, c + 1, nb1, fq);
-      collide_cell<R>(fq, Gx, Gy, Gz, R(a.s_v), R(a.s_e));
+      collide_cell<R, FORCE>(fq, Gx, Gy, Gz, R(a.s_v), R(a.s_e));
     }
     pop_store<PAR>(pop, c + 1, nb1, fq);
   }

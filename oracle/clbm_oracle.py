@@ -313,8 +313,9 @@ def lbm_step(f, fx, fy, fz, s_v: float, s_e: float, ux_in: float, uy_in: float, 
 def f16c_encode(v):
     """float32 -> uint16: sign bit and 15-bit code u of F = fl32(|v| + 2^-15)
     rounded to 12 significant bits (nearest, ties to even), F's exponent in 4
-    bits (F in [2^-15, 2)) and 11 mantissa bits; 0x7fff is NaN, which values
-    rounding to F >= 2 - 2^-11, inf and NaN become."""
+    bits (codes 0-14: F in [2^-15, 1)) and 11 mantissa bits; exponent code 15
+    (u >= 0x7800) holds the non-finite codes, which values rounding to F >= 1
+    become (0x7fff for F >= 2, inf and NaN)."""
     v = np.asarray(v, dtype=np.float32)
     b = v.view(np.uint32).astype(np.uint64)
     sign = (b >> 16) & 0x8000
@@ -327,12 +328,14 @@ def f16c_encode(v):
 
 
 def f16c_decode(h):
-    """uint16 -> float32 (F(u) - 2^-15 with the sign; 0x7fff is NaN)."""
+    """uint16 -> float32: F(u) - 2^-15 with the sign for exponent codes 0-14;
+    0x7800 is infinity and the codes above it NaN (with the sign)."""
     h = np.asarray(h, dtype=np.uint16).astype(np.uint64)
     u = h & 0x7FFF
     F = (((u << 12) + (112 << 23)) & 0xFFFFFFFF).astype(np.uint32).view(np.float32)
     a = (F - np.float32(2.0 ** -15)).astype(np.float32)
-    a = np.where(u == 0x7FFF, np.float32(np.nan), a).astype(np.float32)
+    a = np.where(u >= 0x7800, np.float32(np.inf), a).astype(np.float32)
+    a = np.where(u > 0x7800, np.float32(np.nan), a).astype(np.float32)
     return np.where(h & 0x8000, -a, a).astype(np.float32)
 
 

@@ -2,10 +2,13 @@
 
 The code of |v| is F = fl32(|v| + 2^-15) rounded to 12 significant bits,
 stored as a 4-bit exponent and 11 mantissa bits; decoding gives F - 2^-15.
-Pinned by known codes, a round trip over all 32,767 finite magnitudes and
-their negatives, strict monotonicity, brute-force nearest rounding of F
-(ties to even) against the table of all codes, and the resulting error
-bounds.  A planted truncating encoder must fail the rounding check.
+Exponent code 15 is not finite (0x7800 infinity, above it NaN), so the
+finite range is |v| < 1.  Pinned by known codes, a round trip over all
+30,720 finite magnitudes and their negatives, strict monotonicity,
+brute-force nearest rounding of F (ties to even) against the table of all
+finite codes, the resulting error bounds, and non-finite codes for every
+magnitude that rounds to F >= 1.  A planted truncating encoder must fail the
+rounding check.
 """
 
 import numpy as np
@@ -19,14 +22,30 @@ def test_known_codes(oracle):
     codes = [0x0000, 0x8000, 0x0001, 0x0800, 0x7000, 0xF000, 0x7800, 0x0002,
              0x7FFF, 0x7FFF, 0x7FFF, 0x7FFF]     # 1.5 * 2^-26 ties to the even code 2
     assert list(oracle.f16c_encode(vals)) == codes
-    dec = oracle.f16c_decode(np.array([0x0800, 0x7000, 0x0001, 0x8001], np.uint16))
-    assert list(dec) == [2.0 ** -15, np.float32(0.5 - 2.0 ** -15), 2.0 ** -26, -(2.0 ** -26)]
-    assert np.isnan(oracle.f16c_decode(np.array([0x7FFF, 0xFFFF], np.uint16))).all()
+    dec = oracle.f16c_decode(np.array([0x0800, 0x7000, 0x0001, 0x8001, 0x77FF], np.uint16))
+    assert list(dec) == [2.0 ** -15, np.float32(0.5 - 2.0 ** -15), 2.0 ** -26, -(2.0 ** -26),
+                         np.float32((1 - 2.0 ** -12) - 2.0 ** -15)]
+    assert list(oracle.f16c_decode(np.array([0x7800, 0xF800], np.uint16))) == [np.inf, -np.inf]
+    assert np.isnan(oracle.f16c_decode(np.array([0x7801, 0x7FFF, 0xFFFF], np.uint16))).all()
 
 
 def _table(oracle):
-    codes = np.arange(0x7FFF, dtype=np.uint16)
+    codes = np.arange(0x7800, dtype=np.uint16)
     return codes, oracle.f16c_decode(codes)
+
+
+def test_magnitudes_from_one_up_are_not_finite(oracle):
+    top = oracle.f16c_decode(np.array([0x77FF], np.uint16))[0].astype(np.float64)
+    F_top = top + 2.0 ** -15
+    # the first magnitude whose F rounds up to 1: halfway between F_top and 1
+    rng = np.random.default_rng(2)
+    x = np.concatenate([rng.uniform(1.0, 1.9, 1000), [1.0, 0.99999, (F_top + 1.0) / 2],
+                        rng.uniform(2.0, 1e6, 1000)]).astype(np.float32)
+    x = x[(np.abs(x).astype(np.float32) + np.float32(2.0 ** -15)).astype(np.float64) >
+          (F_top + 1.0) / 2]
+    codes = oracle.f16c_encode(np.concatenate([x, -x]))
+    assert np.all((codes & 0x7FFF) >= 0x7800)
+    assert not np.isfinite(oracle.f16c_decode(codes)).any()
 
 
 def test_round_trip_and_monotone(oracle):
@@ -52,7 +71,7 @@ def _check_nearest(encode, oracle, x):
 
 def test_rounding_is_nearest_ties_even(oracle):
     rng = np.random.default_rng(0)
-    x = np.concatenate([rng.uniform(-1.9, 1.9, 200000),
+    x = np.concatenate([rng.uniform(-0.9997, 0.9997, 200000),
                         rng.uniform(-2e-4, 2e-4, 50000)]).astype(np.float32)
     _, vals = _table(oracle)
     F = vals[:4000].astype(np.float64) + 2.0 ** -15
