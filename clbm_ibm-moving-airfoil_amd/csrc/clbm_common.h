@@ -84,9 +84,9 @@ template <> struct Store<CLBM_STORE_BF16> {
 // above it (|x| >= 2^-13: 2^-12, twice IEEE half's precision).  Exponent
 // code 15 is not finite: 0x7800 is infinity, above it NaN; values rounding to
 // F >= 1, infinity and NaN encode there (0x7fff from F >= 2).  Decode ORs the
-// code, shifted by 12, under the bias 0x78000000, whose bits also cover the
-// sign bit, so exponent code 15 lands on float32 exponent 0xff by itself; an
-// ldexp by -128 undoes the bias.  Both directions are branch-free.
+// code, shifted by 12, under the bias 0x78000000, so exponent code 15 lands on
+// float32 exponent 0xff by itself; an ldexp by -128 undoes the bias.  Both
+// directions are branch-free.
 __device__ __forceinline__ uint16_t f16c_encode(float v) {
   const uint32_t b = __float_as_uint(v);
   const uint32_t ab = b & 0x7fffffffu;
@@ -95,11 +95,14 @@ __device__ __forceinline__ uint16_t f16c_encode(float v) {
   const uint32_t h = min((r - (112u << 23)) >> 12, 0x7fffu);  // too large, inf, NaN -> 0x7fff
   return (uint16_t)(((b >> 16) & 0x8000u) | h);
 }
+// The code is read sign-extended: shifted by 12, its sign copies fill bits
+// 27-31, of which the bias covers 27-30, so the OR gives +-F * 2^128 with the
+// code's sign; the offset 2^-15 is then removed with that sign (code 0x8000
+// decodes to +0).
 __device__ __forceinline__ float f16c_decode(uint16_t hv) {
-  const uint32_t h = hv;
-  const float big = __uint_as_float((h << 12) | 0x78000000u);   // F * 2^128
-  const float a = __builtin_amdgcn_ldexpf(big, -128) - 3.0517578125e-05f;
-  return __uint_as_float(__float_as_uint(a) | ((h << 16) & 0x80000000u));
+  const uint32_t hs = (uint32_t)(int32_t)(int16_t)hv;
+  const float fs = __builtin_amdgcn_ldexpf(__uint_as_float((hs << 12) | 0x78000000u), -128);
+  return fs - copysignf(3.0517578125e-05f, fs);
 }
 template <> struct Store<CLBM_STORE_F16C> {
   using T = uint16_t;

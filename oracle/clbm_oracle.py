@@ -328,15 +328,17 @@ def f16c_encode(v):
 
 
 def f16c_decode(h):
-    """uint16 -> float32: F(u) - 2^-15 with the sign for exponent codes 0-14;
-    0x7800 is infinity and the codes above it NaN (with the sign)."""
+    """uint16 -> float32: +-F(u) minus 2^-15 with the same sign, for exponent
+    codes 0-14 (so 0x8000 gives +0); 0x7800 is infinity and the codes above
+    it NaN (with the sign)."""
     h = np.asarray(h, dtype=np.uint16).astype(np.uint64)
     u = h & 0x7FFF
     F = (((u << 12) + (112 << 23)) & 0xFFFFFFFF).astype(np.uint32).view(np.float32)
-    a = (F - np.float32(2.0 ** -15)).astype(np.float32)
-    a = np.where(u >= 0x7800, np.float32(np.inf), a).astype(np.float32)
-    a = np.where(u > 0x7800, np.float32(np.nan), a).astype(np.float32)
-    return np.where(h & 0x8000, -a, a).astype(np.float32)
+    F = np.where(u >= 0x7800, np.float32(np.inf), F).astype(np.float32)
+    F = np.where(u > 0x7800, np.float32(np.nan), F).astype(np.float32)
+    Fs = np.where(h & 0x8000, -F, F).astype(np.float32)
+    with np.errstate(invalid="ignore"):
+        return (Fs - np.copysign(np.float32(2.0 ** -15), Fs)).astype(np.float32)
 
 
 # --------------------------------------------------------------------------

@@ -26,6 +26,8 @@ def test_known_codes(oracle):
     assert list(dec) == [2.0 ** -15, np.float32(0.5 - 2.0 ** -15), 2.0 ** -26, -(2.0 ** -26),
                          np.float32((1 - 2.0 ** -12) - 2.0 ** -15)]
     assert list(oracle.f16c_decode(np.array([0x7800, 0xF800], np.uint16))) == [np.inf, -np.inf]
+    zero = oracle.f16c_decode(np.array([0x0000, 0x8000], np.uint16))
+    assert list(zero) == [0.0, 0.0] and not np.signbit(zero).any()
     assert np.isnan(oracle.f16c_decode(np.array([0x7801, 0x7FFF, 0xFFFF], np.uint16))).all()
 
 
