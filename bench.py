@@ -6,7 +6,7 @@ clamp into fields) and ``CLBMSolver.step`` (fused collide + stream, inlet and
 outlet planes).  Populations, markers and forces are resident in HBM before
 the timed region starts.  Prints one JSON line.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--storage f32|f16|bf16|f64]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--storage f32|f16|bf16|f16c|f64]
 """
 
 from __future__ import annotations
@@ -35,6 +35,8 @@ def parse(argv=None):
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--storage", default="f32", choices=sorted(ELEM))
+    ap.add_argument("--deviation", default="inflow", choices=["weights", "inflow"],
+                    help="deviation base of a 16-bit --storage")
     ap.add_argument("--case", default="c1", choices=["c0", "c1", "c2"])
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-MiB device buffer 10 times (calibrates the "
@@ -175,7 +177,7 @@ def main(argv=None) -> int:
         tune["lbm"]["outside_lds"] = a.outside_lds
     if a.group is not None:
         tune["ibm"]["group"] = a.group
-    r = measure(pkg, cfg, a.storage, a.steps, a.warmup, dev, tune)
+    r = measure(pkg, cfg, a.storage, a.steps, a.warmup, dev, tune, deviation=a.deviation)
     if a.profile_only:
         print(json.dumps({"steps": a.steps, "seconds": r["dt"], "finite": r["finite"]}))
         return 0

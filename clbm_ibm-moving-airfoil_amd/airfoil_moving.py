@@ -84,13 +84,18 @@ class DroneMovingIBM:
     per marker (for index checks); ``order`` "natural" (the template's own
     order, neighbours along each section) or "morton" (a Z-order sort of the
     template) for the marker groups of the fused kernel; ``group`` markers
-    per workgroup (4, 8 or 16); ``overlap`` False to keep the marker kernels
-    on the caller's stream.  None of these tuning options change results in
-    the fixed spread mode."""
+    per workgroup (4, 8 or 16); ``overlap`` True to run the marker kernels on
+    a stream of their own, concurrently with the solver's next step, False to
+    keep them on the caller's stream, "auto" (the default) for True on
+    lattices of ``OVERLAP_MIN_CELLS`` cells or more, where the step kernels
+    outlast the host work of the split step.  None of these tuning options
+    change results in the fixed spread mode."""
+
+    OVERLAP_MIN_CELLS = 4_000_000
 
     def __init__(self, config=None, device=None, spread_mode: str = "fixed",
                  f_max=None, wrap_weight_quirk=None, record_stencil: bool = False,
-                 order: str = "natural", group: int = 8, overlap: bool = True):
+                 order: str = "natural", group: int = 8, overlap="auto"):
         import torch
         cfg: CaseConfig = resolve(config)
         self.cfg = cfg
@@ -138,7 +143,11 @@ class DroneMovingIBM:
         self._stencil_base = (torch.zeros((self.N, 3), dtype=torch.int32, device=dev)
                               if record_stencil else None)
         # region prediction: hull of the template in the body frame, z range of its stencils
-        self.overlap = bool(overlap)
+        if overlap == "auto":
+            overlap = self.Nx * self.Ny * self.Nz >= self.OVERLAP_MIN_CELLS
+        elif not isinstance(overlap, bool):
+            raise ValueError("overlap must be True, False or 'auto'")
+        self.overlap = overlap
         hull = convex_hull_2d(cfg.X0, cfg.Y0) if self.N else np.zeros(0, np.int64)
         self._hx = np.asarray(cfg.X0, dtype=np.float32)[hull]
         self._hy = np.asarray(cfg.Y0, dtype=np.float32)[hull]

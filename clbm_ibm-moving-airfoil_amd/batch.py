@@ -3,11 +3,16 @@
 No reference counterpart: the reference runs one case per process
 (main_moving.py).  A sweep holds n instances, each a CLBMSolver and a
 DroneMovingIBM of its own grid, rates and kinematics.  All of them advance
-in lockstep, four launches per step for the whole batch:
+in lockstep, a few launches per step for a whole group:
 
 * ``ibm_batch_compute``: the marker kernel and the clamp of every body;
 * ``clbm_batch_step``: the fused step of every lattice;
 * ``ibm_batch_advance``: bump the device step counter.
+
+The instances are grouped so that a group's populations fit in the GPU's
+last-level cache, and a run advances one group for all its steps before the
+next, so a group is read from that cache rather than HBM from step to step
+(``cache_bytes`` 0 keeps one group of everything).
 
 Kinematics come from per-instance device tables indexed by that counter.
 The step sequence therefore needs no host argument and is captured once as a
@@ -75,11 +80,49 @@ def kinematics_table(cfg, first_step: int, steps: int) -> np.ndarray:
     return t
 
 
+def cache_groups(cfgs, storage: str, cache_bytes: int):
+    """Partition instance indices into groups whose populations together fit
+    in ``cache_bytes`` (first fit, largest first); an instance larger than the
+    budget forms a group of its own, and ``cache_bytes`` 0 keeps one group."""
+    idx = list(range(len(cfgs)))
+    if cache_bytes <= 0:
+        return [idx]
+    e = _lib.ELEM_BYTES[_lib.STORAGE[storage]]
+    size = [19 * e * c.Nx * c.Ny * c.Nz for c in cfgs]
+    groups, load = [], []
+    for k in sorted(idx, key=lambda k: -size[k]):
+        for g, used in enumerate(load):
+            if used + size[k] <= cache_bytes:
+                groups[g].append(k)
+                load[g] += size[k]
+                break
+        else:
+            groups.append([k])
+            load.append(size[k])
+    return [sorted(g) for g in groups]
+
+
+class _Group:
+    """Device tables and captured graph of the instances of one cache group."""
+
+    def __init__(self, idx):
+        self.idx = idx
+        self.lb = self.ib = self.lwork = self.iwork = None
+        self.graph = None
+        self.graph_state = None
+
+
 class BatchSweep:
-    """n instances on one GPU, stepped together (see the module docstring)."""
+    """n instances on one GPU, stepped together (see the module docstring).
+
+    The instances are split into groups that fit in ``cache_bytes`` (by
+    default 192 MiB of the MI355X's 256 MB last-level cache) and ``run``
+    advances one group at a time for all its steps, so a group's populations
+    stay in that cache from step to step."""
 
     def __init__(self, cfgs, storage: str = "f32", device=None, steps: int = 1000,
-                 block: int = 256, group: int = 8, first_step: int = 1):
+                 block: int = 256, group: int = 8, first_step: int = 1,
+                 cache_bytes: int = 192 << 20):
         import torch
         self.cfgs = list(cfgs)
         self.device = torch.device(device if device is not None else "cuda")
@@ -92,31 +135,44 @@ class BatchSweep:
                      for c in self.cfgs]
         self.steps_done = 0
         self.state = 0
-        self._graph = None
+        self.groups = [_Group(g) for g in cache_groups(self.cfgs, storage, cache_bytes)]
         self.build(block, group)
 
     # -- tables ---------------------------------------------------------------
     def build(self, block: int, group: int) -> None:
         import torch
-        n = len(self.cfgs)
         self.block, self.group = int(block), int(group)
-        grids = (_lib.ClbmGrid * n)(*[l.grid for l in self.lbms])
-        params = (_lib.ClbmParams * n)(*[l.params for l in self.lbms])
-        f = (C.c_void_p * n)(*[l.state.data_ptr() for l in self.lbms])
-        fx = (C.c_void_p * n)(*[i.fx._t.data_ptr() for i in self.ibms])
-        fy = (C.c_void_p * n)(*[i.fy._t.data_ptr() for i in self.ibms])
-        boxes = (C.c_void_p * n)(*[i.box.data_ptr() for i in self.ibms])
+        first = self.ibms[0].ip
+        for ibm in self.ibms[1:]:
+            if (ibm.ip.f_max, ibm.ip.wrap_weight_quirk, ibm.ip.spread_mode) != \
+                    (first.f_max, first.wrap_weight_quirk, first.spread_mode):
+                raise ValueError("instances of a batch share the spread mode, clamp and quirk")
+        torch.cuda.synchronize(self.device)
+        for gr in self.groups:
+            self._build_group(gr)
+
+    def _build_group(self, gr: "_Group") -> None:
+        import torch
+        lbms = [self.lbms[k] for k in gr.idx]
+        ibms = [self.ibms[k] for k in gr.idx]
+        n = len(gr.idx)
+        grids = (_lib.ClbmGrid * n)(*[l.grid for l in lbms])
+        params = (_lib.ClbmParams * n)(*[l.params for l in lbms])
+        f = (C.c_void_p * n)(*[l.state.data_ptr() for l in lbms])
+        fx = (C.c_void_p * n)(*[i.fx._t.data_ptr() for i in ibms])
+        fy = (C.c_void_p * n)(*[i.fy._t.data_ptr() for i in ibms])
+        boxes = (C.c_void_p * n)(*[i.box.data_ptr() for i in ibms])
         lib = _lib.load()
         nb = lib.clbm_batch_bytes(n, grids, self.block)
         if nb == 0:
             raise _lib.ClbmError(lib.clbm_last_error().decode())
-        self._lwork = torch.empty(nb, dtype=torch.uint8, device=self.device)
-        self.lb = _lib.ClbmBatch()
+        gr.lwork = torch.empty(nb, dtype=torch.uint8, device=self.device)
+        gr.lb = _lib.ClbmBatch()
         torch.cuda.synchronize(self.device)
         _lib.call("clbm_batch_build", n, grids, params, f, fx, fy, boxes, self.block,
-                  C.c_void_p(self._lwork.data_ptr()), nb, C.byref(self.lb))
+                  C.c_void_p(gr.lwork.data_ptr()), nb, C.byref(gr.lb))
         bufs = (_lib.IbmBuffers * n)()
-        for k, ibm in enumerate(self.ibms):
+        for k, ibm in enumerate(ibms):
             b = bufs[k]
             b.n = ibm.N
             b.order = ibm.order.data_ptr() if ibm.order is not None else None
@@ -128,32 +184,31 @@ class BatchSweep:
             b.acc_x, b.acc_y = ibm._acc[0].data_ptr(), ibm._acc[1].data_ptr()
             b.box = ibm.box.data_ptr()
             b.fx, b.fy = ibm.fx._t.data_ptr(), ibm.fy._t.data_ptr()
-            b.kin = self._kin[k].data_ptr()
+            b.kin = self._kin[gr.idx[k]].data_ptr()
             b.gen0 = ibm.gen - self.steps_done    # the generation of counter value c is gen0 + 1 + c
-        ip = _lib.IbmParams(float(self.ibms[0].ip.f_max), int(self.ibms[0].ip.wrap_weight_quirk),
-                            int(self.ibms[0].ip.spread_mode), 0, self.group, 0)
-        for ibm in self.ibms[1:]:
-            if (ibm.ip.f_max, ibm.ip.wrap_weight_quirk, ibm.ip.spread_mode) != \
-                    (ip.f_max, ip.wrap_weight_quirk, ip.spread_mode):
-                raise ValueError("instances of a batch share the spread mode, clamp and quirk")
+        first = self.ibms[0].ip
+        ip = _lib.IbmParams(float(first.f_max), int(first.wrap_weight_quirk),
+                            int(first.spread_mode), 0, self.group, 0)
         ib = lib.ibm_batch_bytes(n, bufs, self.group)
         if ib == 0:
             raise _lib.ClbmError(lib.clbm_last_error().decode())
-        self._iwork = torch.empty(ib, dtype=torch.uint8, device=self.device)
-        self.ib = _lib.IbmBatch()
+        gr.iwork = torch.empty(ib, dtype=torch.uint8, device=self.device)
+        gr.ib = _lib.IbmBatch()
         _lib.call("ibm_batch_build", n, grids, params, f, C.byref(ip), bufs,
-                  C.c_void_p(self._iwork.data_ptr()), ib, C.byref(self.ib))
+                  C.c_void_p(gr.iwork.data_ptr()), ib, C.byref(gr.ib))
         # the counter starts at 0: move it to the steps already taken
-        self._iwork.view(torch.int32)[(self.ib.counter - self._iwork.data_ptr()) // 4] = \
-            self.steps_done
-        self._graph = None
+        gr.iwork.view(torch.int32)[(gr.ib.counter - gr.iwork.data_ptr()) // 4] = self.steps_done
+        gr.graph = None
 
     # -- stepping -------------------------------------------------------------
-    def _launch_step(self, state: int, stream) -> int:
-        _lib.call("ibm_batch_compute", C.byref(self.ib), state, stream)
-        _lib.call("clbm_batch_step", C.byref(self.lb), state, stream)
-        _lib.call("ibm_batch_advance", C.byref(self.ib), stream)
+    def _next_state(self, state: int) -> int:
         return _lib.load().clbm_state_after_step(state, C.byref(self.lbms[0].params))
+
+    def _launch_step(self, gr: "_Group", state: int, stream) -> int:
+        _lib.call("ibm_batch_compute", C.byref(gr.ib), state, stream)
+        _lib.call("clbm_batch_step", C.byref(gr.lb), state, stream)
+        _lib.call("ibm_batch_advance", C.byref(gr.ib), stream)
+        return self._next_state(state)
 
     def _advance_objects(self, k: int) -> None:
         """Keep the solver/IBM objects consistent with k batched steps."""
@@ -166,19 +221,24 @@ class BatchSweep:
             i.gen += k
 
     def step(self, k: int = 1) -> None:
-        """k eager steps of the whole batch."""
+        """k eager steps of the whole batch, group by group."""
         import torch
         if self.steps_done + k > self.capacity:
             raise ValueError("kinematics table exhausted; build the sweep with more steps")
         st = C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
-        for _ in range(k):
-            self.state = self._launch_step(self.state, st)
-            self._advance_objects(1)
+        end = self.state
+        for gr in self.groups:
+            s = self.state
+            for _ in range(k):
+                s = self._launch_step(gr, s, st)
+            end = s
+        self.state = end
+        self._advance_objects(k)
 
     def run(self, k: int, graph: bool = True) -> None:
-        """k steps; with ``graph``, steps after the first go through a captured
-        two-step graph (the inlet/outlet planes are pending from step 2 on, so
-        the state word cycles with period two)."""
+        """k steps, group by group; with ``graph``, a group's steps after the
+        first go through its captured two-step graph (the inlet/outlet planes
+        are pending from step 2 on, so the state word cycles with period two)."""
         import torch
         if not graph:
             self.step(k)
@@ -188,23 +248,27 @@ class BatchSweep:
         if self.steps_done == 0 and k > 0:
             self.step(1)
             k -= 1
-        if k >= 2 and self._graph is not None and self._graph_state != self.state:
-            self.step(1)      # the state word alternates; realign with the captured pair
-            k -= 1
-        if k >= 2 and self._graph is None:
-            self._capture()
-        while k >= 2:
-            self._graph.replay()
-            s = self.state
-            for _ in range(2):
-                s = _lib.load().clbm_state_after_step(s, C.byref(self.lbms[0].params))
-            self.state = s
-            self._advance_objects(2)
-            k -= 2
-        if k:
-            self.step(k)
+        st = C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+        end = self.state
+        for gr in self.groups:
+            s, left = self.state, k
+            if left >= 2 and gr.graph is not None and gr.graph_state != s:
+                s = self._launch_step(gr, s, st)   # realign with the captured pair
+                left -= 1
+            if left >= 2 and gr.graph is None:
+                self._capture(gr, s)
+            while left >= 2:
+                gr.graph.replay()
+                s = self._next_state(self._next_state(s))
+                left -= 2
+            while left:
+                s = self._launch_step(gr, s, st)
+                left -= 1
+            end = s
+        self.state = end
+        self._advance_objects(k)
 
-    def _capture(self) -> None:
+    def _capture(self, gr: "_Group", state: int) -> None:
         import torch
         g = torch.cuda.CUDAGraph()
         side = torch.cuda.Stream(device=self.device)
@@ -212,11 +276,11 @@ class BatchSweep:
         with torch.cuda.stream(side):
             with torch.cuda.graph(g, stream=side):
                 st = C.c_void_p(side.cuda_stream)
-                s = self._launch_step(self.state, st)
-                self._launch_step(s, st)
+                s = self._launch_step(gr, state, st)
+                self._launch_step(gr, s, st)
         torch.cuda.current_stream(self.device).wait_stream(side)
-        self._graph = g
-        self._graph_state = self.state
+        gr.graph = g
+        gr.graph_state = state
 
     # -- tuning and reference timings -------------------------------------------
     def signature(self) -> str:
@@ -281,14 +345,15 @@ class BatchSweep:
 
 
 def sequential(cfgs, steps: int, storage: str = "f32", device=None, first_step: int = 1):
-    """Each instance on its own (its CLBMSolver and DroneMovingIBM with the
-    usual per-instance launches), one after the other; returns the objects and
-    the elapsed seconds."""
+    """Each instance on its own (its CLBMSolver and DroneMovingIBM with their
+    default per-instance launches: the overlapped split step on the large
+    lattices), one after the other; returns the objects and the elapsed
+    seconds."""
     import torch
     objs, total = [], 0.0
     for c in cfgs:
         lbm = CLBMSolver(c, storage=storage, device=device)
-        ibm = DroneMovingIBM(c, device=device, overlap=False)
+        ibm = DroneMovingIBM(c, device=device)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for n in range(first_step, first_step + steps):
@@ -305,7 +370,9 @@ def main(argv=None) -> int:
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16", "f64"])
+    ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16", "f16c", "f64"])
+    ap.add_argument("--cache-mib", type=int, default=192,
+                    help="population bytes per cache group in MiB (0 = one group)")
     ap.add_argument("--sequential-steps", type=int, default=200,
                     help="steps of the per-instance sequential timing (0 = skip)")
     ap.add_argument("--no-tune", action="store_true")
@@ -313,7 +380,7 @@ def main(argv=None) -> int:
     a = ap.parse_args(argv)
     import torch
     cfgs = draw_instances(a.instances, a.seed)
-    sweep = BatchSweep(cfgs, storage=a.storage, steps=a.steps + 64)
+    sweep = BatchSweep(cfgs, storage=a.storage, steps=a.steps + 64, cache_bytes=a.cache_mib << 20)
     tune = None if a.no_tune else sweep.autotune()
     print(f"[batch] {len(cfgs)} instances, {sweep.cells / 1e6:.1f} M cells, block {sweep.block}, "
           f"group {sweep.group}", file=sys.stderr, flush=True)
@@ -328,7 +395,8 @@ def main(argv=None) -> int:
         print(f"[batch] {done} steps", file=sys.stderr, flush=True)
     dt = time.perf_counter() - t0
     res = {"instances": len(cfgs), "cells": sweep.cells, "steps": a.steps, "storage": a.storage,
-           "block": sweep.block, "group": sweep.group, "tuning_ms_per_step": tune,
+           "block": sweep.block, "group": sweep.group, "cache_groups": len(sweep.groups),
+           "cache_mib": a.cache_mib, "tuning_ms_per_step": tune,
            "seconds": dt, "mlups": sweep.cells * a.steps / dt / 1e6,
            "finite": all(l.health()[1] for l in sweep.lbms)}
     if a.sequential_steps:

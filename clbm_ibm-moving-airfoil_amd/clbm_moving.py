@@ -15,7 +15,7 @@ Differences of mechanism (not of results):
   when read, instead of after every step; the immersed-boundary object reads
   the force-free velocity straight from the populations;
 * when the forces come from an immersed-boundary object that computed them on
-  its own stream (``DroneMovingIBM(overlap=True)``), the step runs in two
+  its own stream (``DroneMovingIBM(overlap=...)``), the step runs in two
   parts: the cells outside the body's region first, concurrently with the
   marker kernels (they touch no population the markers read), then the
   region with the force once the marker kernels are done.

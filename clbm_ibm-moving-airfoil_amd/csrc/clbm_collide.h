@@ -15,7 +15,8 @@
 // Results agree with the reference formula to rounding; the order of the
 // floating-point operations differs.  FORCE = false drops the forcing terms
 // of a force-free launch (IEEE rules keep u * 0 and x + 0 in the code
-// otherwise); the result is the same except for the sign of zero sums.
+// otherwise), and a cell whose force is zero skips Guo's term at run time;
+// either way the result is the same except for the sign of zero sums.
 #pragma once
 
 #include "clbm_common.h"
@@ -126,7 +127,7 @@ __device__ __forceinline__ void collide_cell(R* f, R Fx, R Fy, R Fz, R s_v, R s_
   }
 
   // ---- Guo forcing ----
-  if (!FORCE) return;
+  if (!FORCE || (Fx == R(0) && Fy == R(0) && Fz == R(0))) return;
   const R c = R(1) - half * s_v;
   const R uF = u * Fx + v * Fy + w * Fz;
 #pragma unroll

@@ -2,10 +2,11 @@
 
 CPU: the seeded draws and the device kinematics table (what
 ibm_compute_force receives per step: float32 arguments, cos/sin of the
-float32 angle rounded once from float64).  GPU: every instance of a batch
-equals the same instance run on its own, bit for bit (fixed-point
-spreading); graph replays equal eager steps; tuning restores the initial
-state; malformed batches are rejected.
+float32 angle rounded once from float64), and the partition of the
+instances into cache groups.  GPU: every instance of a batch equals the same
+instance run on its own, bit for bit (fixed-point spreading), with one cache
+group or several; graph replays equal eager steps; tuning restores the
+initial state; malformed batches are rejected.
 """
 
 import ctypes as C
@@ -44,6 +45,21 @@ def test_kinematics_table_matches_the_host_arguments(pkg):
         assert t[j, 6] == np.float32(om)
 
 
+def test_cache_groups_partition_the_instances(pkg):
+    bm = load_sub("batch")
+    cfgs = bm.draw_instances(40, seed=1)
+    size = [19 * 4 * c.Nx * c.Ny * c.Nz for c in cfgs]
+    budget = 192 << 20
+    groups = bm.cache_groups(cfgs, "f32", budget)
+    assert sorted(k for g in groups for k in g) == list(range(40))
+    for g in groups:
+        assert len(g) == 1 or sum(size[k] for k in g) <= budget
+    big = [k for k, s in enumerate(size) if s > budget]
+    assert all([k] in groups for k in big)
+    assert bm.cache_groups(cfgs, "f32", 0) == [list(range(40))]
+    assert len(bm.cache_groups(cfgs, "f16", budget)) < len(groups)
+
+
 # ---------------------------------------------------------------------------
 # GPU
 # ---------------------------------------------------------------------------
@@ -73,11 +89,13 @@ def test_batch_equals_each_instance_alone(pkg, gpu, block, group):
 
 
 @pytest.mark.gpu
-def test_graph_replay_equals_eager(pkg, gpu):
+@pytest.mark.parametrize("cache_bytes,ngroups", [(192 << 20, 1), (800_000, 2), (1, 3)])
+def test_graph_replay_equals_eager(pkg, gpu, cache_bytes, ngroups):
     bm = load_sub("batch")
     cfgs = _small(pkg)[:3]
-    a = bm.BatchSweep(cfgs, steps=32)
-    b = bm.BatchSweep(cfgs, steps=32)
+    a = bm.BatchSweep(cfgs, steps=32, cache_bytes=cache_bytes)
+    b = bm.BatchSweep(cfgs, steps=32, cache_bytes=cache_bytes)
+    assert len(a.groups) == ngroups
     a.run(11, graph=True)
     a.run(6, graph=True)       # resumes on the other parity: realigns, then replays
     b.step(17)
