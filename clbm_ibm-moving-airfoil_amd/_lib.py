@@ -23,7 +23,7 @@ BOX_SUBSLOTS = 64
 BOX_WORDS = 2 * 6 * BOX_SUBSLOTS + 4        # uint64 words of a force-box record
 FORCE_BOX_OFFSET = 2 * 6 * BOX_SUBSLOTS     # word offset of the decoded int32[6] box
 REGION_FLAG_OFFSET = 2 * 6 * BOX_SUBSLOTS + 3   # word offset of the region-violation flag
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 
 class ClbmGrid(C.Structure):
@@ -102,7 +102,6 @@ _SIGNATURES = {
     "ibm_batch_build": (_I, [_I, _G, _LP, _P, _IP, C.POINTER(IbmBuffers), _P, C.c_size_t,
                              C.POINTER(IbmBatch)]),
     "ibm_batch_compute": (_I, [C.POINTER(IbmBatch), _I, _P]),
-    "ibm_batch_advance": (_I, [C.POINTER(IbmBatch), _P]),
     "closure_param_count": (_I, []),
     "closure_packed_bytes": (C.c_size_t, []),
     "closure_pack": (_I, [_P, _P, _P]),

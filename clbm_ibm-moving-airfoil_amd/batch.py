@@ -5,9 +5,9 @@ No reference counterpart: the reference runs one case per process
 DroneMovingIBM of its own grid, rates and kinematics.  All of them advance
 in lockstep, a few launches per step for a whole group:
 
-* ``ibm_batch_compute``: the marker kernel and the clamp of every body;
-* ``clbm_batch_step``: the fused step of every lattice;
-* ``ibm_batch_advance``: bump the device step counter.
+* ``ibm_batch_compute``: the marker kernel and the clamp of every body (the
+  clamp's last workgroup bumps the device step counter);
+* ``clbm_batch_step``: the fused step of every lattice.
 
 The instances are grouped so that a group's populations fit in the GPU's
 last-level cache, and a run advances one group for all its steps before the
@@ -205,9 +205,8 @@ class BatchSweep:
         return _lib.load().clbm_state_after_step(state, C.byref(self.lbms[0].params))
 
     def _launch_step(self, gr: "_Group", state: int, stream) -> int:
-        _lib.call("ibm_batch_compute", C.byref(gr.ib), state, stream)
+        _lib.call("ibm_batch_compute", C.byref(gr.ib), state, stream)   # advances the counter
         _lib.call("clbm_batch_step", C.byref(gr.lb), state, stream)
-        _lib.call("ibm_batch_advance", C.byref(gr.ib), stream)
         return self._next_state(state)
 
     def _advance_objects(self, k: int) -> None:

@@ -648,18 +648,24 @@ __global__ __launch_bounds__(IBLK) void k_clip(ClipArgs a, uint32_t gen) {
   clip_body<SMODE>(a, gen, blockIdx.x, gridDim.x);
 }
 
-// bpi blocks per body, body i = blockIdx.x / bpi.
+// bpi blocks per body, body i = blockIdx.x / bpi.  Every block reads the
+// batch's step counter, then counts itself in *done; the last block to
+// finish (all others have read the counter by then) clears *done and
+// advances the counter for the next step.  No block waits for another.
 template <int SMODE>
 __global__ __launch_bounds__(IBLK) void k_clip_batch(const ClipArgs* __restrict__ args, int bpi,
-                                                     const int32_t* __restrict__ step) {
+                                                     int32_t* step, int32_t* done) {
   const int i = blockIdx.x / bpi;
   const ClipArgs a = args[i];
   clip_body<SMODE>(a, a.gen0 + (uint32_t)*step + 1u, blockIdx.x - i * bpi, bpi);
-}
-
-// The step counter of a batch (one thread).
-__global__ void k_counter_add(int32_t* c, int d) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *c += d;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(done, 1) == (int)gridDim.x - 1) {
+      atomicExch(done, 0);
+      atomicAdd(step, 1);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -841,6 +847,10 @@ constexpr int CLIP_BPI = 32;  // clamp blocks per body in a batch
 
 static inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
+// int32 offset of the clamp kernel's done count from the step counter (its
+// own 128-byte line of the counter's 256 bytes)
+constexpr int BATCH_DONE_OFFSET = 32;
+
 struct IbmBatchLayout {
   size_t args, clip, inst, first, counter, total;
   int blocks;
@@ -860,7 +870,7 @@ static int ibm_batch_layout(int n, const ibm_buffers_t* bufs, int group, IbmBatc
   L.clip = align256(sizeof(IbmArgs) * n);
   L.inst = L.clip + align256(sizeof(ClipArgs) * n);
   L.first = L.inst + align256(sizeof(uint16_t) * (size_t)blocks);
-  L.counter = L.first + align256(sizeof(int32_t) * n);
+  L.counter = L.first + align256(sizeof(int32_t) * n);   // step counter, then the done count
   L.total = L.counter + 256;
   return 0;
 }
@@ -883,7 +893,8 @@ static void launch_batch(const ibm_batch_t* b, int par, int planes, hipStream_t 
   else
     launch_compute_batch<S, SMODE, 8>(b, par, planes, st);
   hipLaunchKernelGGL(k_clip_batch<SMODE>, dim3(b->clip_blocks * b->n), dim3(IBLK), 0, st,
-                     (const ClipArgs*)b->clip_args, b->clip_blocks, b->counter);
+                     (const ClipArgs*)b->clip_args, b->clip_blocks, b->counter,
+                     b->counter + BATCH_DONE_OFFSET);
 }
 
 #define CLBM_STORAGE_SWITCH(storage, CALL)                   \
@@ -1138,12 +1149,6 @@ int ibm_batch_compute(const ibm_batch_t* b, int state, void* stream) {
     CLBM_STORAGE_SWITCH(b->storage, (launch_batch<SS, IBM_SPREAD_ATOMIC_F32>(b, par, planes, st)));
   }
   return check_launch("ibm_batch_compute");
-}
-
-int ibm_batch_advance(const ibm_batch_t* b, void* stream) {
-  if (!b || !b->counter) return fail(-1, "ibm_batch_advance: batch is not built");
-  hipLaunchKernelGGL(k_counter_add, dim3(1), dim3(64), 0, (hipStream_t)stream, b->counter, 1);
-  return check_launch("ibm_batch_advance");
 }
 
 int ibm_force_sum_f64(int n, const float* FX, const float* FY, const float* dV, double* out2,

@@ -49,7 +49,7 @@
 extern "C" {
 #endif
 
-#define CLBM_ABI_VERSION 3
+#define CLBM_ABI_VERSION 4
 
 /* population storage formats */
 #define CLBM_STORE_F32 0  /* float32 populations, float32 arithmetic (the drop-in) */
@@ -251,11 +251,12 @@ int ibm_force_sum_f64(int n, const float* FX, const float* FY, const float* dV, 
  * No reference counterpart (the reference runs one case per process).  A
  * batch advances n instances, each a lattice with its own grid, rates and
  * inflow (all in one storage format) and one body, in lockstep: per step one
- * ibm_batch_compute (marker kernel + clamp of every body), one clbm_batch_step
- * (fused step of every lattice) and one ibm_batch_advance, so that a step of
- * the whole batch is a fixed sequence of four launches that a caller can
- * capture in a graph and replay.  Kinematics come from per-body device tables
- * indexed by the batch's device step counter, never from host arguments.
+ * ibm_batch_compute (marker kernel + clamp of every body, which also advances
+ * the batch's device step counter) and one clbm_batch_step (fused step of
+ * every lattice), so that a step of the whole batch is a fixed sequence of
+ * three launches that a caller can capture in a graph and replay.  Kinematics
+ * come from per-body device tables indexed by the step counter, never from
+ * host arguments.
  * Every instance computes what its own single-instance calls would, bit for
  * bit (fixed-point spreading).  Descriptor tables live in caller-owned device
  * work buffers of the sizes the *_bytes functions return; building copies
@@ -299,7 +300,8 @@ typedef struct {
   void* clip_args;    /* device: per-body clamp arguments */
   uint16_t* inst;
   int32_t* first;
-  int32_t* counter;   /* device: steps taken by the batch */
+  int32_t* counter;   /* device: steps taken by the batch (followed by the clamp
+                         kernel's own count in the same 256 bytes) */
 } ibm_batch_t;
 
 /* Device bytes of the body tables (the step counter included). */
@@ -310,10 +312,9 @@ int ibm_batch_build(int n, const clbm_grid_t* grids, const clbm_params_t* params
                     const ibm_params_t* ip, const ibm_buffers_t* bufs, void* work,
                     size_t work_bytes, ibm_batch_t* out);
 /* ibm_compute_force + ibm_clip_forces of every body at the counter's step,
- * velocity from each lattice's populations in state `state`. */
+ * velocity from each lattice's populations in state `state`; then counter
+ * += 1 (the clamp kernel's last workgroup). */
 int ibm_batch_compute(const ibm_batch_t* b, int state, void* stream);
-/* counter += 1 (after the lattices have stepped). */
-int ibm_batch_advance(const ibm_batch_t* b, void* stream);
 
 /* ---- learned closure (no reference counterpart: the reference fixes the
  * third- and fourth-order rates at 1, clbm_moving.py:188-196) -------------
