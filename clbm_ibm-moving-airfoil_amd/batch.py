@@ -122,14 +122,16 @@ class BatchSweep:
 
     def __init__(self, cfgs, storage: str = "f32", device=None, steps: int = 1000,
                  block: int = 256, group: int = 8, first_step: int = 1,
-                 cache_bytes: int = 192 << 20):
+                 cache_bytes: int = 192 << 20, deviation: str = "weights"):
         import torch
         self.cfgs = list(cfgs)
         self.device = torch.device(device if device is not None else "cuda")
         self.storage = storage
+        self.deviation = deviation
         self.first_step = first_step
         self.capacity = int(steps)
-        self.lbms = [CLBMSolver(c, storage=storage, device=self.device) for c in self.cfgs]
+        self.lbms = [CLBMSolver(c, storage=storage, device=self.device, deviation=deviation)
+                     for c in self.cfgs]
         self.ibms = [DroneMovingIBM(c, device=self.device, overlap=False) for c in self.cfgs]
         self._kin = [torch.from_numpy(kinematics_table(c, first_step, self.capacity)).to(self.device)
                      for c in self.cfgs]
@@ -343,7 +345,8 @@ class BatchSweep:
         return sum(c.Nx * c.Ny * c.Nz for c in self.cfgs)
 
 
-def sequential(cfgs, steps: int, storage: str = "f32", device=None, first_step: int = 1):
+def sequential(cfgs, steps: int, storage: str = "f32", device=None, first_step: int = 1,
+               deviation: str = "weights"):
     """Each instance on its own (its CLBMSolver and DroneMovingIBM with their
     default per-instance launches: the overlapped split step on the large
     lattices), one after the other; returns the objects and the elapsed
@@ -351,7 +354,7 @@ def sequential(cfgs, steps: int, storage: str = "f32", device=None, first_step: 
     import torch
     objs, total = [], 0.0
     for c in cfgs:
-        lbm = CLBMSolver(c, storage=storage, device=device)
+        lbm = CLBMSolver(c, storage=storage, device=device, deviation=deviation)
         ibm = DroneMovingIBM(c, device=device)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -370,6 +373,8 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16", "f16c", "f64"])
+    ap.add_argument("--deviation", default="weights", choices=["weights", "inflow"],
+                    help="deviation base of 16-bit storage")
     ap.add_argument("--cache-mib", type=int, default=192,
                     help="population bytes per cache group in MiB (0 = one group)")
     ap.add_argument("--sequential-steps", type=int, default=200,
@@ -379,7 +384,8 @@ def main(argv=None) -> int:
     a = ap.parse_args(argv)
     import torch
     cfgs = draw_instances(a.instances, a.seed)
-    sweep = BatchSweep(cfgs, storage=a.storage, steps=a.steps + 64, cache_bytes=a.cache_mib << 20)
+    sweep = BatchSweep(cfgs, storage=a.storage, steps=a.steps + 64, cache_bytes=a.cache_mib << 20,
+                       deviation=a.deviation)
     tune = None if a.no_tune else sweep.autotune()
     print(f"[batch] {len(cfgs)} instances, {sweep.cells / 1e6:.1f} M cells, block {sweep.block}, "
           f"group {sweep.group}", file=sys.stderr, flush=True)
@@ -394,6 +400,7 @@ def main(argv=None) -> int:
         print(f"[batch] {done} steps", file=sys.stderr, flush=True)
     dt = time.perf_counter() - t0
     res = {"instances": len(cfgs), "cells": sweep.cells, "steps": a.steps, "storage": a.storage,
+           "deviation": a.deviation,
            "block": sweep.block, "group": sweep.group, "cache_groups": len(sweep.groups),
            "cache_mib": a.cache_mib, "tuning_ms_per_step": tune,
            "seconds": dt, "mlups": sweep.cells * a.steps / dt / 1e6,
@@ -401,7 +408,7 @@ def main(argv=None) -> int:
     if a.sequential_steps:
         del sweep
         torch.cuda.empty_cache()
-        _, ts = sequential(cfgs, a.sequential_steps, a.storage)
+        _, ts = sequential(cfgs, a.sequential_steps, a.storage, deviation=a.deviation)
         res["sequential_steps"] = a.sequential_steps
         res["sequential_mlups"] = sum(c.Nx * c.Ny * c.Nz for c in cfgs) * a.sequential_steps / ts / 1e6
         res["speedup_vs_sequential"] = res["mlups"] / res["sequential_mlups"]

@@ -89,6 +89,23 @@ def test_batch_equals_each_instance_alone(pkg, gpu, block, group):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("storage,deviation", [("f16c", "inflow"), ("f16", "weights")])
+def test_16bit_batch_equals_each_instance_alone(pkg, gpu, storage, deviation):
+    bm = load_sub("batch")
+    cfgs = _small(pkg)[:3]
+    sweep = bm.BatchSweep(cfgs, storage=storage, deviation=deviation, steps=16)
+    sweep.run(9, graph=True)
+    seq, _ = bm.sequential(cfgs, 9, storage=storage, deviation=deviation)
+    import torch
+    for bl, (sl, _) in zip(sweep.lbms, seq):
+        assert bl.deviation == deviation and bl.state_word == sl.state_word
+        # the stored 16-bit codes themselves, not only their decoded values
+        np.testing.assert_array_equal(bl.state.cpu().view(torch.uint8).numpy(),
+                                      sl.state.cpu().view(torch.uint8).numpy())
+    _assert_same(list(zip(sweep.lbms, sweep.ibms)), seq)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("cache_bytes,ngroups", [(192 << 20, 1), (800_000, 2), (1, 3)])
 def test_graph_replay_equals_eager(pkg, gpu, cache_bytes, ngroups):
     bm = load_sub("batch")
