@@ -7,6 +7,7 @@
 
 #include <cstdint>
 #include <string>
+#include <type_traits>
 
 #include "../../include/clbm_amd.h"
 
@@ -97,12 +98,18 @@ __device__ __forceinline__ uint16_t f16c_encode(float v) {
 }
 // The code is read sign-extended: shifted by 12, its sign copies fill bits
 // 27-31, of which the bias covers 27-30, so the OR gives +-F * 2^128 with the
-// code's sign; the offset 2^-15 is then removed with that sign (code 0x8000
-// decodes to +0).
+// code's sign.  One FMA on its magnitude scales it back and removes the offset
+// 2^-15 (both exact, so the FMA's single rounding changes nothing), and the
+// code's sign is put back (code 0x8000 decodes to -0, which the load's
+// addition of the deviation base turns into the base itself).  The scale
+// 2^-128 is a float32 denormal: the kernels run with float32 denormals
+// preserved (the compiler's default for gfx950), which the GPU tests of the
+// codec rely on.
 __device__ __forceinline__ float f16c_decode(uint16_t hv) {
   const uint32_t hs = (uint32_t)(int32_t)(int16_t)hv;
-  const float fs = __builtin_amdgcn_ldexpf(__uint_as_float((hs << 12) | 0x78000000u), -128);
-  return fs - copysignf(3.0517578125e-05f, fs);
+  const float fs = __uint_as_float((hs << 12) | 0x78000000u);
+  const float m = __builtin_fmaf(fabsf(fs), 0x1p-128f, -3.0517578125e-05f);
+  return copysignf(m, fs);
 }
 template <> struct Store<CLBM_STORE_F16C> {
   using T = uint16_t;
@@ -265,22 +272,35 @@ __device__ __forceinline__ int sweep_block(int bid, int nblk) {
   return PAR == 0 ? bid : nblk - 1 - xcd_runs(bid, nblk);
 }
 
-// Address (element index) of post-stream population q of cell c in an AA
-// state of parity PAR: parity 0 keeps it at [q][c], parity 1 at
-// [opp(q)][c - e_q].
-template <int PAR>
-__device__ __forceinline__ size_t slot_in(int q, size_t N, int c, const Nbr& nb) {
-  if (PAR == 0) return (size_t)q * N + c;
-  return (size_t)opp(q) * N + (c + nb.off(-ex(q), -ey(q), -ez(q)));
+// Element `cell` of population plane `plane` (N cells per plane).  The plane's
+// base is uniform across the launch and the cell's byte offset a 32-bit value,
+// so loads and stores take the scalar-base + 32-bit-offset addressing form and
+// one offset register serves all planes; a plane must therefore hold less than
+// 4 GiB (check_grid rejects larger grids).  The empty asm keeps each plane's
+// base in a scalar register pair as a value of its own (otherwise the compiler
+// chains the bases into 64-bit vector adds, one per plane), and the pointer is
+// rebuilt in the global address space so that the accesses stay global, not
+// flat, ones.
+template <typename T>
+__device__ __forceinline__ uint64_t plane_base(T* f, size_t N, int plane) {
+  uint64_t base = reinterpret_cast<uint64_t>(f + (size_t)plane * N);
+  asm("" : "+s"(base));
+  return base;
 }
-// Address where the post-collision population q of cell c is written by a
-// step that starts from parity PAR: [opp(q)][c] for PAR 0, [q][c + e_q] for PAR 1.
-template <int PAR>
-__device__ __forceinline__ size_t slot_out(int q, size_t N, int c, const Nbr& nb) {
-  if (PAR == 0) return (size_t)opp(q) * N + c;
-  return (size_t)q * N + (c + nb.off(ex(q), ey(q), ez(q)));
+template <typename T>
+__device__ __forceinline__ T* base_elem(uint64_t base, int cell) {
+  using B = std::conditional_t<std::is_const<T>::value, const char, char>;
+  using GB = __attribute__((address_space(1))) B;
+  GB* p = reinterpret_cast<GB*>(base) + (uint32_t)cell * (uint32_t)sizeof(T);
+  return (T*)p;
+}
+template <typename T>
+__device__ __forceinline__ T* plane_elem(T* f, size_t N, int plane, int cell) {
+  return base_elem<T>(plane_base(f, N, plane), cell);
 }
 
+// Post-stream populations of cell c in an AA state of parity PAR: parity 0
+// keeps population q at [q][c], parity 1 at [opp(q)][c - e_q].
 template <int S, int PAR>
 __device__ __forceinline__ void load_state(const typename Store<S>::T* __restrict__ f,
                                            const Geo& g, int c, const Nbr& nb,
@@ -288,7 +308,12 @@ __device__ __forceinline__ void load_state(const typename Store<S>::T* __restric
   const DevBase<S> dv(g);
   const size_t N = (size_t)g.ncell;
 #pragma unroll
-  for (int q = 0; q < Q; ++q) fq[q] = Store<S>::ld(f + slot_in<PAR>(q, N, c, nb), dv[q]);
+  for (int q = 0; q < Q; ++q) {
+    if (PAR == 0)
+      fq[q] = Store<S>::ld(plane_elem(f, N, q, c), dv[q]);
+    else
+      fq[q] = Store<S>::ld(plane_elem(f, N, opp(q), c + nb.off(-ex(q), -ey(q), -ez(q))), dv[q]);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -297,15 +322,17 @@ __device__ __forceinline__ void load_state(const typename Store<S>::T* __restric
 template <int S> struct Pop {
   using T = typename Store<S>::T;
   using R = typename Store<S>::R;
-  T* f;
-  size_t N;
+  uint64_t base[Q];    // plane bases, computed once (uniform)
   DevBase<S> dv;
-  __device__ __forceinline__ Pop(T* f_, const Geo& g) : f(f_), N((size_t)g.ncell), dv(g) {}
+  __device__ __forceinline__ Pop(T* f, const Geo& g) : dv(g) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) base[q] = plane_base(f, (size_t)g.ncell, q);
+  }
   __device__ __forceinline__ R ld(int plane, int cell, int q) const {
-    return Store<S>::ld(f + (size_t)plane * N + cell, dv[q]);
+    return Store<S>::ld(base_elem<T>(base[plane], cell), dv[q]);
   }
   __device__ __forceinline__ void st(int plane, int cell, int q, R v) const {
-    Store<S>::st(f + (size_t)plane * N + cell, v, dv[q]);
+    Store<S>::st(base_elem<T>(base[plane], cell), v, dv[q]);
   }
 };
 
@@ -346,8 +373,13 @@ __device__ __forceinline__ void pop_store(const P& pop, int c, const Nbr& nb, co
 
 // The reference's inlet rule at x = 0 (CLBMSolver._bc_inlet, clbm_moving.py:
 // 266-294): the five east-going populations from the known ones, in place.
+// The empty asm reads the five populations the rule replaces, so that their
+// loads stay with the others above the inlet branch: the compiler would
+// otherwise move them into the other path, where they issue only after the
+// first loads have been waited for.
 template <typename R>
 __device__ __forceinline__ void inlet_rule(R* a, R ux_in, R uy_in) {
+  asm("" : "+v"(a[1]), "+v"(a[7]), "+v"(a[9]), "+v"(a[11]), "+v"(a[13]));
   const R s_known = a[0] + a[3] + a[4] + a[5] + a[6] + a[15] + a[16] + a[17] + a[18];
   const R s_out = a[2] + a[8] + a[10] + a[12] + a[14];
   const R rho_in = (s_known + R(2) * s_out) / (R(1) - ux_in);

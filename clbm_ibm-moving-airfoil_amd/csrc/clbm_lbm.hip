@@ -38,6 +38,9 @@ int check_grid(const clbm_grid_t* g, bool need_inlet_width) {
   long long nc = (long long)g->nx * g->ny * g->nz;
   if (nc >= (1ll << 31)) return fail(-1, "grid has 2^31 cells or more");
   if (elem_bytes(g->storage) == 0) return fail(-1, "unknown storage format");
+  // kernels address a population plane with 32-bit byte offsets
+  if ((unsigned long long)nc * elem_bytes(g->storage) > 0xffffffffull)
+    return fail(-1, "grid too large for this storage: one population plane would exceed 4 GiB");
   return 0;
 }
 int check_state(const clbm_grid_t* g, const clbm_params_t* p, int state, Planes& pl) {
@@ -124,22 +127,30 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, int t) {
   pop_load<PAR>(pop, c, nb, fq);
   force_at<R, FORCE, BOX>(c, x, y, z, a.fx, a.fy, a.fz, a.box, Fx, Fy, Fz);
   const bool outlet = pl.on && x == g.nx - 2;
+  // The stores' cell index and the outlet cell's are kept opaque: a thread
+  // stores into the slots it loaded from (in both AA parities), and left to
+  // itself the compiler forms full 64-bit addresses once to share them
+  // between the loads and the stores (and the outlet cell's stores), at a
+  // vector add per population instead of the scalar-base form.
+  int cs = c, c1 = c + 1;
+  asm("" : "+v"(cs));
+  asm("" : "+v"(c1));
   bool twin = true;
   if (outlet) {
-    force_at<R, FORCE, BOX>(c + 1, x + 1, y, z, a.fx, a.fy, a.fz, a.box, Gx, Gy, Gz);
+    force_at<R, FORCE, BOX>(c1, x + 1, y, z, a.fx, a.fy, a.fz, a.box, Gx, Gy, Gz);
     twin = (Gx == Fx) & (Gy == Fy) & (Gz == Fz);
-    if (!twin) pop_store_in<PAR>(pop, c + 1, make_nbr(g, x + 1, y, z), fq);
+    if (!twin) pop_store_in<PAR>(pop, c1, make_nbr(g, x + 1, y, z), fq);
   }
   if (pl.on && x == 0) inlet_rule<R>(fq, R(pl.ux_in), R(pl.uy_in));
   collide_cell<R, FORCE>(fq, Fx, Fy, Fz, R(a.s_v), R(a.s_e));
-  pop_store<PAR>(pop, c, nb, fq);
+  pop_store<PAR>(pop, cs, nb, fq);
   if (outlet) {
     const Nbr nb1 = make_nbr(g, x + 1, y, z);
     if (!twin) {
-      pop_load<PAR>(pop, c + 1, nb1, fq);
+      pop_load<PAR>(pop, c1, nb1, fq);
       collide_cell<R, FORCE>(fq, Gx, Gy, Gz, R(a.s_v), R(a.s_e));
     }
-    pop_store<PAR>(pop, c + 1, nb1, fq);
+    pop_store<PAR>(pop, c1, nb1, fq);
   }
 }
 

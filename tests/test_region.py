@@ -180,6 +180,19 @@ def test_overlapped_run_equals_single_stream_run(pkg, gpu, where):
 
 
 @pytest.mark.gpu
+def test_overlapped_run_equals_single_stream_run_at_size(pkg, gpu):
+    """A lattice large enough (1.6 M cells) that the outside part is still
+    sweeping while the marker kernels and the region's part run beside it."""
+    cfg = pkg.config.naca_case(320, 160, 32, 40.0, le_x=80.0, name="overlap-size")
+    a_lbm, a_ibm, split = _coupled(pkg, cfg, 16, True)
+    b_lbm, b_ibm, nosplit = _coupled(pkg, cfg, 16, False)
+    assert nosplit == 0 and split == 16
+    np.testing.assert_array_equal(a_lbm.f.to_numpy(), b_lbm.f.to_numpy())
+    np.testing.assert_array_equal(a_ibm.force_history(), b_ibm.force_history())
+    assert a_ibm.region_violations() == 0
+
+
+@pytest.mark.gpu
 def test_overlap_splits_the_benchmark_shape(pkg, gpu):
     """On a body small against the lattice every step takes the split path."""
     cfg = pkg.config.naca_case(96, 48, 4, 12.0, name="small")
