@@ -75,7 +75,8 @@ extern "C" {
 #define IBM_REGION_FLAG(box) ((box) + 2 * 6 * IBM_BOX_SUBSLOTS + 3)
 
 typedef struct {
-  int32_t nx, ny, nz; /* lattice size, Nx >= 3 */
+  int32_t nx, ny, nz; /* lattice size, Nx >= 3; one population plane (nx*ny*nz elements of
+                         the storage format) must stay under 4 GiB, or calls fail with -1 */
   int32_t storage;    /* CLBM_STORE_* */
   float dev_ux, dev_uy; /* 16-bit formats store f_q - feq_q(rho = 1, u = (dev_ux, dev_uy, 0));
                            0, 0 stores f_q - w_q.  Part of the buffer's format: every call
