@@ -147,11 +147,13 @@ def measure(pkg, cfg, storage, steps, warmup, dev, tune, deviation="weights"):
         if timed:
             cells.append(lbm.last_kernel_cells)
         step += 1
+    t_host = time.perf_counter() - t0      # the host has queued every step
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     rho_min, finite = lbm.health()
     k_ms = float(np.mean([e0.elapsed_time(e1) for e0, e1 in ev]))
-    out = dict(dt=dt, k_ms=k_ms, k_cells=float(np.mean(cells)), rho_min=rho_min, finite=finite)
+    out = dict(dt=dt, k_ms=k_ms, k_cells=float(np.mean(cells)), rho_min=rho_min, finite=finite,
+               host_ms=t_host / steps * 1e3)
     del lbm, ibm
     torch.cuda.empty_cache()
     return out
@@ -195,6 +197,7 @@ def main(argv=None) -> int:
             others[st] = {"value": round(m, 1), "deviation": "inflow",
                           "ms_per_step": round(o["dt"] / a.steps * 1e3, 5),
                           "kernel_ms": round(o["k_ms"], 5),
+                          "host_ms_per_step": round(o["host_ms"], 5),
                           "kernel_frac": round(2 * 19 * 2 * o["k_cells"] / (o["k_ms"] * 1e-3)
                                                / 1e9 / HBM_PEAK_GBS, 4),
                           "finite": o["finite"]}
@@ -223,6 +226,7 @@ def main(argv=None) -> int:
                      "cells_per_launch": round(k_cells),
                      "bytes_per_launch": round(b_alg),
                      "step_frac": round(mlups * 1e6 * 2 * 19 * ELEM[a.storage] / 1e9 / HBM_PEAK_GBS, 4)},
+        "host_ms_per_step": round(r["host_ms"], 5),
         "cpu_baseline": cpu,
         "health": {"rho_min": rho_min, "finite": finite},
         "storage_16bit": {k: dict(v, ratio_vs_f32=round(v["value"] / mlups, 3))

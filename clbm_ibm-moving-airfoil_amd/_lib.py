@@ -23,7 +23,7 @@ BOX_SUBSLOTS = 64
 BOX_WORDS = 2 * 6 * BOX_SUBSLOTS + 4        # uint64 words of a force-box record
 FORCE_BOX_OFFSET = 2 * 6 * BOX_SUBSLOTS     # word offset of the decoded int32[6] box
 REGION_FLAG_OFFSET = 2 * 6 * BOX_SUBSLOTS + 3   # word offset of the region-violation flag
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 
 class ClbmGrid(C.Structure):
@@ -77,6 +77,11 @@ _SIGNATURES = {
     "clbm_step": (_I, [_G, _LP, _P, _I, _P, _P, _P, _P, _P]),
     "clbm_step_part": (_I, [_G, _LP, _P, _I, _P, _P, _P, _P, _P, _I, _P]),
     "clbm_step_region": (_I, [_G, _LP, _P, _P]),
+    "ibm_step_region": (_I, [_G, _LP, _P, _P, _I, _I, _I, C.c_double, C.c_double, C.c_double,
+                             _P]),
+    "clbm_event_create": (_I, [C.POINTER(C.c_void_p)]),
+    "clbm_event_destroy": (_I, [_P]),
+    "clbm_stream_after": (_I, [_P, _P, _P]),
     "clbm_macro": (_I, [_G, _LP, _P, _I, _P, _P, _P, _P, _P]),
     "clbm_export_state": (_I, [_G, _LP, _P, _I, _P, _P]),
     "clbm_export_state_f64": (_I, [_G, _LP, _P, _I, _P, _P]),
@@ -141,13 +146,29 @@ def load():
         return lib
 
 
+_fns = {}
+
+
+def fn(name: str):
+    """The ctypes function object of an entry point (looked up once)."""
+    f = _fns.get(name)
+    if f is None:
+        f = _fns[name] = getattr(load(), name)
+    return f
+
+
+def check(rc: int, name: str) -> None:
+    """Raise ClbmError with the library's message when an entry point returned rc != 0."""
+    if rc != 0:
+        msg = load().clbm_last_error().decode(errors="replace")
+        raise ClbmError(f"{name} failed ({rc}): {msg}")
+
+
 def call(name: str, *args) -> None:
     """Call an entry point and raise ClbmError with the library's message on failure."""
-    lib = load()
-    rc = getattr(lib, name)(*args)
+    rc = fn(name)(*args)
     if rc != 0:
-        msg = lib.clbm_last_error().decode(errors="replace")
-        raise ClbmError(f"{name} failed ({rc}): {msg}")
+        check(rc, name)
 
 
 def exported_symbols() -> list:

@@ -46,10 +46,14 @@ def stencil_region(hx, hy, zr, shape, xc, yc, theta):
     every other marker's float32 position differs from its exact one by a few
     ulp, so one cell of margin on each side covers it.  x is clamped to the
     grid; a y range that crosses the periodic boundary takes the whole axis.
-    None for non-finite kinematics."""
+    None for non-finite kinematics.  The library's host function
+    ibm_step_region computes the same box (and its normalisation) per step;
+    this is its statement in NumPy."""
     nx, ny, nz = shape
     f32 = np.float32
     t = np.float64(f32(theta))
+    if not math.isfinite(t):
+        return None
     c, s = f32(math.cos(t)), f32(math.sin(t))
     X = (f32(xc) + hx * c) - hy * s
     Y = (f32(yc) + hx * s) + hy * c
@@ -136,6 +140,9 @@ class DroneMovingIBM:
         self.fz = Field(shape, torch.zeros((self.Nz, self.Ny, self.Nx), dtype=f32, device=dev),
                         refresh=self._join, owner=self, box=box_ptr)
         self._acc = torch.zeros((2, ncell), dtype=torch.int64, device=dev)
+        # what the solver's split step reads: the force fields and their box
+        self._force_ptrs = (C.c_void_p(self.fx._t.data_ptr()), C.c_void_p(self.fy._t.data_ptr()),
+                            C.c_void_p(box_ptr))
         self.ip = _lib.IbmParams(float(cfg.f_max if f_max is None else f_max),
                                  int(cfg.wrap_weight_quirk if wrap_weight_quirk is None
                                      else wrap_weight_quirk),
@@ -149,11 +156,19 @@ class DroneMovingIBM:
             raise ValueError("overlap must be True, False or 'auto'")
         self.overlap = overlap
         hull = convex_hull_2d(cfg.X0, cfg.Y0) if self.N else np.zeros(0, np.int64)
-        self._hx = np.asarray(cfg.X0, dtype=np.float32)[hull]
-        self._hy = np.asarray(cfg.Y0, dtype=np.float32)[hull]
+        self._hx = np.ascontiguousarray(np.asarray(cfg.X0, dtype=np.float32)[hull])
+        self._hy = np.ascontiguousarray(np.asarray(cfg.Y0, dtype=np.float32)[hull])
         self._zr = stencil_z_range(cfg.Z0, self.Nz) if self.N else (0, self.Nz - 1)
+        self._hull_args = (C.c_void_p(self._hx.ctypes.data), C.c_void_p(self._hy.ctypes.data),
+                           len(self._hx), int(self._zr[0]), int(self._zr[1]))
+        # the concurrent stream and the two events that order it against the
+        # caller's stream (fork before the marker kernels, join after them)
         self._side = None
-        self._ev_start = self._ev_done = None
+        self._side_h = None
+        self._ev = []
+        # ctypes arguments of the marker and clamp calls that stay fixed for a
+        # velocity source: (source key, before-state, after-state, clamp prefix)
+        self._argcache = None
         self._hist = None
         self._hist_n = 0
         self._sums = torch.zeros(2, dtype=torch.float64, device=dev)
@@ -166,78 +181,96 @@ class DroneMovingIBM:
         fields (read straight from its populations) or any [Nz,Ny,Nx] float32
         device Field/tensor.  Returns (fx, fy, fz), this object's own fields."""
         import torch
-        p = lambda m: C.c_void_p(m._t.data_ptr() if isinstance(m, MarkerField) else m.data_ptr())
-        grid, lp, f, state, uxp, uyp = self._velocity_source(ux, uy)
-        from .clbm_moving import CLBMSolver
-        solver = getattr(ux, "owner", None) if uxp == 0 else None
-        solver = solver if isinstance(solver, CLBMSolver) else None
+        pre, post, clip, solver, state = self._call_args(ux, uy)
         region = self._region(solver, xc, yc, theta) if solver is not None else None
-        main = torch.cuda.current_stream(self.device)
+        main_h = torch.cuda.current_stream(self.device).cuda_stream
         if region is not None:
             # run on this object's stream once everything queued so far is done
             if self._side is None:
                 self._side = torch.cuda.Stream(device=self.device)
-                self._ev_start = torch.cuda.Event()
-                self._ev_done = torch.cuda.Event()
-            self._ev_start.record(main)
-            self._side.wait_event(self._ev_start)
-            st = C.c_void_p(self._side.cuda_stream)
+                self._side_h = C.c_void_p(self._side.cuda_stream)
+                self._ev = [self._new_event(), self._new_event()]
+            _lib.check(_lib.fn("clbm_stream_after")(self._side_h, main_h, self._ev[0]),
+                       "clbm_stream_after")
+            st = self._side_h
             self.ip.check_region = 1
             self.ip.region[:] = list(region)
         else:
             self._join()
-            st = C.c_void_p(main.cuda_stream)
+            st = main_h
             self.ip.check_region = 0
         self.gen += 1
-        boxp = C.c_void_p(self.box.data_ptr())
-        acc0, acc1 = C.c_void_p(self._acc[0].data_ptr()), C.c_void_p(self._acc[1].data_ptr())
-        _lib.call("ibm_compute_force", C.byref(grid), C.byref(self.ip), lp, C.c_void_p(f), state,
-                  C.c_void_p(uxp), C.c_void_p(uyp), self.N,
-                  C.c_void_p(self.order.data_ptr() if self.order is not None else 0),
-                  p(self.X0), p(self.Y0),
-                  p(self.Z0), float(xc), float(yc), float(theta), float(vx), float(vy),
-                  float(omega), p(self.X_marker), p(self.Y_marker), p(self.Z_marker),
-                  p(self.VX_body), p(self.VY_body), p(self.U_k), p(self.V_k), p(self.FX_k),
-                  p(self.FY_k),
-                  C.c_void_p(self._stencil_base.data_ptr() if self._stencil_base is not None
-                             else 0),
-                  acc0, acc1, boxp, self.gen, st)
-        _lib.call("ibm_clip_forces", C.byref(grid), C.byref(self.ip), acc0, acc1,
-                  C.c_void_p(self.fx._t.data_ptr()), C.c_void_p(self.fy._t.data_ptr()), boxp,
-                  self.gen, st)
+        _lib.check(_lib.fn("ibm_compute_force")(*pre, state, *post[0], float(xc), float(yc),
+                                                float(theta), float(vx), float(vy), float(omega),
+                                                *post[1], self.gen, st), "ibm_compute_force")
+        _lib.check(_lib.fn("ibm_clip_forces")(*clip, self.gen, st), "ibm_clip_forces")
         if region is not None:
-            self._ev_done.record(self._side)
             self._pending = True
-            solver._add_reader(self, region, self._ev_done)
+            solver._add_reader(self, region)
         return self.fx, self.fy, self.fz
 
+    def _call_args(self, ux, uy):
+        """ctypes arguments of ibm_compute_force / ibm_clip_forces around the
+        per-call ones, the solver whose populations give the velocity (or
+        None) and its state word.  Cached for a solver source; built anew for
+        velocity tensors."""
+        from .clbm_moving import CLBMSolver
+        owner = getattr(ux, "owner", None)
+        solver = owner if (isinstance(owner, CLBMSolver) and getattr(uy, "owner", None) is owner) \
+            else None
+        if solver is not None and self._argcache is not None and self._argcache[0] is solver:
+            return (*self._argcache[1:], solver, solver.state_word)
+        grid, lp, f, state, uxp, uyp = self._velocity_source(ux, uy)
+        P = C.c_void_p
+        m = lambda a: P(a._t.data_ptr())
+        gref, ipref = C.byref(grid), C.byref(self.ip)
+        acc0, acc1, boxp = P(self._acc[0].data_ptr()), P(self._acc[1].data_ptr()), P(self.box.data_ptr())
+        pre = (gref, ipref, lp, P(f))
+        post = ((P(uxp), P(uyp), self.N, P(self.order.data_ptr() if self.order is not None else 0),
+                 m(self.X0), m(self.Y0), m(self.Z0)),
+                (m(self.X_marker), m(self.Y_marker), m(self.Z_marker), m(self.VX_body),
+                 m(self.VY_body), m(self.U_k), m(self.V_k), m(self.FX_k), m(self.FY_k),
+                 P(self._stencil_base.data_ptr() if self._stencil_base is not None else 0),
+                 acc0, acc1, boxp))
+        clip = (gref, ipref, acc0, acc1, P(self.fx._t.data_ptr()), P(self.fy._t.data_ptr()), boxp)
+        if solver is not None:
+            self._argcache = (solver, pre, post, clip)
+        return pre, post, clip, solver, state
+
     # -- the concurrent stream ------------------------------------------------
+    def _new_event(self) -> C.c_void_p:
+        ev = C.c_void_p()
+        _lib.call("clbm_event_create", C.byref(ev))
+        return ev
+
+    def __del__(self):
+        try:
+            for ev in self._ev:
+                _lib.fn("clbm_event_destroy")(ev)
+        except Exception:  # noqa: BLE001 (interpreter shutdown)
+            pass
+
     def _region(self, solver, xc, yc, theta):
         """The normalised int32[6] region holding every stencil cell of this
-        call, or None when overlapping would not pay (region over half the
-        lattice) or is switched off."""
+        call (ibm_step_region), or None when overlapping would not pay (region
+        over half the lattice) or is switched off."""
         if not self.overlap or self.N == 0:
             return None
-        r = stencil_region(self._hx, self._hy, self._zr, (self.Nx, self.Ny, self.Nz), xc, yc, theta)
-        if r is None:
-            return None
-        raw = (C.c_int32 * 6)(*r)
         out = (C.c_int32 * 6)()
-        _lib.call("clbm_step_region", C.byref(solver.grid), C.byref(solver.params), raw, out)
-        vol = (out[3] - out[0] + 1) * (out[4] - out[1] + 1) * (out[5] - out[2] + 1)
-        if 2 * vol > self.Nx * self.Ny * self.Nz:
-            return None
-        return out
+        rc = _lib.fn("ibm_step_region")(C.byref(solver.grid), C.byref(solver.params),
+                                        *self._hull_args, float(xc), float(yc), float(theta), out)
+        if rc < 0:
+            _lib.check(rc, "ibm_step_region")
+        return out if rc == 1 else None
 
     def _join(self) -> None:
         """Order the current stream after this object's pending kernels."""
         if self._pending:
             import torch
-            torch.cuda.current_stream(self.device).wait_event(self._ev_done)
+            main_h = torch.cuda.current_stream(self.device).cuda_stream
+            _lib.check(_lib.fn("clbm_stream_after")(main_h, self._side_h, self._ev[1]),
+                       "clbm_stream_after")
             self._pending = False
-
-    def _joined(self) -> None:
-        self._pending = False
 
     @property
     def stencil_base(self):

@@ -128,6 +128,9 @@ class CLBMSolver:
         nbytes = Q * self.Nx * self.Ny * self.Nz * _lib.ELEM_BYTES[self.grid.storage]
         self.state = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
         self.state_word = 0
+        # ctypes arguments that stay fixed for this solver
+        self._gref, self._pref = C.byref(self.grid), C.byref(self.params)
+        self._state_ptr = C.c_void_p(self.state.data_ptr())
         self.steps = 0
         shape = (self.Nx, self.Ny, self.Nz)
         self._macro = None
@@ -166,22 +169,21 @@ class CLBMSolver:
         self.kernel_events = None
         main = torch.cuda.current_stream(self.device)
         st = C.c_void_p(main.cuda_stream)
-        state = C.c_void_p(self.state.data_ptr())
+        state = self._state_ptr
         if split is not None:
-            ibm, region, done = split
+            ibm, region = split
+            part = _lib.fn("clbm_step_part")
             if ev is not None:
                 ev[0].record(main)
-            _lib.call("clbm_step_part", C.byref(self.grid), C.byref(self.params), state,
-                      self.state_word, None, None, None, None, region, 0, st)
+            _lib.check(part(self._gref, self._pref, state, self.state_word, None, None, None,
+                            None, region, 0, st), "clbm_step_part")
             if ev is not None:
                 ev[1].record(main)
-            main.wait_event(done)
-            ibm._joined()
+            ibm._join()
             self._readers.clear()
-            _lib.call("clbm_step_part", C.byref(self.grid), C.byref(self.params), state,
-                      self.state_word, C.c_void_p(ibm.fx._t.data_ptr()),
-                      C.c_void_p(ibm.fy._t.data_ptr()), None, C.c_void_p(ibm.fx.box), region, 1,
-                      st)
+            fxp, fyp, boxp = ibm._force_ptrs
+            _lib.check(part(self._gref, self._pref, state, self.state_word, fxp, fyp, None, boxp,
+                            region, 1, st), "clbm_step_part")
             r = list(region)
             self.last_kernel_cells = self.ncell - ((r[3] - r[0] + 1) * (r[4] - r[1] + 1)
                                                    * (r[5] - r[2] + 1))
@@ -211,35 +213,32 @@ class CLBMSolver:
             if ev is not None:
                 ev[1].record(main)
             self.last_kernel_cells = self.ncell
-        self.state_word = _lib.load().clbm_state_after_step(self.state_word, C.byref(self.params))
+        self.state_word = _lib.fn("clbm_state_after_step")(self.state_word, self._pref)
         self.steps += 1
         self._macro_valid = False
 
     # -- concurrent readers (immersed-boundary objects on their own stream) --
-    def _add_reader(self, ibm, region, done_event) -> None:
-        self._readers[id(ibm)] = (ibm, region, done_event)
+    def _add_reader(self, ibm, region) -> None:
+        self._readers[id(ibm)] = (ibm, region)
 
     def _wait_readers(self) -> None:
         """Order the current stream after every pending concurrent reader
         (before anything that writes the populations)."""
         if not self._readers:
             return
-        import torch
-        main = torch.cuda.current_stream(self.device)
-        for ibm, _, done in self._readers.values():
-            main.wait_event(done)
-            ibm._joined()
+        for ibm, _ in self._readers.values():
+            ibm._join()
         self._readers.clear()
 
     def _split_source(self, fx, fy, fz):
-        """(ibm, region, event) when (fx, fy, fz) are the fields of the one
+        """(ibm, region) when (fx, fy, fz) are the fields of the one
         immersed-boundary object reading this solver on its own stream."""
         if len(self._readers) != 1:
             return None
-        ibm, region, done = next(iter(self._readers.values()))
+        ibm, region = next(iter(self._readers.values()))
         if fx is not ibm.fx or fy is not ibm.fy or (fz is not None and fz is not ibm.fz):
             return None
-        return ibm, region, done
+        return ibm, region
 
     @property
     def parity(self) -> int:

@@ -1151,6 +1151,47 @@ int ibm_batch_compute(const ibm_batch_t* b, int state, void* stream) {
   return check_launch("ibm_batch_compute");
 }
 
+int ibm_step_region(const clbm_grid_t* g, const clbm_params_t* p, const float* hx,
+                    const float* hy, int nh, int z0, int z1, double xc, double yc, double theta,
+                    int32_t* out) {
+  // float32 placement as the marker kernel does it: cos/sin of the float32
+  // angle rounded once from float64, each product and sum rounded on its own
+#pragma clang fp contract(off)
+  if (!p || !out || (nh > 0 && (!hx || !hy))) return fail(-1, "ibm_step_region: NULL argument");
+  if (int e = check_grid(g, p->inlet != 0)) return e;
+  if (nh < 0 || z0 > z1) return fail(-1, "ibm_step_region: bad hull count or z range");
+  if (nh == 0) return 0;
+  const float t = (float)theta;
+  const float c = (float)std::cos((double)t), s = (float)std::sin((double)t);
+  const float fxc = (float)xc, fyc = (float)yc;
+  float xmin = 0.f, xmax = 0.f, ymin = 0.f, ymax = 0.f;
+  for (int i = 0; i < nh; ++i) {
+    const float hxc = hx[i] * c, hys = hy[i] * s, hxs = hx[i] * s, hyc = hy[i] * c;
+    const float X = (fxc + hxc) - hys;
+    const float Y = (fyc + hxs) + hyc;
+    if (!std::isfinite(X) || !std::isfinite(Y)) return 0;
+    if (i == 0 || X < xmin) xmin = X;
+    if (i == 0 || X > xmax) xmax = X;
+    if (i == 0 || Y < ymin) ymin = Y;
+    if (i == 0 || Y > ymax) ymax = Y;
+  }
+  // stencil cells floor(p) - 1 .. floor(p) + 2, plus one cell of margin
+  const double x0 = std::max(std::floor((double)xmin) - 2.0, 0.0);
+  const double x1 = std::min(std::floor((double)xmax) + 3.0, (double)(g->nx - 1));
+  if (x0 > x1) return 0;
+  double y0 = std::floor((double)ymin) - 2.0, y1 = std::floor((double)ymax) + 3.0;
+  if (y0 < 0.0 || y1 > (double)(g->ny - 1)) {  // crosses the periodic boundary: whole axis
+    y0 = 0.0;
+    y1 = (double)(g->ny - 1);
+  }
+  const int32_t raw[6] = {(int32_t)x0, (int32_t)y0, z0, (int32_t)x1, (int32_t)y1, z1};
+  if (int e = clbm_step_region(g, p, raw, out)) return e;
+  const long long vol = (long long)(out[3] - out[0] + 1) * (out[4] - out[1] + 1) *
+                        (out[5] - out[2] + 1);
+  const long long ncell = (long long)g->nx * g->ny * g->nz;
+  return 2 * vol > ncell ? 0 : 1;  // a region over half the lattice does not pay
+}
+
 int ibm_force_sum_f64(int n, const float* FX, const float* FY, const float* dV, double* out2,
                       void* stream) {
   if (n < 0) return fail(-1, "ibm_force_sum_f64: n < 0");

@@ -537,6 +537,30 @@ int clbm_batch_step(const clbm_batch_t* b, int state, void* stream) {
 
 int clbm_abi_version(void) { return CLBM_ABI_VERSION; }
 
+int clbm_event_create(void** ev) {
+  if (!ev) return fail(-1, "clbm_event_create: NULL argument");
+  hipEvent_t e = nullptr;
+  const hipError_t r = hipEventCreateWithFlags(&e, hipEventDisableTiming);
+  if (r != hipSuccess) return fail((int)r, std::string("clbm_event_create: ") + hipGetErrorString(r));
+  *ev = (void*)e;
+  return 0;
+}
+
+int clbm_event_destroy(void* ev) {
+  if (!ev) return 0;
+  const hipError_t r = hipEventDestroy((hipEvent_t)ev);
+  if (r != hipSuccess) return fail((int)r, std::string("clbm_event_destroy: ") + hipGetErrorString(r));
+  return 0;
+}
+
+int clbm_stream_after(void* waiter, void* signaler, void* ev) {
+  if (!ev) return fail(-1, "clbm_stream_after: NULL event");
+  hipError_t r = hipEventRecord((hipEvent_t)ev, (hipStream_t)signaler);
+  if (r == hipSuccess) r = hipStreamWaitEvent((hipStream_t)waiter, (hipEvent_t)ev, 0);
+  if (r != hipSuccess) return fail((int)r, std::string("clbm_stream_after: ") + hipGetErrorString(r));
+  return 0;
+}
+
 const char* clbm_last_error(void) { return g_last_error.c_str(); }
 
 size_t clbm_state_bytes(const clbm_grid_t* g) {

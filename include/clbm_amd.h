@@ -49,7 +49,7 @@
 extern "C" {
 #endif
 
-#define CLBM_ABI_VERSION 4
+#define CLBM_ABI_VERSION 5
 
 /* population storage formats */
 #define CLBM_STORE_F32 0  /* float32 populations, float32 arithmetic (the drop-in) */
@@ -157,6 +157,30 @@ int clbm_step_part(const clbm_grid_t* g, const clbm_params_t* p, void* f, int st
  * neighbour); result in `out`. */
 int clbm_step_region(const clbm_grid_t* g, const clbm_params_t* p, const int32_t* region,
                      int32_t* out);
+
+/* Host-only: the split region of one body for one step, from its template's
+ * convex-hull vertices (hx, hy: nh float32 body-frame points) placed by
+ * (xc, yc, theta) with the marker kernel's float32 arithmetic, the stencil
+ * reach floor(p) - 1 .. floor(p) + 2 plus one cell of margin in x and y, x
+ * clipped to the grid, y the whole axis when it crosses the periodic
+ * boundary, z from z0..z1, normalised as clbm_step_region does.  Returns 1
+ * with the region in `out`, 0 when no split pays (non-finite kinematics, body
+ * off the grid in x, or a region over half the lattice), < 0 on argument
+ * errors.  No reference counterpart: the reference runs the whole step after
+ * the markers (main_moving.py:116-122). */
+int ibm_step_region(const clbm_grid_t* g, const clbm_params_t* p, const float* hx,
+                    const float* hy, int nh, int z0, int z1, double xc, double yc, double theta,
+                    int32_t* out);
+
+/* Host-only stream ordering for running the marker kernels beside the
+ * lattice step (no reference counterpart; Taichi runs one queue):
+ * clbm_event_create makes a synchronisation-only HIP event, clbm_event_destroy
+ * releases it, and clbm_stream_after records `ev` on `signaler` and makes
+ * `waiter` wait for it (work queued on `waiter` afterwards starts after
+ * everything queued on `signaler` so far). */
+int clbm_event_create(void** ev);
+int clbm_event_destroy(void* ev);
+int clbm_stream_after(void* waiter, void* signaler, void* ev);
 
 /* Replaces CLBMSolver._macro_post (clbm_moving.py:305-321): force-free rho
  * and u of the current state into [z][y][x] float32 fields (each nullable). */

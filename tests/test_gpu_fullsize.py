@@ -1,0 +1,115 @@
+"""Parity at the full 600 x 200 x 50 workload (the benchmarked case: chord 80,
+8,200 markers, tau 0.56), where the NumPy oracle would take minutes per step.
+
+* Against the C restatement of the reference's step (oracle/clbm_ref.c,
+  float32, OpenMP): marker positions and stencil base indices bit for bit
+  (both place markers with float32 arithmetic from cos/sin of the float32
+  angle rounded once from float64); populations, marker velocities and
+  forces, force fields and force sums within float32 tolerances.  The two
+  paths evaluate the same float32 formulas in different orders (the small
+  grids of the other tests put each within 2e-6 of the float64 oracle per
+  few steps), so after 4 coupled steps populations agree to 1e-5 absolute,
+  marker quantities and force fields to 2e-6 absolute, force sums to 1e-4
+  relative.
+* Mass: with x periodic (no inlet/outlet planes) the moving body's Guo
+  forcing adds no mass, so the lattice mass is conserved: 1e-12 relative
+  over 30 steps with float64 storage, 1e-6 with float32.
+* The overlapped split step (marker kernels on their own stream beside the
+  outside part) equals the one-stream step bit for bit.
+"""
+
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+pytestmark = pytest.mark.gpu
+
+STEPS = 4
+
+
+@pytest.fixture(scope="module")
+def ref_c():
+    so = os.path.join(ROOT, "oracle", "libclbm_ref.so")
+    if not os.path.exists(so):
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle")], check=True,
+                       stdout=subprocess.DEVNULL)
+    import clbm_ref_c
+    return clbm_ref_c
+
+
+def test_full_size_coupled_steps_match_the_c_restatement(pkg, gpu, ref_c):
+    cfg = pkg.config.case_c1()
+    assert (cfg.Nx, cfg.Ny, cfg.Nz, cfg.N_MARKERS) == (600, 200, 50, 8200)
+    sim = ref_c.RefSim(cfg, threads=min(16, os.cpu_count() or 1))
+    lbm = pkg.CLBMSolver(cfg, device=gpu)
+    ibm = pkg.DroneMovingIBM(cfg, device=gpu, record_stencil=True)
+    sums_gpu, sums_c = [], []
+    out = np.zeros(2)
+    for n in range(1, STEPS + 1):
+        kin = cfg.motion_state(n)
+        fx, fy, fz = ibm.compute_force(lbm.ux, lbm.uy, *kin)
+        sums_gpu.append(ibm.force_sums().cpu().numpy().copy())
+        lbm.step(fx, fy, fz)
+        lib = ref_c._load()
+        lib.ref_compute_force(sim.h, *[float(k) for k in kin])
+        lib.ref_force_sum(sim.h, ref_c._ptr(sim.dV), ref_c._ptr(out))
+        sums_c.append(out.copy())
+        if n == STEPS:
+            m = sim.markers()
+            cfx, cfy = sim.forces()
+        lib.ref_step(sim.h)
+    # markers after the last compute_force: positions and stencils exact
+    np.testing.assert_array_equal(ibm.X_marker.to_numpy(), m["X"])
+    np.testing.assert_array_equal(ibm.Y_marker.to_numpy(), m["Y"])
+    base = ibm.stencil_base.cpu().numpy()
+    want = np.stack([np.floor(m[k]).astype(np.int64) - 1 for k in ("X", "Y", "Z")], 1)
+    np.testing.assert_array_equal(base, want)
+    for k, mine in (("U", ibm.U_k), ("V", ibm.V_k), ("FX", ibm.FX_k), ("FY", ibm.FY_k)):
+        np.testing.assert_allclose(mine.to_numpy(), m[k], rtol=0, atol=2e-6, err_msg=k)
+    np.testing.assert_allclose(ibm.fx.to_numpy(), cfx, rtol=0, atol=2e-6)
+    np.testing.assert_allclose(ibm.fy.to_numpy(), cfy, rtol=0, atol=2e-6)
+    np.testing.assert_allclose(np.array(sums_gpu), np.array(sums_c), rtol=1e-4, atol=1e-9)
+    # populations after the last step
+    d = np.abs(lbm.f.to_numpy() - sim.f())
+    assert d.max() < 1e-5, f"max |f_gpu - f_c| = {d.max():.3e}"
+
+
+@pytest.mark.parametrize("storage,rtol", [("f64", 1e-12), ("f32", 1e-6)])
+def test_full_size_mass_is_conserved_in_an_x_periodic_box(pkg, gpu, storage, rtol):
+    cfg = pkg.config.case_c1()
+    lbm = pkg.CLBMSolver(cfg, storage=storage, device=gpu, inlet=False)
+    ibm = pkg.DroneMovingIBM(cfg, device=gpu)
+
+    def mass():
+        # from the populations in their own precision (the macroscopic fields
+        # are float32 whatever the storage); float64 pairwise summation
+        return float(lbm.f.to_numpy().sum(dtype=np.float64))
+
+    m0 = mass()
+    for n in range(1, 31):
+        fx, fy, fz = ibm.compute_force(lbm.ux, lbm.uy, *cfg.motion_state(n))
+        lbm.step(fx, fy, fz)
+    assert float(np.abs(ibm.FX_k.to_numpy()).max()) > 0      # the body does push
+    assert abs(mass() - m0) / m0 < rtol
+
+
+def test_full_size_overlapped_step_equals_one_stream(pkg, gpu):
+    cfg = pkg.config.case_c1()
+    runs = []
+    for overlap in (True, False):
+        lbm = pkg.CLBMSolver(cfg, device=gpu)
+        ibm = pkg.DroneMovingIBM(cfg, device=gpu, overlap=overlap)
+        split = 0
+        for n in range(1, 7):
+            fx, fy, fz = ibm.compute_force(lbm.ux, lbm.uy, *cfg.motion_state(n))
+            lbm.step(fx, fy, fz)
+            split += lbm.last_kernel_cells < lbm.ncell
+        runs.append((lbm.f.to_numpy(), ibm.FX_k.to_numpy(), split))
+        del lbm, ibm
+    assert runs[0][2] == 6 and runs[1][2] == 0
+    np.testing.assert_array_equal(runs[0][0], runs[1][0])
+    np.testing.assert_array_equal(runs[0][1], runs[1][1])

@@ -58,6 +58,51 @@ def test_predicted_region_holds_every_stencil_cell(pkg, oracle, seed):
         assert kk.min() >= z0 and kk.max() <= z1
 
 
+@pytest.mark.parametrize("seed", range(3))
+def test_library_region_equals_the_python_statement(pkg, seed):
+    """ibm_step_region (host code of the library) gives the normalised box of
+    stencil_region, or no split where that gives none or the box exceeds half
+    the lattice, bit for bit over random, rounded, off-grid, seam-crossing and
+    non-finite kinematics."""
+    am = load_sub("airfoil_moving")
+    lib = load_sub("_lib")
+    L = lib.load()
+    X0, Y0 = _template(pkg)
+    h = pkg.config.convex_hull_2d(X0, Y0)
+    hx = np.ascontiguousarray(X0[h], dtype=np.float32)
+    hy = np.ascontiguousarray(Y0[h], dtype=np.float32)
+    rng = np.random.default_rng(seed)
+    for nx, ny, nz, inlet in ((600, 200, 50, 1), (240, 120, 8, 1), (96, 40, 4, 0)):
+        grid = lib.ClbmGrid(nx, ny, nz, lib.STORE_F32)
+        params = lib.ClbmParams(1.8, 1.0, 0.05, 0.0, inlet, 0)
+        for zr in ((0, nz - 1), (1, nz - 2)):
+            for i in range(150):
+                xc = float(rng.uniform(-60, nx + 60))
+                yc = float(rng.uniform(-20, ny + 20))
+                th = float(rng.uniform(-np.pi, np.pi))
+                if i % 5 == 0:
+                    xc, yc = float(np.round(xc)), float(np.round(yc))
+                if i == 7:
+                    xc = float("nan")
+                if i == 8:
+                    th = float("inf")
+                if i == 9:
+                    xc = 1e30
+                r = am.stencil_region(hx, hy, zr, (nx, ny, nz), xc, yc, th)
+                want = None
+                if r is not None:
+                    raw = (C.c_int32 * 6)(*r)
+                    norm = (C.c_int32 * 6)()
+                    assert L.clbm_step_region(C.byref(grid), C.byref(params), raw, norm) == 0
+                    vol = (norm[3] - norm[0] + 1) * (norm[4] - norm[1] + 1) * (norm[5] - norm[2] + 1)
+                    want = None if 2 * vol > nx * ny * nz else list(norm)
+                out = (C.c_int32 * 6)()
+                rc = L.ibm_step_region(C.byref(grid), C.byref(params), hx.ctypes.data,
+                                       hy.ctypes.data, len(hx), zr[0], zr[1], xc, yc, th, out)
+                assert rc in (0, 1), L.clbm_last_error()
+                assert (list(out) if rc == 1 else None) == want, (xc, yc, th, zr)
+
+
 def test_z_range(pkg):
     am = load_sub("airfoil_moving")
     assert am.stencil_z_range(np.array([3.0, 4.5], np.float32), 10) == (2, 6)
