@@ -200,6 +200,7 @@ def main(argv=None) -> int:
                           "host_ms_per_step": round(o["host_ms"], 5),
                           "kernel_frac": round(2 * 19 * 2 * o["k_cells"] / (o["k_ms"] * 1e-3)
                                                / 1e9 / HBM_PEAK_GBS, 4),
+                          "traffic": traffic_record(a.case, st),
                           "finite": o["finite"]}
 
     mlups = a.steps * ncell / dt / 1e6
