@@ -103,10 +103,13 @@ def cache_groups(cfgs, storage: str, cache_bytes: int):
 
 
 class _Group:
-    """Device tables and captured graph of the instances of one cache group."""
+    """Device tables and captured graph of the instances of one cache group.
+    A ``solo`` group (one lattice large enough that its step is bound by HBM
+    on its own) has neither: its instance steps through its own objects."""
 
-    def __init__(self, idx):
+    def __init__(self, idx, solo: bool = False):
         self.idx = idx
+        self.solo = solo
         self.lb = self.ib = self.lwork = self.iwork = None
         self.graph = None
         self.graph_state = None
@@ -118,11 +121,19 @@ class BatchSweep:
     The instances are split into groups that fit in ``cache_bytes`` (by
     default 192 MiB of the MI355X's 256 MB last-level cache) and ``run``
     advances one group at a time for all its steps, so a group's populations
-    stay in that cache from step to step."""
+    stay in that cache from step to step.
+
+    A group of one lattice of ``solo_min_cells`` cells or more (by default
+    the size from which DroneMovingIBM overlaps its marker kernels with the
+    step) is stepped through that instance's own solver and IBM objects:
+    the split step with the marker kernels on a second stream beside the
+    lattice away from the body, which batching cannot improve on for a
+    lattice that fills the GPU by itself.  Results are the same bit for bit."""
 
     def __init__(self, cfgs, storage: str = "f32", device=None, steps: int = 1000,
                  block: int = 256, group: int = 8, first_step: int = 1,
-                 cache_bytes: int = 192 << 20, deviation: str = "weights"):
+                 cache_bytes: int = 192 << 20, deviation: str = "weights",
+                 solo_min_cells: int = DroneMovingIBM.OVERLAP_MIN_CELLS):
         import torch
         self.cfgs = list(cfgs)
         self.device = torch.device(device if device is not None else "cuda")
@@ -137,7 +148,14 @@ class BatchSweep:
                      for c in self.cfgs]
         self.steps_done = 0
         self.state = 0
-        self.groups = [_Group(g) for g in cache_groups(self.cfgs, storage, cache_bytes)]
+        cells = [c.Nx * c.Ny * c.Nz for c in self.cfgs]
+        self.groups = [_Group(g, solo=len(g) == 1 and cells[g[0]] >= solo_min_cells)
+                       for g in cache_groups(self.cfgs, storage, cache_bytes)]
+        self._solo = set()
+        for gr in self.groups:
+            if gr.solo:
+                self._solo.add(gr.idx[0])
+                self.ibms[gr.idx[0]].overlap = True
         self.build(block, group)
 
     # -- tables ---------------------------------------------------------------
@@ -151,7 +169,8 @@ class BatchSweep:
                 raise ValueError("instances of a batch share the spread mode, clamp and quirk")
         torch.cuda.synchronize(self.device)
         for gr in self.groups:
-            self._build_group(gr)
+            if not gr.solo:
+                self._build_group(gr)
 
     def _build_group(self, gr: "_Group") -> None:
         import torch
@@ -211,15 +230,28 @@ class BatchSweep:
         _lib.call("clbm_batch_step", C.byref(gr.lb), state, stream)
         return self._next_state(state)
 
+    def _solo_steps(self, gr: "_Group", k: int) -> None:
+        """k steps of a solo group's instance through its own objects (on the
+        current stream, with its marker kernels on their own)."""
+        i = gr.idx[0]
+        lbm, ibm, cfg = self.lbms[i], self.ibms[i], self.cfgs[i]
+        for j in range(k):
+            fx, fy, fz = ibm.compute_force(lbm.ux, lbm.uy,
+                                           *cfg.motion_state(self.first_step + self.steps_done + j))
+            lbm.step(fx, fy, fz)
+
     def _advance_objects(self, k: int) -> None:
-        """Keep the solver/IBM objects consistent with k batched steps."""
+        """Keep the solver/IBM objects consistent with k batched steps (solo
+        instances keep their own count)."""
         self.steps_done += k
-        for l in self.lbms:
-            l.state_word = self.state
-            l.steps += k
-            l._macro_valid = False
-        for i in self.ibms:
-            i.gen += k
+        for n, l in enumerate(self.lbms):
+            if n not in self._solo:
+                l.state_word = self.state
+                l.steps += k
+                l._macro_valid = False
+        for n, i in enumerate(self.ibms):
+            if n not in self._solo:
+                i.gen += k
 
     def step(self, k: int = 1) -> None:
         """k eager steps of the whole batch, group by group."""
@@ -227,13 +259,14 @@ class BatchSweep:
         if self.steps_done + k > self.capacity:
             raise ValueError("kinematics table exhausted; build the sweep with more steps")
         st = C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
-        end = self.state
         for gr in self.groups:
+            if gr.solo:
+                self._solo_steps(gr, k)
+                continue
             s = self.state
             for _ in range(k):
                 s = self._launch_step(gr, s, st)
-            end = s
-        self.state = end
+        self.state = self._state_after(k)
         self._advance_objects(k)
 
     def run(self, k: int, graph: bool = True) -> None:
@@ -250,8 +283,10 @@ class BatchSweep:
             self.step(1)
             k -= 1
         st = C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
-        end = self.state
         for gr in self.groups:
+            if gr.solo:
+                self._solo_steps(gr, k)
+                continue
             s, left = self.state, k
             if left >= 2 and gr.graph is not None and gr.graph_state != s:
                 s = self._launch_step(gr, s, st)   # realign with the captured pair
@@ -265,9 +300,14 @@ class BatchSweep:
             while left:
                 s = self._launch_step(gr, s, st)
                 left -= 1
-            end = s
-        self.state = end
+        self.state = self._state_after(k)
         self._advance_objects(k)
+
+    def _state_after(self, k: int) -> int:
+        s = self.state
+        for _ in range(k):
+            s = self._next_state(s)
+        return s
 
     def _capture(self, gr: "_Group", state: int) -> None:
         import torch
@@ -402,6 +442,7 @@ def main(argv=None) -> int:
     res = {"instances": len(cfgs), "cells": sweep.cells, "steps": a.steps, "storage": a.storage,
            "deviation": a.deviation,
            "block": sweep.block, "group": sweep.group, "cache_groups": len(sweep.groups),
+           "solo_groups": sum(g.solo for g in sweep.groups),
            "cache_mib": a.cache_mib, "tuning_ms_per_step": tune,
            "seconds": dt, "mlups": sweep.cells * a.steps / dt / 1e6,
            "finite": all(l.health()[1] for l in sweep.lbms)}

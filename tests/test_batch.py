@@ -122,6 +122,27 @@ def test_graph_replay_equals_eager(pkg, gpu, cache_bytes, ngroups):
 
 
 @pytest.mark.gpu
+def test_solo_group_steps_its_own_objects_and_equals_each_instance_alone(pkg, gpu):
+    """A lattice over solo_min_cells in a group of its own takes the split step
+    (marker kernels on their own stream); the batch still equals the
+    instances run one by one, bit for bit, across eager steps and replays."""
+    bm = load_sub("batch")
+    cfgs = _small(pkg)[:2] + [pkg.config.naca_case(96, 48, 4, 12.0, name="solo")]
+    sweep = bm.BatchSweep(cfgs, steps=32, cache_bytes=1, solo_min_cells=10_000)
+    assert [g.solo for g in sweep.groups] == [len(g.idx) == 1 and g.idx[0] == 2
+                                            for g in sweep.groups]
+    assert sum(g.solo for g in sweep.groups) == 1
+    sweep.run(11, graph=True)
+    assert sweep.lbms[2].last_kernel_cells < sweep.lbms[2].ncell     # the split step ran
+    sweep.step(2)
+    sweep.run(4, graph=True)
+    assert sweep.lbms[2].steps == 17 and sweep.lbms[0].steps == 17
+    assert sweep.lbms[2].state_word == sweep.lbms[0].state_word == sweep.state
+    seq, _ = bm.sequential(cfgs, 17)
+    _assert_same(list(zip(sweep.lbms, sweep.ibms)), seq)
+
+
+@pytest.mark.gpu
 def test_autotune_restores_the_initial_state(pkg, gpu, tmp_path):
     bm = load_sub("batch")
     cfgs = _small(pkg)[:2]
