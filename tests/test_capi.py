@@ -70,6 +70,16 @@ def test_state_after_step():
     (lambda lib: lib.call("ibm_update_markers", C.byref(lib.ClbmGrid(8, 4, 4, 0)), -1,
                           *([None] * 3), *([0.0] * 6), *([None] * 6), 1, None), "n < 0"),
     (lambda lib: lib.call("ibm_force_sum_f64", 5, None, None, None, None, None), "NULL"),
+    # one float32 population plane of 1.2 G cells exceeds the 4 GiB of 32-bit offsets
+    (lambda lib: lib.call("clbm_init_feq", C.byref(lib.ClbmGrid(1200, 1000, 1000, 0)),
+                          C.c_void_p(1), 1.0, 0.0, 0.0, 0.0, None), "4 GiB"),
+    (lambda lib: lib.call("ibm_step_region", C.byref(lib.ClbmGrid(8, 4, 4, 0)), None, None, None,
+                          3, 0, 3, 1.0, 1.0, 0.0, None), "NULL"),
+    (lambda lib: lib.call("ibm_step_region", C.byref(lib.ClbmGrid(8, 4, 4, 0)),
+                          C.byref(lib.ClbmParams(1.7, 1.0, 0.05, 0.0, 1, 0)), None, None, 0, 3,
+                          1, 1.0, 1.0, 0.0, (C.c_int32 * 6)()), "z range"),
+    (lambda lib: lib.call("clbm_stream_after", None, None, None), "NULL event"),
+    (lambda lib: lib.call("clbm_event_create", None), "NULL"),
 ])
 def test_argument_errors_are_reported(call, match):
     lib = load_sub("_lib")
