@@ -14,8 +14,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SOURCES = ["csrc/clbm_lbm.hip", "csrc/clbm_ibm.hip", "csrc/clbm_closure.hip"]
 LIB_NAME = "libclbm_amd.so"
 
-# Packed-FP32 instructions are kept out of the device code on purpose.
-DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+# Packed-FP32 instructions are kept out of the device code on purpose, and
+# float32 denormals are kept (the f16c decode scales by 2^-128, a denormal).
+DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+                "-fno-gpu-flush-denormals-to-zero"]
 
 
 def hipcc() -> str:

@@ -210,20 +210,43 @@ class Trainer:
 # training data from the method's own refined runs (host-side preprocessing)
 # ---------------------------------------------------------------------------
 def box_filter(f_fine: np.ndarray) -> np.ndarray:
-    """(2X, 2Y, 2Z, 19) -> (X, Y, Z, 19): mean over each 2x2x2 block."""
-    a = np.asarray(f_fine, dtype=np.float64)
+    """(2X, 2Y, 2Z, 19) -> (X, Y, Z, 19) float64: mean over each 2x2x2 block
+    (accumulated in float64 without a float64 copy of the input)."""
+    a = np.asarray(f_fine)
     X, Y, Z = a.shape[0] // 2, a.shape[1] // 2, a.shape[2] // 2
     a = a[:2 * X, :2 * Y, :2 * Z].reshape(X, 2, Y, 2, Z, 2, 19)
-    return a.mean(axis=(1, 3, 5))
+    return a.mean(axis=(1, 3, 5), dtype=np.float64)
 
 
-def _moments(f):
-    """Central moments about u = m / rho (no force) of f (..., 19), raw-moment order."""
-    rho = f.sum(-1)
-    u = (f * EX).sum(-1) / rho
-    v = (f * EY).sum(-1) / rho
-    w = (f * EZ).sum(-1) / rho
-    return rho, u, v, w
+# raw moments m_abc, a, b, c in {0, 1, 2}: one product with this matrix
+_E27 = [(a, b, c) for a in range(3) for b in range(3) for c in range(3)]
+_M27 = np.array([[EX[q] ** a * EY[q] ** b * EZ[q] ** c for q in range(19)] for (a, b, c) in _E27],
+                dtype=np.float64)
+
+
+def _central_moments(fT, u, v, w):
+    """Central moments sum_q f_q (ex-u)^p (ey-v)^q (ez-w)^r of populations fT
+    (19, n) about per-cell (u, v, w): (19, n) in PQR order, from the raw
+    moments by the binomial shift (on D3Q19 the raw moments with all three
+    exponents >= 1 vanish; the matrix above carries them as zero rows)."""
+    R = _M27 @ fT
+    col = {e: i for i, e in enumerate(_E27)}
+    pu, pv, pw = [None, -u, u * u], [None, -v, v * v], [None, -w, w * w]
+    out = np.empty((19, fT.shape[1]))
+    for i, (p, q, r) in enumerate(PQR):
+        acc = np.zeros(fT.shape[1])
+        for a in range(p + 1):
+            for b in range(q + 1):
+                for c in range(r + 1):
+                    if not _M27[col[(a, b, c)]].any():
+                        continue
+                    t = float(math.comb(p, a) * math.comb(q, b) * math.comb(r, c)) * R[col[(a, b, c)]]
+                    for powers, k in ((pu, p - a), (pv, q - b), (pw, r - c)):
+                        if k:
+                            t = t * powers[k]
+                    acc += t
+        out[i] = acc
+    return out
 
 
 def features_and_targets(f_now: np.ndarray, f_next: np.ndarray, mask=None):
@@ -233,35 +256,29 @@ def features_and_targets(f_now: np.ndarray, f_next: np.ndarray, mask=None):
     cells should lie away from the body."""
     f = np.asarray(f_now, dtype=np.float64)
     fn = np.asarray(f_next, dtype=np.float64)
-    # ideal post-collision populations: f*_q(x) = f_next_q(x + e_q)
-    fs = np.empty_like(fn)
-    for q in range(19):
-        fs[..., q] = np.roll(fn[..., q], shift=(-EX[q], -EY[q], -EZ[q]), axis=(0, 1, 2))
     if mask is None:
         mask = np.ones(f.shape[:3], dtype=bool)
-    f, fs = f[mask], fs[mask]
-    rho, u, v, w = _moments(f)
-    xi = [EX - u[:, None], EY - v[:, None], EZ - w[:, None]]
-    pre = np.empty((len(f), 19))
-    post = np.empty((len(f), 19))
-    for i, (p, q, r) in enumerate(PQR):
-        k = xi[0] ** p * xi[1] ** q * xi[2] ** r
-        pre[:, i] = (f * k).sum(-1) / rho
-        post[:, i] = (fs * k).sum(-1) / rho
-    X = pre - FEATURE_EQ
-    X[:, 0] = rho - 1.0
-    Dpre = pre - FEATURE_EQ
-    Dpost = post - FEATURE_EQ
-    T = np.zeros((len(f), 4))
-    W = np.zeros(len(f))
+    # populations as (19, n) rows over the listed cells; the ideal
+    # post-collision populations are f*_q(x) = f_next_q(x + e_q)
+    fT = np.stack([f[..., q][mask] for q in range(19)])
+    fsT = np.stack([np.roll(fn[..., q], shift=(-EX[q], -EY[q], -EZ[q]), axis=(0, 1, 2))[mask]
+                    for q in range(19)])
+    rho = fT.sum(0)
+    u, v, w = (EX @ fT) / rho, (EY @ fT) / rho, (EZ @ fT) / rho
+    Dpre = _central_moments(fT, u, v, w) / rho - FEATURE_EQ[:, None]
+    Dpost = _central_moments(fsT, u, v, w) / rho - FEATURE_EQ[:, None]
+    X = Dpre.copy()
+    X[0] = rho - 1.0
+    T = np.zeros((4, len(rho)))
+    W = np.zeros(len(rho))
     for j, idx in enumerate(GROUPS):
-        num = (Dpre[:, idx] * Dpost[:, idx]).sum(1)
-        den = (Dpre[:, idx] ** 2).sum(1)
+        num = (Dpre[idx] * Dpost[idx]).sum(0)
+        den = (Dpre[idx] ** 2).sum(0)
         s = 1.0 - num / np.maximum(den, 1e-300)
-        T[:, j] = np.clip(s, 0.0, 2.0) - 1.0
+        T[j] = np.clip(s, 0.0, 2.0) - 1.0
         W += den
-    ok = np.all(np.isfinite(X), 1) & np.all(np.isfinite(T), 1) & np.isfinite(W)
-    X, T, W = X[ok], T[ok], W[ok]
+    ok = np.all(np.isfinite(X), 0) & np.all(np.isfinite(T), 0) & np.isfinite(W)
+    X, T, W = np.ascontiguousarray(X[:, ok].T), np.ascontiguousarray(T[:, ok].T), W[ok]
     return X, T, W / max(W.mean(), 1e-300) if len(W) else W
 
 
@@ -369,11 +386,25 @@ def run_forces(cfg, steps: int, net=None, storage: str = "f32", device=None, ski
 
 def study(coarse=(300, 100, 24), chord=None, Re=1000.0, train_seeds=range(8), test_seeds=range(100, 104),
           warm=4000, pairs=3, gap=400, samples=600000, epochs=40, eval_steps=4000, device=None,
-          log=None):
-    """The full pipeline at a given scale; returns a dict of results."""
+          log=None, net=None, apriori=True, aposteriori=True):
+    """The full pipeline at a given scale; returns (dict of results, network).
+    With ``net`` given the training stage is skipped; ``apriori`` and
+    ``aposteriori`` switch the two evaluation stages."""
     chord = coarse[1] / 2.5 if chord is None else chord
     log = log or (lambda s: print(f"[closure] {s}", file=sys.stderr, flush=True))
     res = {"coarse": list(coarse), "fine": [2 * v for v in coarse], "chord": chord, "Re": Re}
+    if net is None:
+        net = _train_stage(res, coarse, chord, Re, train_seeds, warm, pairs, gap, samples, epochs,
+                           device, log)
+    if apriori:
+        _apriori_stage(res, net, coarse, chord, Re, test_seeds, warm, gap, device, log)
+    if aposteriori:
+        _aposteriori_stage(res, net, coarse, chord, Re, test_seeds, eval_steps, device, log)
+    return res, net
+
+
+def _train_stage(res, coarse, chord, Re, train_seeds, warm, pairs, gap, samples, epochs, device,
+                 log):
     Xs, Ts, Ws = [], [], []
     rng = np.random.default_rng(0)
     for s in train_seeds:
@@ -394,7 +425,10 @@ def study(coarse=(300, 100, 24), chord=None, Re=1000.0, train_seeds=range(8), te
     hist = tr.fit(epochs=epochs, batch=16384, verbose=False)
     res["loss_first_last"] = [float(hist[0]), float(hist[-1])]
     res["loss_plain"] = float(np.average((T ** 2).sum(1), weights=W))
-    net = tr.export()
+    return tr.export()
+
+
+def _apriori_stage(res, net, coarse, chord, Re, test_seeds, warm, gap, device, log):
     # a-priori on held-out kinematics: the rate loss of the targets and one
     # coarse step from filtered states
     errs = {"plain": [], "closure": []}
@@ -412,6 +446,9 @@ def study(coarse=(300, 100, 24), chord=None, Re=1000.0, train_seeds=range(8), te
     res["one_step_rel_error"] = {k: float(np.mean(v)) for k, v in errs.items()}
     res["heldout_rate_loss"] = {k: float(np.mean(v)) for k, v in rate_loss.items()}
     log(f"one-step error: {res['one_step_rel_error']}")
+
+
+def _aposteriori_stage(res, net, coarse, chord, Re, test_seeds, eval_steps, device, log):
     # a-posteriori: coarse with / without the network against the fine run
     post = []
     for s in test_seeds:
@@ -433,7 +470,6 @@ def study(coarse=(300, 100, 24), chord=None, Re=1000.0, train_seeds=range(8), te
                                     for k in ("plain", "closure")}
     res["wall_time_ratio_fine_over_closure"] = float(
         np.mean([r["fine"][2] / r["closure"][2] for r in post]))
-    return res, net
 
 
 def main(argv=None) -> int:
@@ -450,11 +486,20 @@ def main(argv=None) -> int:
     ap.add_argument("--eval-steps", type=int, default=4000)
     ap.add_argument("--json", default=None)
     ap.add_argument("--save", default=None, help="write the trained network (.npz)")
+    ap.add_argument("--load", default=None, help="use this network (.npz) instead of training one")
+    ap.add_argument("--stage", default="all", choices=["all", "train", "eval"],
+                    help="train: data, training and the a-priori check; eval: the a-posteriori "
+                         "runs only (needs --load)")
     a = ap.parse_args(argv)
     coarse = tuple(int(v) for v in a.coarse.split(","))
+    if a.stage == "eval" and not a.load:
+        ap.error("--stage eval needs --load")
+    net = ClosureNet.load(a.load) if a.load else None
     res, net = study(coarse, Re=a.re, train_seeds=range(a.train_seeds),
                      test_seeds=range(100, 100 + a.test_seeds), warm=a.warm, pairs=a.pairs,
-                     gap=a.gap, epochs=a.epochs, eval_steps=a.eval_steps)
+                     gap=a.gap, epochs=a.epochs, eval_steps=a.eval_steps, net=net,
+                     apriori=a.stage != "eval", aposteriori=a.stage != "train")
+    res["stage"] = a.stage
     if a.save:
         net.save(a.save, coarse=list(coarse), Re=a.re)
     text = json.dumps(res, indent=1)
