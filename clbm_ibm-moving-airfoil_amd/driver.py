@@ -45,14 +45,16 @@ def grid_meta(cfg) -> np.ndarray:
 
 
 def run(cfg, steps=None, out_dir=None, storage="f32", write_frames=True, record_forces=True,
-        verbose=True, spread_mode="fixed"):
-    """Run the coupled loop; returns a dict with the force history and timing."""
+        verbose=True, spread_mode="fixed", deviation="weights"):
+    """Run the coupled loop; returns a dict with the force history and timing.
+    ``forces.npz`` is written whenever the forces are recorded, frames and
+    ``meta.npz`` only with ``write_frames``."""
     import torch
     steps = cfg.N_steps if steps is None else steps
     out_dir = cfg.DATA_DIR if out_dir is None else out_dir
-    if write_frames:
+    if write_frames or record_forces:
         os.makedirs(out_dir, exist_ok=True)
-    lbm = CLBMSolver(cfg, storage=storage)
+    lbm = CLBMSolver(cfg, storage=storage, deviation=deviation)
     ibm = DroneMovingIBM(cfg, spread_mode=spread_mode)
     snaps = dict(xc=[], yc=[], theta=[], step=[])
     frame = 0
@@ -100,8 +102,8 @@ def run(cfg, steps=None, out_dir=None, storage="f32", write_frames=True, record_
                 X_htail_tmpl=e.get("X_htail_tmpl", np.zeros(0, np.float32)),
                 Y_htail_tmpl=e.get("Y_htail_tmpl", np.zeros(0, np.float32)),
                 grid_meta=grid_meta(cfg))
-        if record_forces:
-            np.savez_compressed(os.path.join(out_dir, "forces.npz"), sums=hist, cd=cd, cl=cl)
+    if record_forces:
+        np.savez_compressed(os.path.join(out_dir, "forces.npz"), sums=hist, cd=cd, cl=cl)
     ncell = cfg.Nx * cfg.Ny * cfg.Nz
     res = dict(steps=step, elapsed=elapsed, mlups=step * ncell / elapsed / 1e6, history=hist,
                cd=cd, cl=cl, diverged=diverged)
@@ -116,11 +118,14 @@ def main(argv=None) -> int:
     ap.add_argument("--case", default="c0", choices=sorted(CASES))
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--out", default=None)
-    ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16", "f64"])
+    ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16", "f16c", "f64"])
+    ap.add_argument("--deviation", default="weights", choices=["weights", "inflow"],
+                    help="deviation base of 16-bit storage")
     ap.add_argument("--no-frames", action="store_true")
     a = ap.parse_args(argv)
     cfg = CASES[a.case]()
-    res = run(cfg, a.steps, a.out, a.storage, write_frames=not a.no_frames)
+    res = run(cfg, a.steps, a.out, a.storage, write_frames=not a.no_frames,
+              deviation=a.deviation)
     return 1 if res["diverged"] else 0
 
 
