@@ -132,7 +132,8 @@ class ClosureNet:
 
 
 class Trainer:
-    """Adam on the weighted mean squared rate error, all on the device."""
+    """Adam on the weighted mean squared rate error, all on the device.  W is
+    one weight per sample (n,) or one per output (n, 4)."""
 
     def __init__(self, X, T, W, device=None, blocks: int = 256, lr: float = 3e-3,
                  params=None, seed: int = 0):
@@ -146,7 +147,11 @@ class Trainer:
         dev = self.device
         self.X = torch.from_numpy(np.ascontiguousarray(Xs, dtype=np.float32)).to(dev)
         self.T = torch.from_numpy(np.ascontiguousarray(T, dtype=np.float32)).to(dev)
-        self.W = torch.from_numpy(np.ascontiguousarray(W, dtype=np.float32)).to(dev)
+        W = np.asarray(W, dtype=np.float32)
+        if W.shape not in ((X.shape[0],), (X.shape[0], 4)):
+            raise ValueError("W must have shape (n,) or (n, 4)")
+        self.wdim = 1 if W.ndim == 1 else 4
+        self.W = torch.from_numpy(np.ascontiguousarray(W)).to(dev)
         self.wsum = float(np.sum(W))
         self.net = ClosureNet(init_params(seed) if params is None else params, device=dev)
         self.m = torch.zeros(NP, dtype=torch.float32, device=dev)
@@ -167,7 +172,7 @@ class Trainer:
         _lib.call("closure_grad", C.c_void_p(self.net.packed.data_ptr()),
                   C.c_void_p(self.net.params.data_ptr()), n,
                   C.c_void_p(self.X[lo:hi].data_ptr()), C.c_void_p(self.T[lo:hi].data_ptr()),
-                  C.c_void_p(self.W[lo:hi].data_ptr()), C.c_float(wsum),
+                  C.c_void_p(self.W[lo:hi].data_ptr()), self.wdim, C.c_float(wsum),
                   C.c_void_p(self.partial.data_ptr()), self.blocks,
                   C.c_void_p(self.grad.data_ptr()), self.net._st())
         return self.grad
@@ -249,11 +254,18 @@ def _central_moments(fT, u, v, w):
     return out
 
 
-def features_and_targets(f_now: np.ndarray, f_next: np.ndarray, mask=None):
-    """Features (n, 19), target outputs y = s - 1 (n, 4) and weights (n,) from
-    two consecutive filtered states on the coarse lattice (x, y, z periodic
+def features_and_targets(f_now: np.ndarray, f_next: np.ndarray, mask=None,
+                         per_output: bool = False):
+    """Features (n, 19), target outputs y = s - 1 (n, 4) and weights from two
+    consecutive filtered states on the coarse lattice (x, y, z periodic
     neighbours; cells listed by ``mask``).  Force is taken as zero, so the
-    cells should lie away from the body."""
+    cells should lie away from the body.
+
+    A rate's target is the least-squares fit over its group's moments, whose
+    squared error grows with den_j = sum of the group's squared pre-collision
+    non-equilibrium moments times (s_j - s*_j)^2.  The weights are those den_j
+    normalised to a mean total of 1: per output (n, 4) with ``per_output``,
+    else their sum per cell (n,)."""
     f = np.asarray(f_now, dtype=np.float64)
     fn = np.asarray(f_next, dtype=np.float64)
     if mask is None:
@@ -270,16 +282,18 @@ def features_and_targets(f_now: np.ndarray, f_next: np.ndarray, mask=None):
     X = Dpre.copy()
     X[0] = rho - 1.0
     T = np.zeros((4, len(rho)))
-    W = np.zeros(len(rho))
+    D = np.zeros((4, len(rho)))
     for j, idx in enumerate(GROUPS):
         num = (Dpre[idx] * Dpost[idx]).sum(0)
-        den = (Dpre[idx] ** 2).sum(0)
-        s = 1.0 - num / np.maximum(den, 1e-300)
+        D[j] = (Dpre[idx] ** 2).sum(0)
+        s = 1.0 - num / np.maximum(D[j], 1e-300)
         T[j] = np.clip(s, 0.0, 2.0) - 1.0
-        W += den
+    W = D.sum(0)
     ok = np.all(np.isfinite(X), 0) & np.all(np.isfinite(T), 0) & np.isfinite(W)
-    X, T, W = np.ascontiguousarray(X[:, ok].T), np.ascontiguousarray(T[:, ok].T), W[ok]
-    return X, T, W / max(W.mean(), 1e-300) if len(W) else W
+    X, T = np.ascontiguousarray(X[:, ok].T), np.ascontiguousarray(T[:, ok].T)
+    W, D = W[ok], np.ascontiguousarray(D[:, ok].T)
+    scale = max(W.mean(), 1e-300) if len(W) else 1.0
+    return X, T, (D if per_output else W) / scale
 
 
 def one_step_error(cfg, f_now, f_next, net=None, mask=None, device=None):
@@ -410,21 +424,24 @@ def _train_stage(res, coarse, chord, Re, train_seeds, warm, pairs, gap, samples,
     for s in train_seeds:
         pr, masks = fine_pairs(s, coarse, chord, Re, warm, pairs, gap, device=device, log=log)
         for (a, b), m in zip(pr, masks):
-            X, T, W = features_and_targets(a, b, m)
+            X, T, W = features_and_targets(a, b, m, per_output=True)
             Xs.append(X)
             Ts.append(T)
             Ws.append(W)
     X, T, W = np.concatenate(Xs), np.concatenate(Ts), np.concatenate(Ws)
-    if len(X) > samples:      # keep the cells with the most non-equilibrium signal
-        p = W / W.sum()
-        idx = rng.choice(len(X), samples, replace=False, p=p)
-        X, T, W = X[idx], T[idx], np.ones(samples)
+    if len(X) > samples:
+        # keep cells in proportion to their non-equilibrium signal; each
+        # output then keeps its share of its cell's weight
+        tot = W.sum(1)
+        idx = rng.choice(len(X), samples, replace=False, p=tot / tot.sum())
+        X, T, W = X[idx], T[idx], W[idx] / tot[idx, None]
     res["train_samples"] = int(len(X))
+    res["weights"] = "per output"
     log(f"training on {len(X)} samples")
     tr = Trainer(X, T, W, device=device, lr=3e-3)
     hist = tr.fit(epochs=epochs, batch=16384, verbose=False)
     res["loss_first_last"] = [float(hist[0]), float(hist[-1])]
-    res["loss_plain"] = float(np.average((T ** 2).sum(1), weights=W))
+    res["loss_plain"] = float((W * T ** 2).sum() / W.sum())
     return tr.export()
 
 
@@ -439,10 +456,10 @@ def _apriori_stage(res, net, coarse, chord, Re, test_seeds, warm, gap, device, l
         for (a, b), m in zip(pr, masks):
             errs["plain"].append(one_step_error(cfg, a, b, None, m, device))
             errs["closure"].append(one_step_error(cfg, a, b, net, m, device))
-            Xh, Th, Wh = features_and_targets(a, b, m)
+            Xh, Th, Wh = features_and_targets(a, b, m, per_output=True)
             Yh = net.infer(Xh).cpu().numpy().astype(np.float64)
-            rate_loss["plain"].append(float(np.average((Th ** 2).sum(1), weights=Wh)))
-            rate_loss["closure"].append(float(np.average(((Yh - Th) ** 2).sum(1), weights=Wh)))
+            rate_loss["plain"].append(float((Wh * Th ** 2).sum() / Wh.sum()))
+            rate_loss["closure"].append(float((Wh * (Yh - Th) ** 2).sum() / Wh.sum()))
     res["one_step_rel_error"] = {k: float(np.mean(v)) for k, v in errs.items()}
     res["heldout_rate_loss"] = {k: float(np.mean(v)) for k, v in rate_loss.items()}
     log(f"one-step error: {res['one_step_rel_error']}")

@@ -432,15 +432,16 @@ __global__ __launch_bounds__(CBLK) void k_closure_infer(ClosureArgs n, int count
 }
 
 // ---------------------------------------------------------------------------
-// training: partial gradients of L = sum_n w_n |y_n - t_n|^2 / wsum
+// training: partial gradients of L = sum_n w_n |y_n - t_n|^2 / wsum, or with
+// one weight per output (wdim 4) L = sum_nj w_nj (y_nj - t_nj)^2 / wsum
 // ---------------------------------------------------------------------------
 constexpr int GPART = CP + 1;  // gradient + loss per workgroup
 __global__ __launch_bounds__(64) void k_closure_grad(const bf16x8* __restrict__ Ag,
                                                      const float* __restrict__ params, int count,
                                                      const float* __restrict__ X,
                                                      const float* __restrict__ Tg,
-                                                     const float* __restrict__ Wg, float wsum,
-                                                     float* __restrict__ partial) {
+                                                     const float* __restrict__ Wg, int wdim,
+                                                     float wsum, float* __restrict__ partial) {
   __shared__ bf16x8 sA[P_COUNT * 64];
   __shared__ float sb[68];
   __shared__ __bf16 sx[64 * 32];
@@ -464,9 +465,12 @@ __global__ __launch_bounds__(64) void k_closure_grad(const bf16x8* __restrict__ 
   for (int base = blockIdx.x * 64; base < count; base += gridDim.x * 64) {
     const int smp = base + lane;
     const bool live = smp < count;
-    float t[4], wl = live ? Wg[smp] : 0.0f;
+    float t[4], wl[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) t[k] = live ? Tg[(size_t)smp * 4 + k] : 0.0f;
+    for (int k = 0; k < 4; ++k) {
+      t[k] = live ? Tg[(size_t)smp * 4 + k] : 0.0f;
+      wl[k] = live ? Wg[wdim == 4 ? (size_t)smp * 4 + k : (size_t)smp] : 0.0f;
+    }
 #pragma unroll
     for (int k = 0; k < 32; ++k) {
       const __bf16 v = (__bf16)(live && k < 19 ? X[(size_t)smp * 19 + k] : 0.0f);
@@ -480,10 +484,12 @@ __global__ __launch_bounds__(64) void k_closure_grad(const bf16x8* __restrict__ 
       const bf16x8 xb = *(const bf16x8*)&sx[col * 32 + 8 * g];
       Fwd o;
       forward_block(sA, sb, xb, lane, o);
-      float ti[4], wi;
+      float ti[4], wi[4];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) ti[k] = __shfl(t[k], col);
-      wi = __shfl(wl, col);
+      for (int k = 0; k < 4; ++k) {
+        ti[k] = __shfl(t[k], col);
+        wi[k] = __shfl(wl[k], col);
+      }
       bf16x8 bdy;
 #pragma unroll
       for (int k = 0; k < 8; ++k) bdy[k] = (__bf16)0.0f;
@@ -491,8 +497,8 @@ __global__ __launch_bounds__(64) void k_closure_grad(const bf16x8* __restrict__ 
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const float d = o.y[i] - ti[i];
-          const float dy = 2.0f * wi * d * inv;
-          loss += wi * d * d * inv;
+          const float dy = 2.0f * wi[i] * d * inv;
+          loss += wi[i] * d * d * inv;
           gb3[i] += dy;
           bdy[i] = (__bf16)dy;
           tDY[i * 64 + col] = (__bf16)dy;
@@ -696,15 +702,16 @@ int clbm_step_closure(const clbm_grid_t* g, const clbm_params_t* p, void* f, int
 size_t closure_partial_bytes(int blocks) { return blocks > 0 ? (size_t)blocks * GPART * 4 : 0; }
 
 int closure_grad(const void* packed, const float* params, int count, const float* X,
-                 const float* T, const float* W, float wsum, float* partial, int blocks,
+                 const float* T, const float* W, int wdim, float wsum, float* partial, int blocks,
                  float* grad, void* stream) {
   if (!packed || !params || !X || !T || !W || !partial || !grad)
     return fail(-1, "closure_grad: NULL buffer");
   if (count <= 0 || blocks <= 0) return fail(-1, "closure_grad: count and blocks must be > 0");
+  if (wdim != 1 && wdim != 4) return fail(-1, "closure_grad: wdim must be 1 or 4");
   if (!(wsum > 0.0f)) return fail(-1, "closure_grad: weight sum must be > 0");
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(k_closure_grad, dim3(blocks), dim3(64), 0, st, (const bf16x8*)packed, params,
-                     count, X, T, W, wsum, partial);
+                     count, X, T, W, wdim, wsum, partial);
   hipLaunchKernelGGL(k_closure_reduce, dim3((GPART + 255) / 256), dim3(256), 0, st, partial,
                      blocks, grad);
   return check_launch("closure_grad");

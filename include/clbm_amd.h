@@ -49,7 +49,7 @@
 extern "C" {
 #endif
 
-#define CLBM_ABI_VERSION 5
+#define CLBM_ABI_VERSION 6
 
 /* population storage formats */
 #define CLBM_STORE_F32 0  /* float32 populations, float32 arithmetic (the drop-in) */
@@ -367,11 +367,13 @@ int clbm_step_closure(const clbm_grid_t* g, const clbm_params_t* p, void* f, int
                       const void* packed, const float* params, float lo, float hi, void* stream);
 /* Bytes of the partial-gradient buffer of `blocks` workgroups. */
 size_t closure_partial_bytes(int blocks);
-/* grad[0..1827] = d/dparams of L = sum_n W[n] |network(X[n]) - T[n]|^2 / wsum,
- * grad[1828] = L, over count samples with `blocks` workgroups (fixed
- * summation order: the same inputs give the same bits). */
+/* grad[0..1827] = d/dparams of L = sum_n W[n] |network(X[n]) - T[n]|^2 / wsum
+ * (wdim 1: one weight per sample) or L = sum_nj W[n][j] (network(X[n])_j -
+ * T[n][j])^2 / wsum (wdim 4: one weight per output), grad[1828] = L, over
+ * count samples with `blocks` workgroups (fixed summation order: the same
+ * inputs give the same bits). */
 int closure_grad(const void* packed, const float* params, int count, const float* X,
-                 const float* T, const float* W, float wsum, float* partial, int blocks,
+                 const float* T, const float* W, int wdim, float wsum, float* partial, int blocks,
                  float* grad, void* stream);
 /* One Adam step (bias-corrected, step number t >= 1) on params with moments m, v. */
 int closure_adam(float* params, float* m, float* v, const float* grad, float lr, float beta1,

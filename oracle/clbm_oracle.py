@@ -611,12 +611,15 @@ def mlp_backward(params, cache, dY):
 
 
 def closure_loss(params, X, T, wgt):
-    """Weighted mean squared rate error sum_n w_n |Y_n - T_n|^2 / sum_n w_n and its gradient."""
+    """Weighted mean squared rate error and its gradient: with one weight per
+    sample (wgt (n,)) sum_n w_n |Y_n - T_n|^2 / sum_n w_n, with one per output
+    (wgt (n, 4)) sum_nj w_nj (Y_nj - T_nj)^2 / sum_nj w_nj."""
     Y, cache = mlp_forward(params, X)
     d = Y - T
+    w = wgt[:, None] if wgt.ndim == 1 else wgt
     W = wgt.sum()
-    loss = float((wgt[:, None] * d * d).sum() / W)
-    grads = mlp_backward(params, cache, 2.0 * wgt[:, None] * d / W)
+    loss = float((w * d * d).sum() / W)
+    grads = mlp_backward(params, cache, 2.0 * w * d / W)
     return loss, grads
 
 

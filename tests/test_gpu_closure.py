@@ -54,12 +54,13 @@ def test_infer_matches_bf16_forward(pkg, oracle, gpu):
     assert np.median(np.abs(Y - ref)) < 1e-3 * scale
 
 
-def test_gradient_matches_fp64_oracle_and_is_deterministic(pkg, oracle, gpu):
+@pytest.mark.parametrize("wshape", [(3000,), (3000, 4)])
+def test_gradient_matches_fp64_oracle_and_is_deterministic(pkg, oracle, gpu, wshape):
     cl = load_sub("closure")
     rng = np.random.default_rng(3)
     X = rng.standard_normal((3000, 19))
     T = 0.2 * rng.standard_normal((3000, 4))
-    W = rng.uniform(0.5, 1.5, 3000)
+    W = rng.uniform(0.5, 1.5, wshape)
     tr = cl.Trainer(X, T, W, blocks=16, params=_params(4))
     g1 = tr.gradient().cpu().numpy().astype(np.float64)
     g2 = tr.gradient().cpu().numpy().astype(np.float64)

@@ -97,14 +97,36 @@ def _fd_check(oracle, params, X, T, wgt, grads, eps=1e-6, n=40, rng=None):
     return worst
 
 
-def test_mlp_gradients_match_finite_differences(oracle):
+@pytest.mark.parametrize("wshape", [(50,), (50, 4)])
+def test_mlp_gradients_match_finite_differences(oracle, wshape):
     rng = np.random.default_rng(3)
     params = _params(rng)
     X = rng.standard_normal((50, 19))
     T = 0.1 * rng.standard_normal((50, 4))
-    wgt = rng.uniform(0.5, 2.0, 50)
+    wgt = rng.uniform(0.5, 2.0, wshape)
     _, grads = oracle.closure_loss(params, X, T, wgt)
     assert _fd_check(oracle, params, X, T, wgt, grads) < 1e-6
+
+
+def test_per_output_weights_reduce_to_per_sample_ones(oracle):
+    """Equal weights across the four outputs give the per-sample loss, and a
+    zero weight removes that output's error from the loss entirely."""
+    rng = np.random.default_rng(6)
+    params = _params(rng)
+    X = rng.standard_normal((40, 19))
+    T = 0.1 * rng.standard_normal((40, 4))
+    w = rng.uniform(0.5, 2.0, 40)
+    l1, g1 = oracle.closure_loss(params, X, T, w)
+    l4, g4 = oracle.closure_loss(params, X, T, np.repeat(w[:, None], 4, 1) / 4)
+    assert abs(l1 - 4 * l4) < 1e-12 * l1
+    for a, b in zip(g1, g4):
+        np.testing.assert_allclose(a, 4 * b, rtol=1e-12, atol=1e-15)
+    w4 = np.ones((40, 4))
+    w4[:, 2] = 0.0
+    T2 = T.copy()
+    T2[:, 2] += 5.0                      # an output with zero weight cannot matter
+    assert oracle.closure_loss(params, X, T, w4)[0] == pytest.approx(
+        oracle.closure_loss(params, X, T2, w4)[0], rel=1e-14)
 
 
 def test_planted_gelu_derivative_is_caught(oracle, monkeypatch):
