@@ -27,6 +27,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import weakref
 
 import numpy as np
 
@@ -167,7 +168,8 @@ class DroneMovingIBM:
         self._side_h = None
         self._ev = []
         # ctypes arguments of the marker and clamp calls that stay fixed for a
-        # velocity source: (source key, before-state, after-state, clamp prefix)
+        # velocity source: (weak reference to the solver, before-state,
+        # after-state, clamp prefix)
         self._argcache = None
         self._hist = None
         self._hist_n = 0
@@ -218,7 +220,7 @@ class DroneMovingIBM:
         owner = getattr(ux, "owner", None)
         solver = owner if (isinstance(owner, CLBMSolver) and getattr(uy, "owner", None) is owner) \
             else None
-        if solver is not None and self._argcache is not None and self._argcache[0] is solver:
+        if solver is not None and self._argcache is not None and self._argcache[0]() is solver:
             return (*self._argcache[1:], solver, solver.state_word)
         grid, lp, f, state, uxp, uyp = self._velocity_source(ux, uy)
         P = C.c_void_p
@@ -234,7 +236,8 @@ class DroneMovingIBM:
                  acc0, acc1, boxp))
         clip = (gref, ipref, acc0, acc1, P(self.fx._t.data_ptr()), P(self.fy._t.data_ptr()), boxp)
         if solver is not None:
-            self._argcache = (solver, pre, post, clip)
+            # a weak reference: the cache must not keep a discarded solver's buffers alive
+            self._argcache = (weakref.ref(solver), pre, post, clip)
         return pre, post, clip, solver, state
 
     # -- the concurrent stream ------------------------------------------------
