@@ -332,7 +332,10 @@ template <int S> struct Pop {
     return Store<S>::ld(base_elem<T>(base[plane], cell), dv[q]);
   }
   __device__ __forceinline__ void st(int plane, int cell, int q, R v) const {
-    Store<S>::st(base_elem<T>(base[plane], cell), v, dv[q]);
+    uint32_t off = (uint32_t)cell * (uint32_t)sizeof(T);
+    asm("" : "+v"(off));
+    using GT = __attribute__((address_space(1))) char;
+    Store<S>::st((T*)(reinterpret_cast<GT*>(base[plane]) + off), v, dv[q]);
   }
 };
 

@@ -132,8 +132,7 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, int t) {
   // itself the compiler forms full 64-bit addresses once to share them
   // between the loads and the stores (and the outlet cell's stores), at a
   // vector add per population instead of the scalar-base form.
-  int cs = c, c1 = c + 1;
-  asm("" : "+v"(cs));
+  int c1 = c + 1;
   asm("" : "+v"(c1));
   bool twin = true;
   if (outlet) {
@@ -143,7 +142,7 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, int t) {
   }
   if (pl.on && x == 0) inlet_rule<R>(fq, R(pl.ux_in), R(pl.uy_in));
   collide_cell<R, FORCE>(fq, Fx, Fy, Fz, R(a.s_v), R(a.s_e));
-  pop_store<PAR>(pop, cs, nb, fq);
+  pop_store<PAR>(pop, c, nb, fq);
   if (outlet) {
     const Nbr nb1 = make_nbr(g, x + 1, y, z);
     if (!twin) {
