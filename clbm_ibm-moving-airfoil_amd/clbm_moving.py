@@ -87,6 +87,10 @@ class PopulationField:
             torch.cuda.current_stream(s.device).synchronize()
 
 
+# inlet argument -> clbm_params_t.inlet (CLBM_INLET_*)
+_INLETS = {False: 0, True: 1, "hecht-harting": 2}
+
+
 class CLBMSolver:
     """D3Q19 cascaded LBM with Guo forcing on an Nx x Ny x Nz lattice (HIP kernels).
 
@@ -95,7 +99,10 @@ class CLBMSolver:
     ['config_moving']`` or the reference airplane case), ``storage`` one of
     f32 (drop-in), f16, bf16, f16c (16-bit deviation storage) or f64 (error
     reference),
-    ``inlet`` False for an x-periodic box, ``outside_lds`` the LDS bytes per
+    ``inlet`` True for the reference's inlet rule at x = 0 and outlet copy at
+    x = Nx-1, "hecht-harting" for the same outlet with the on-site velocity
+    inlet of Hecht and Harting (J. Stat. Mech. 2010, P01018; not in the
+    reference, opt-in), False for an x-periodic box, ``outside_lds`` the LDS bytes per
     workgroup the outside part of a split step reserves (a tuning choice that
     leaves each CU room for a concurrent marker workgroup; by default 33 KiB,
     4 workgroups per CU, for 4- and 8-byte populations, which are bound by
@@ -105,7 +112,7 @@ class CLBMSolver:
     f_q - feq_q(1, inlet velocity), which is near zero wherever the flow is
     near the free stream."""
 
-    def __init__(self, config=None, storage: str = "f32", device=None, inlet: bool = True,
+    def __init__(self, config=None, storage: str = "f32", device=None, inlet=True,
                  outside_lds=None, deviation: str = "weights"):
         import torch
         cfg: CaseConfig = resolve(config)
@@ -119,12 +126,15 @@ class CLBMSolver:
         if deviation not in ("weights", "inflow"):
             raise ValueError("deviation must be 'weights' or 'inflow'")
         self.deviation = deviation
+        if inlet not in _INLETS:
+            raise ValueError("inlet must be True, False or 'hecht-harting'")
+        self.inlet = inlet
         du, dv = (float(cfg.ux_in), float(cfg.uy_in)) if deviation == "inflow" else (0.0, 0.0)
         self.grid = _lib.ClbmGrid(self.Nx, self.Ny, self.Nz, _lib.STORAGE[storage], du, dv)
         if outside_lds is None:
             outside_lds = 27648 if _lib.ELEM_BYTES[self.grid.storage] == 2 else 33792
         self.params = _lib.ClbmParams(float(cfg.s_v), float(cfg.s_e), float(cfg.ux_in),
-                                      float(cfg.uy_in), 1 if inlet else 0, int(outside_lds))
+                                      float(cfg.uy_in), _INLETS[inlet], int(outside_lds))
         nbytes = Q * self.Nx * self.Ny * self.Nz * _lib.ELEM_BYTES[self.grid.storage]
         self.state = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
         self.state_word = 0

@@ -380,7 +380,7 @@ __global__ __launch_bounds__(CBLK) void k_step_closure(StepArgs a, ClosureArgs n
       twin = (Gx == Fx) & (Gy == Fy) & (Gz == Fz);
       if (!twin) pop_store_in<PAR>(pop, c + 1, make_nbr(g, x + 1, y, z), fq);
     }
-    if (pl.on && x == 0) inlet_rule<R>(fq, R(pl.ux_in), R(pl.uy_in));
+    if (pl.on && x == 0) inlet_rule<R>(fq, R(pl.ux_in), R(pl.uy_in), pl.rule);
     float ff[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) ff[q] = float(fq[q]);
@@ -677,7 +677,7 @@ int clbm_step_closure(const clbm_grid_t* g, const clbm_params_t* p, void* f, int
   if (!(lo <= hi)) return fail(-1, "clbm_step_closure: need lo <= hi");
   Planes pl;
   if (int e = check_state(g, p, state, pl)) return e;
-  if (p->inlet && !(p->ux_in < 1.0)) return fail(-1, "clbm_step_closure: inlet needs ux_in < 1");
+  if (int e = check_inlet(p, "clbm_step_closure")) return e;
   StepArgs a{};
   a.g = make_geo(*g);
   a.s_v = p->s_v;

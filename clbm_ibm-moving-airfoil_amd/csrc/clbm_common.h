@@ -376,31 +376,50 @@ __device__ __forceinline__ void pop_store(const P& pop, int c, const Nbr& nb, co
 
 // The reference's inlet rule at x = 0 (CLBMSolver._bc_inlet, clbm_moving.py:
 // 266-294): the five east-going populations from the known ones, in place.
-// The empty asm reads the five populations the rule replaces, so that their
-// loads stay with the others above the inlet branch: the compiler would
-// otherwise move them into the other path, where they issue only after the
-// first loads have been waited for.
+// rule 1 replaces it with the on-site velocity boundary of Hecht and Harting
+// (J. Stat. Mech. 2010, P01018; no counterpart in the reference): the
+// normal population by bounce-back of its non-equilibrium part and each
+// diagonal from its opposite, the equilibrium difference rho (ux +- u_t) / 6
+// and a transverse correction that makes the cell's momentum exactly
+// rho_in (ux_in, uy_in, 0).  The empty asm reads the five populations the
+// rule replaces, so that their loads stay with the others above the inlet
+// branch: the compiler would otherwise move them into the other path, where
+// they issue only after the first loads have been waited for.
 template <typename R>
-__device__ __forceinline__ void inlet_rule(R* a, R ux_in, R uy_in) {
+__device__ __forceinline__ void inlet_rule(R* a, R ux_in, R uy_in, int rule) {
   asm("" : "+v"(a[1]), "+v"(a[7]), "+v"(a[9]), "+v"(a[11]), "+v"(a[13]));
   const R s_known = a[0] + a[3] + a[4] + a[5] + a[6] + a[15] + a[16] + a[17] + a[18];
   const R s_out = a[2] + a[8] + a[10] + a[12] + a[14];
   const R rho_in = (s_known + R(2) * s_out) / (R(1) - ux_in);
-  const R dy = a[3] - a[4], dz = a[5] - a[6];
-  const R e = (R(1) / R(6)) * rho_in * ux_in;
-  a[1] = a[2] + (R(2) / R(3)) * rho_in * ux_in;
-  a[7] = a[10] + e + R(0.5) * rho_in * uy_in - R(0.5) * dy;
-  a[9] = a[8] + e - R(0.5) * rho_in * uy_in + R(0.5) * dy;
-  a[11] = a[14] + e - R(0.5) * dz;
-  a[13] = a[12] + e + R(0.5) * dz;
+  if (rule == 0) {
+    const R dy = a[3] - a[4], dz = a[5] - a[6];
+    const R e = (R(1) / R(6)) * rho_in * ux_in;
+    a[1] = a[2] + (R(2) / R(3)) * rho_in * ux_in;
+    a[7] = a[10] + e + R(0.5) * rho_in * uy_in - R(0.5) * dy;
+    a[9] = a[8] + e - R(0.5) * rho_in * uy_in + R(0.5) * dy;
+    a[11] = a[14] + e - R(0.5) * dz;
+    a[13] = a[12] + e + R(0.5) * dz;
+  } else {
+    const R third = R(1) / R(3), sixth = R(1) / R(6);
+    const R n_y = R(0.5) * ((a[3] - a[4]) + (a[15] - a[16]) + (a[17] - a[18])) -
+                  third * rho_in * uy_in;
+    const R n_z = R(0.5) * ((a[5] - a[6]) + (a[15] + a[16]) - (a[17] + a[18]));
+    a[1] = a[2] + third * rho_in * ux_in;
+    a[7] = a[10] + sixth * rho_in * (ux_in + uy_in) - n_y;
+    a[9] = a[8] + sixth * rho_in * (ux_in - uy_in) + n_y;
+    a[11] = a[14] + sixth * rho_in * ux_in - n_z;
+    a[13] = a[12] + sixth * rho_in * ux_in + n_z;
+  }
 }
 
 // Boundary planes of a state: `on` when the x planes still hold the raw
 // streamed values, so that readers apply the inlet rule at x = 0 and take
 // cell nx-2's populations for x = nx-1 (CLBMSolver._bc_outlet,
-// clbm_moving.py:296-300) as they load.
+// clbm_moving.py:296-300) as they load; `rule` selects the inlet rule
+// (0 the reference's, 1 Hecht-Harting).
 struct Planes {
   int on;
+  int rule;
   double ux_in, uy_in;
 };
 
@@ -417,7 +436,7 @@ __device__ __forceinline__ void load_cell(const typename Store<S>::T* __restrict
   }
   const Nbr nb = make_nbr(g, x, y, z);
   load_state<S, PAR>(f, g, c, nb, a);
-  if (pl.on && x == 0) inlet_rule<R>(a, R(pl.ux_in), R(pl.uy_in));
+  if (pl.on && x == 0) inlet_rule<R>(a, R(pl.ux_in), R(pl.uy_in), pl.rule);
 }
 
 // ---------------------------------------------------------------------------
@@ -427,6 +446,7 @@ void set_error(const std::string& s);
 int fail(int code, const std::string& s);
 int check_launch(const char* what);
 int check_grid(const clbm_grid_t* g, bool need_inlet_width);
+int check_inlet(const clbm_params_t* p, const char* who);
 // Validates a state word and its boundary parameters; fills `pl`.
 int check_state(const clbm_grid_t* g, const clbm_params_t* p, int state, Planes& pl);
 

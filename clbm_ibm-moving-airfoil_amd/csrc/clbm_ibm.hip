@@ -911,6 +911,7 @@ static void launch_batch(const ibm_batch_t* b, int par, int planes, hipStream_t 
 static int velocity_source(const clbm_grid_t* g, const clbm_params_t* p, const void* f, int state,
                            const float* ux, const float* uy, Planes& pl, int& umode) {
   pl.on = 0;
+  pl.rule = 0;
   pl.ux_in = pl.uy_in = 0.0;
   if (ux && uy) {
     umode = 0;
@@ -1027,7 +1028,7 @@ int ibm_spread(const clbm_grid_t* g, const ibm_params_t* ip, int n, const float*
   if (n == 0) return 0;
   if (!X || !Y || !Z || !FX || !FY || !acc_x || !acc_y) return fail(-1, "ibm_spread: NULL buffer");
   const Geo geo = make_geo(*g);
-  const Planes pl{0, 0.0, 0.0};
+  const Planes pl{0, 0, 0.0, 0.0};
   hipStream_t st = (hipStream_t)stream;
   if (ip->spread_mode == IBM_SPREAD_FIXED)
     launch_marker<CLBM_STORE_F32, false, true, IBM_SPREAD_FIXED>(
@@ -1086,6 +1087,7 @@ int ibm_batch_build(int n, const clbm_grid_t* grids, const clbm_params_t* params
     const ibm_buffers_t& m = bufs[i];
     const clbm_params_t& p = params[i];
     if (int e = check_grid(&grids[i], p.inlet != 0)) return e;
+    if (int e = check_inlet(&p, "ibm_batch_build")) return e;
     if (grids[i].storage != grids[0].storage) return fail(-1, "ibm_batch_build: storage formats differ");
     if (!f[i] || !m.X0 || !m.Y0 || !m.Z0 || !m.X || !m.Y || !m.Z || !m.VX || !m.VY || !m.U || !m.V ||
         !m.FX || !m.FY || !m.acc_x || !m.acc_y || !m.box || !m.kin)
@@ -1093,6 +1095,7 @@ int ibm_batch_build(int n, const clbm_grid_t* grids, const clbm_params_t* params
     IbmArgs a{};
     a.g = make_geo(grids[i]);
     a.pl.on = p.inlet ? 1 : 0;
+    a.pl.rule = p.inlet == CLBM_INLET_HECHT_HARTING ? 1 : 0;
     a.pl.ux_in = p.ux_in;
     a.pl.uy_in = p.uy_in;
     a.quirk = ip->wrap_weight_quirk ? 1 : 0;

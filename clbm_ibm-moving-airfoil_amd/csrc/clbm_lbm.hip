@@ -43,14 +43,22 @@ int check_grid(const clbm_grid_t* g, bool need_inlet_width) {
     return fail(-1, "grid too large for this storage: one population plane would exceed 4 GiB");
   return 0;
 }
+int check_inlet(const clbm_params_t* p, const char* who) {
+  if (p->inlet < CLBM_INLET_NONE || p->inlet > CLBM_INLET_HECHT_HARTING)
+    return fail(-1, std::string(who) + ": inlet must be 0 (periodic x), 1 (the reference's rule) or 2 (Hecht-Harting)");
+  if (p->inlet && !(p->ux_in < 1.0)) return fail(-1, std::string(who) + ": inlet needs ux_in < 1");
+  return 0;
+}
 int check_state(const clbm_grid_t* g, const clbm_params_t* p, int state, Planes& pl) {
   if (state & ~(CLBM_STATE_PARITY | CLBM_STATE_PLANES))
     return fail(-1, "state has unknown bits (expected parity | planes flag)");
   pl.on = (state & CLBM_STATE_PLANES) ? 1 : 0;
+  pl.rule = 0;
   pl.ux_in = pl.uy_in = 0.0;
   if (pl.on) {
     if (!p) return fail(-1, "a state with pending boundary planes needs the params (inlet velocity)");
     if (g->nx < 3) return fail(-1, "the inlet/outlet rule needs nx >= 3");
+    pl.rule = p->inlet == CLBM_INLET_HECHT_HARTING ? 1 : 0;
     pl.ux_in = p->ux_in;
     pl.uy_in = p->uy_in;
   }
@@ -140,7 +148,7 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, int t) {
     twin = (Gx == Fx) & (Gy == Fy) & (Gz == Fz);
     if (!twin) pop_store_in<PAR>(pop, c1, make_nbr(g, x + 1, y, z), fq);
   }
-  if (pl.on && x == 0) inlet_rule<R>(fq, R(pl.ux_in), R(pl.uy_in));
+  if (pl.on && x == 0) inlet_rule<R>(fq, R(pl.ux_in), R(pl.uy_in), pl.rule);
   collide_cell<R, FORCE>(fq, Fx, Fy, Fz, R(a.s_v), R(a.s_e));
   pop_store<PAR>(pop, c, nb, fq);
   if (outlet) {
@@ -498,12 +506,13 @@ int clbm_batch_build(int n, const clbm_grid_t* grids, const clbm_params_t* param
     const clbm_params_t& p = params[i];
     if (int e = check_grid(&grids[i], p.inlet != 0)) return e;
     if (!f[i] || !fx[i] || !fy[i] || !boxes[i]) return fail(-1, "clbm_batch_build: NULL buffer");
-    if (p.inlet && !(p.ux_in < 1.0)) return fail(-1, "clbm_batch_build: inlet needs ux_in < 1");
+    if (int e = check_inlet(&p, "clbm_batch_build")) return e;
     StepArgs a{};
     a.g = make_geo(grids[i]);
     a.s_v = p.s_v;
     a.s_e = p.s_e;
     a.pl.on = p.inlet ? 1 : 0;   // the planes of a step are those of the state AND this flag
+    a.pl.rule = p.inlet == CLBM_INLET_HECHT_HARTING ? 1 : 0;
     a.pl.ux_in = p.ux_in;
     a.pl.uy_in = p.uy_in;
     a.f = f[i];
@@ -586,7 +595,7 @@ int clbm_step(const clbm_grid_t* g, const clbm_params_t* p, void* f, int state, 
   if (!f) return fail(-1, "clbm_step: f is NULL");
   Planes pl;
   if (int e = check_state(g, p, state, pl)) return e;
-  if (p->inlet && !(p->ux_in < 1.0)) return fail(-1, "clbm_step: inlet needs ux_in < 1");
+  if (int e = check_inlet(p, "clbm_step")) return e;
   const Region none{};
   return dispatch_storage<StepRun>(g->storage, make_geo(*g), p, pl, f, state & CLBM_STATE_PARITY,
                                    fx, fy, fz, force_box, none, -1, (hipStream_t)stream);
@@ -623,7 +632,7 @@ int clbm_step_part(const clbm_grid_t* g, const clbm_params_t* p, void* f, int st
     return fail(-1, "clbm_step_part: outside_lds must be in [0, 65536]");
   Planes pl;
   if (int e = check_state(g, p, state, pl)) return e;
-  if (p->inlet && !(p->ux_in < 1.0)) return fail(-1, "clbm_step_part: inlet needs ux_in < 1");
+  if (int e = check_inlet(p, "clbm_step_part")) return e;
   int32_t r[6];
   if (int e = clbm_step_region(g, p, region, r)) return e;
   for (int a = 0; a < 6; ++a)

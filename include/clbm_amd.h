@@ -49,7 +49,7 @@
 extern "C" {
 #endif
 
-#define CLBM_ABI_VERSION 6
+#define CLBM_ABI_VERSION 7
 
 /* population storage formats */
 #define CLBM_STORE_F32 0  /* float32 populations, float32 arithmetic (the drop-in) */
@@ -62,6 +62,11 @@ extern "C" {
 /* force spreading modes */
 #define IBM_SPREAD_FIXED 0 /* order-independent 64-bit fixed-point sums (deterministic) */
 #define IBM_SPREAD_ATOMIC_F32 1 /* float32 atomics, as ti.atomic_add (order-dependent) */
+
+/* inlet rules at x = 0 (clbm_params_t.inlet) */
+#define CLBM_INLET_NONE 0           /* x periodic, no outlet copy */
+#define CLBM_INLET_REFERENCE 1      /* CLBMSolver._bc_inlet (clbm_moving.py:266-294) */
+#define CLBM_INLET_HECHT_HARTING 2  /* Hecht-Harting on-site velocity inlet (opt-in) */
 
 #define CLBM_STATE_PARITY 1
 #define CLBM_STATE_PLANES 2
@@ -88,8 +93,13 @@ typedef struct {
   double s_e;          /* bulk rate (config_moving.py:41) */
   double ux_in, uy_in; /* inlet velocity _UX_IN/_UY_IN (clbm_moving.py:33-35); rates and
                           velocities are rounded to the arithmetic type of the storage format */
-  int32_t inlet;      /* 1: inlet rule at x=0 and outlet copy at x=Nx-1 after every
-                         stream (clbm_moving.py:266-300); 0: x stays periodic */
+  int32_t inlet;      /* CLBM_INLET_REFERENCE (1): inlet rule at x=0 and outlet copy at
+                         x=Nx-1 after every stream (clbm_moving.py:266-300);
+                         CLBM_INLET_HECHT_HARTING (2): the same outlet copy with the
+                         on-site velocity inlet of Hecht and Harting (J. Stat. Mech.
+                         2010, P01018), which imposes the cell's momentum exactly (no
+                         counterpart in the reference, opt-in);
+                         CLBM_INLET_NONE (0): x stays periodic */
   int32_t outside_lds; /* bytes of LDS each workgroup of clbm_step_part part 0 reserves
                           (unused by it): caps that launch's workgroups per CU so that
                           kernels on another stream find room beside it; 0 = none */

@@ -21,6 +21,9 @@ Pinning (what checks each function, in ``tests/test_oracle_*.py``):
                                                    central moments, rest fixed point, Guo moments
 * ``stream``                                    -- permutation / brute force on tiny grids
 * ``bc_inlet`` / ``bc_outlet``                  -- closed-form moment identities of the inlet rule
+* ``bc_inlet_hecht_harting``                    -- imposed density/momentum exactly, normal
+                                                   non-equilibrium bounce-back, planted drop of
+                                                   the transverse f15..f18 terms caught
 * ``phi1d``                                     -- Peskin identities (partition of unity, sum phi^2 = 3/8)
 * ``update_markers``                            -- closed form (rotation, rigid velocity)
 * ``interpolate`` / ``spread``                  -- adjointness, partition of unity, seam weight 0.75,
@@ -284,6 +287,36 @@ def bc_inlet(f, ux_in: float, uy_in: float):
     return f
 
 
+def bc_inlet_hecht_harting(f, ux_in: float, uy_in: float):
+    """The textbook alternative to the reference's inlet (no reference
+    counterpart; opt-in): the on-site velocity boundary of Hecht and Harting
+    (J. Stat. Mech. 2010, P01018) for D3Q19 at x = 0 with u = (ux_in, uy_in, 0),
+    in place.  rho from mass and x-momentum; the normal population by
+    bounce-back of its non-equilibrium part, f1 = f2 + rho ux / 3; each
+    diagonal from its opposite plus the equilibrium difference
+    rho (ux +- u_t) / 6 and a transverse correction N_t fixed by the
+    prescribed transverse momentum:
+    N_y = (f3 - f4 + f15 - f16 + f17 - f18) / 2 - rho uy / 3,
+    N_z = (f5 - f6 + f15 + f16 - f17 - f18) / 2 (uz = 0)."""
+    dt = f.dtype.type
+    ux_in = dt(ux_in)
+    uy_in = dt(uy_in)
+    g = f[0]
+    s_known = g[..., 0] + g[..., 3] + g[..., 4] + g[..., 5] + g[..., 6] + g[..., 15] + g[..., 16] + g[..., 17] + g[..., 18]
+    s_out = g[..., 2] + g[..., 8] + g[..., 10] + g[..., 12] + g[..., 14]
+    rho_in = (s_known + dt(2.0) * s_out) / (dt(1.0) - ux_in)
+    half, third, sixth = dt(0.5), dt(1.0) / dt(3.0), dt(1.0) / dt(6.0)
+    n_y = half * ((g[..., 3] - g[..., 4]) + (g[..., 15] - g[..., 16]) + (g[..., 17] - g[..., 18])) \
+        - third * rho_in * uy_in
+    n_z = half * ((g[..., 5] - g[..., 6]) + (g[..., 15] + g[..., 16]) - (g[..., 17] + g[..., 18]))
+    g[..., 1] = g[..., 2] + third * rho_in * ux_in
+    g[..., 7] = g[..., 10] + sixth * rho_in * (ux_in + uy_in) - n_y
+    g[..., 9] = g[..., 8] + sixth * rho_in * (ux_in - uy_in) + n_y
+    g[..., 11] = g[..., 14] + sixth * rho_in * ux_in - n_z
+    g[..., 13] = g[..., 12] + sixth * rho_in * ux_in + n_z
+    return f
+
+
 def bc_outlet(f):
     """K6 ``_bc_outlet`` (clbm_moving.py:296-300): f[Nx-1] = f[Nx-2], all q, in place."""
     f[-1] = f[-2]
@@ -291,16 +324,20 @@ def bc_outlet(f):
 
 
 def lbm_step(f, fx, fy, fz, s_v: float, s_e: float, ux_in: float, uy_in: float, rates=None,
-             inlet: bool = True):
+             inlet=True):
     """``CLBMSolver.step`` (clbm_moving.py:326-343): K2 -> K3 -> K4 -> K5 -> K6 -> K7.
     Returns (f_new, rho, ux, uy, uz) with the K7 (force-free) macroscopic fields.
     ``rates``: per-cell higher-order rates (see ``collide``); ``inlet`` False
-    leaves x periodic."""
+    leaves x periodic, True (or "reference") applies the reference's inlet
+    rule, "hecht-harting" the textbook one (``bc_inlet_hecht_harting``)."""
     rho, ux, uy, uz = macro_with_force(f, fx, fy, fz)
     f_post = collide(f, rho, ux, uy, uz, fx, fy, fz, s_v, s_e, rates=rates)
     f_new = stream(f_post)
     if inlet:
-        bc_inlet(f_new, ux_in, uy_in)
+        if inlet == "hecht-harting":
+            bc_inlet_hecht_harting(f_new, ux_in, uy_in)
+        else:
+            bc_inlet(f_new, ux_in, uy_in)
         bc_outlet(f_new)
     rho, ux, uy, uz = macro_post(f_new)
     return f_new, rho, ux, uy, uz

@@ -55,6 +55,13 @@ def test_state_after_step():
     assert so.clbm_state_after_step(0, C.byref(p)) == lib.STATE_PARITY | lib.STATE_PLANES
     assert so.clbm_state_after_step(lib.STATE_PARITY | lib.STATE_PLANES, C.byref(p)) == lib.STATE_PLANES
     assert so.clbm_state_after_step(lib.STATE_PARITY | lib.STATE_PLANES, C.byref(q)) == 0
+    hh = lib.ClbmParams(1.7, 1.0, 0.05, 0.0, 2, 0)      # Hecht-Harting inlet: same planes
+    assert so.clbm_state_after_step(0, C.byref(hh)) == lib.STATE_PARITY | lib.STATE_PLANES
+
+
+def test_solver_rejects_an_unknown_inlet(pkg):
+    with pytest.raises(ValueError, match="inlet"):
+        pkg.CLBMSolver(pkg.config.naca_case(16, 8, 4, 4.0), inlet="zou-he")
 
 
 @pytest.mark.parametrize("call,match", [
@@ -78,6 +85,13 @@ def test_state_after_step():
     (lambda lib: lib.call("ibm_step_region", C.byref(lib.ClbmGrid(8, 4, 4, 0)),
                           C.byref(lib.ClbmParams(1.7, 1.0, 0.05, 0.0, 1, 0)), None, None, 0, 3,
                           1, 1.0, 1.0, 0.0, (C.c_int32 * 6)()), "z range"),
+    # inlet rules: 0, 1 and 2 only, and an inlet needs ux_in < 1 (no launch happens)
+    (lambda lib: lib.call("clbm_step", C.byref(lib.ClbmGrid(8, 4, 4, 0)),
+                          C.byref(lib.ClbmParams(1.7, 1.0, 0.05, 0.0, 3, 0)), C.c_void_p(1), 0,
+                          None, None, None, None, None), "inlet must be"),
+    (lambda lib: lib.call("clbm_step", C.byref(lib.ClbmGrid(8, 4, 4, 0)),
+                          C.byref(lib.ClbmParams(1.7, 1.0, 1.0, 0.0, 2, 0)), C.c_void_p(1), 0,
+                          None, None, None, None, None), "ux_in < 1"),
     (lambda lib: lib.call("clbm_stream_after", None, None, None), "NULL event"),
     (lambda lib: lib.call("clbm_event_create", None), "NULL"),
 ])

@@ -75,6 +75,32 @@ def test_steps_with_force_match_oracle(pkg, oracle, gpu, storage, atol, shape):
 
 
 @pytest.mark.parametrize("storage,atol", [("f32", 2e-6), ("f64", 1e-12)])
+@pytest.mark.parametrize("shape", [(9, 7, 5), (16, 12, 8)])
+def test_hecht_harting_inlet_matches_oracle(pkg, oracle, gpu, storage, atol, shape):
+    """The opt-in Hecht-Harting inlet (no reference counterpart): 1..6 steps
+    against clbm_oracle.lbm_step(inlet="hecht-harting"), and the x = 0 cells
+    then carry exactly the momentum rho_in (ux_in, uy_in, 0) of their density."""
+    cfg = _case(pkg, *shape)
+    s = pkg.CLBMSolver(cfg, storage=storage, inlet="hecht-harting")
+    f = _rand_state(oracle, shape, 2)
+    F = _rand_force(shape, 2)
+    s.f.from_numpy(f)
+    tF = _set_force(s, F)
+    for n in range(6):
+        s.step(*tF)
+        f, *_ = oracle.lbm_step(f, *F, cfg.s_v, cfg.s_e, cfg.ux_in, cfg.uy_in,
+                                inlet="hecht-harting")
+        np.testing.assert_allclose(s.f.to_numpy(), f, rtol=0, atol=atol, err_msg=f"step {n + 1}")
+    g = s.f.to_numpy()[0].astype(np.float64)
+    e = np.stack([oracle.EX, oracle.EY, oracle.EZ], 1).astype(np.float64)
+    rho, m = g.sum(-1), g @ e
+    mtol = 1e-13 if storage == "f64" else 2e-6
+    np.testing.assert_allclose(m[..., 0], rho * cfg.ux_in, rtol=0, atol=mtol)
+    np.testing.assert_allclose(m[..., 1], rho * cfg.uy_in, rtol=0, atol=mtol)
+    np.testing.assert_allclose(m[..., 2], 0.0, rtol=0, atol=mtol)
+
+
+@pytest.mark.parametrize("storage,atol", [("f32", 2e-6), ("f64", 1e-12)])
 @pytest.mark.parametrize("dense", [True, False])
 def test_outlet_rows_with_equal_and_unequal_force(pkg, oracle, gpu, storage, atol, dense):
     """The x = Nx-1 column collides the outlet copy of x = Nx-2.  Rows whose

@@ -66,6 +66,63 @@ def test_inlet_rule_moment_identities(oracle, u_in, v_in):
     assert (f[1:] == _inlet_state(oracle, 1)[1:]).all()
 
 
+def _hh_check(oracle, f0, u_in, v_in):
+    """What the Hecht-Harting rule must impose at x = 0: the cell's density is
+    rho_in and its momentum exactly rho_in (u_in, v_in, 0); f1 - f1^eq =
+    f2 - f2^eq (bounce-back of the normal non-equilibrium part); known
+    populations untouched."""
+    f = f0.copy()
+    g0 = f0[0]
+    known = [0, 2, 3, 4, 5, 6, 8, 10, 12, 14, 15, 16, 17, 18]
+    s_known = g0[..., [0, 3, 4, 5, 6, 15, 16, 17, 18]].sum(-1)
+    s_out = g0[..., [2, 8, 10, 12, 14]].sum(-1)
+    rho_in = (s_known + 2 * s_out) / (1 - u_in)
+    oracle.bc_inlet_hecht_harting(f, u_in, v_in)
+    g = f[0]
+    np.testing.assert_array_equal(g[..., known], g0[..., known])
+    e = np.stack([oracle.EX, oracle.EY, oracle.EZ], 1).astype(float)
+    m = g @ e
+    np.testing.assert_allclose(g.sum(-1), rho_in, atol=1e-15)
+    np.testing.assert_allclose(m[..., 0], rho_in * u_in, atol=1e-15)
+    np.testing.assert_allclose(m[..., 1], rho_in * v_in, atol=1e-15)
+    np.testing.assert_allclose(m[..., 2], 0.0, atol=1e-15)
+    # equilibrium difference of the normal pair: w (6 e.u) rho with w = 1/18
+    np.testing.assert_allclose(g[..., 1] - g[..., 2], rho_in * u_in / 3, atol=1e-15)
+    assert (f[1:] == f0[1:]).all()
+    return f
+
+
+@pytest.mark.parametrize("u_in,v_in", [(0.05, 0.0), (0.0495, -0.00696), (0.1, 0.02)])
+def test_hecht_harting_inlet_imposes_the_velocity(oracle, u_in, v_in):
+    f = _hh_check(oracle, _inlet_state(oracle, 4), u_in, v_in)
+    g = f.copy()
+    oracle.bc_inlet_hecht_harting(g, u_in, v_in)             # idempotent
+    np.testing.assert_allclose(g, f, atol=1e-16)
+    # at rest with the equilibrium of the inlet velocity, the rule changes nothing
+    eq, *_ = oracle.init_feq(3, 4, 3, u_in, v_in)
+    h = eq.copy()
+    oracle.bc_inlet_hecht_harting(h, u_in, v_in)
+    np.testing.assert_allclose(h, eq, atol=1e-16)
+
+
+def test_planted_hecht_harting_mistake_is_caught(oracle, monkeypatch):
+    """Dropping the f15..f18 terms of the transverse correction (what the
+    reference's own rule does) breaks the imposed y momentum."""
+    orig = oracle.bc_inlet_hecht_harting
+
+    def planted(f, u_in, v_in):
+        g = f[0]
+        save = g[..., 15:19].copy()
+        g[..., 15:19] = 0.0
+        orig(f, u_in, v_in)
+        g[..., 15:19] = save
+        return f
+
+    monkeypatch.setattr(oracle, "bc_inlet_hecht_harting", planted)
+    with pytest.raises(AssertionError):
+        _hh_check(oracle, _inlet_state(oracle, 5), 0.05, 0.01)
+
+
 def test_inlet_rule_is_idempotent(oracle):
     f = _inlet_state(oracle, 2)
     oracle.bc_inlet(f, 0.05, 0.01)
