@@ -277,6 +277,19 @@ class CLBMSolver:
                   C.c_void_p(m[2].data_ptr()), C.c_void_p(m[3].data_ptr()), self._stream())
         self._macro_valid = True
 
+    def macro_into(self, out) -> None:
+        """K7 (clbm_moving.py:305-321) into a caller's float32 device tensor
+        [4, Nz, Ny, Nx] (rho, ux, uy, uz), queued on the current stream: a
+        snapshot that later steps do not overwrite."""
+        import torch
+        if (not isinstance(out, torch.Tensor) or out.dtype != torch.float32 or not out.is_cuda
+                or tuple(out.shape) != (4, self.Nz, self.Ny, self.Nx) or not out.is_contiguous()):
+            raise TypeError("out must be a contiguous float32 device tensor [4, Nz, Ny, Nx]")
+        _lib.call("clbm_macro", C.byref(self.grid), C.byref(self.params),
+                  C.c_void_p(self.state.data_ptr()), self.state_word,
+                  C.c_void_p(out[0].data_ptr()), C.c_void_p(out[1].data_ptr()),
+                  C.c_void_p(out[2].data_ptr()), C.c_void_p(out[3].data_ptr()), self._stream())
+
     def health(self):
         """(rho_min, all_finite) of the current state, computed on the device
         (the divergence guard of main_moving.py:143-145)."""

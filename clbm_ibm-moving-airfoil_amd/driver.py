@@ -18,6 +18,7 @@ import math
 import os
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -44,11 +45,74 @@ def grid_meta(cfg) -> np.ndarray:
         e.get("h_vtail", 0.0)], dtype=np.float64)
 
 
+FRAME_KEYS = ("rho", "ux", "uy", "uz")
+
+
+class FrameWriter:
+    """Frames leave the device without stopping the step loop.
+
+    The reference reads four fields to the host and compresses them before
+    its next step (main_moving.py:125-155).  Here a frame is the force-free
+    macroscopic fields written by the library into one of ``depth`` device
+    snapshot buffers on the solver's stream; a copy stream moves the snapshot
+    into a pinned host buffer, and a worker thread turns it into the
+    reference's (Nx, Ny, Nz) arrays and writes the .npz, while the GPU steps
+    on.  A slot is reused only after its file is written, so a writer that
+    falls behind holds the loop back instead of dropping frames."""
+
+    def __init__(self, lbm, depth: int = 3, compress: bool = True):
+        import torch
+        self.lbm = lbm
+        self.compress = compress
+        shape = (4, lbm.Nz, lbm.Ny, lbm.Nx)
+        self._dev = [torch.empty(shape, dtype=torch.float32, device=lbm.device)
+                     for _ in range(depth)]
+        self._host = [torch.empty(shape, dtype=torch.float32, pin_memory=True)
+                      for _ in range(depth)]
+        self._done = [None] * depth
+        self._copy = torch.cuda.Stream(device=lbm.device)
+        self._pool = ThreadPoolExecutor(max_workers=depth)
+        self._k = 0
+
+    def submit(self, path: str) -> None:
+        import torch
+        k = self._k % len(self._dev)
+        self._k += 1
+        if self._done[k] is not None:
+            self._done[k].result()          # the slot's previous frame is on disk
+        dev, host = self._dev[k], self._host[k]
+        self.lbm.macro_into(dev)
+        self._copy.wait_stream(torch.cuda.current_stream(self.lbm.device))
+        with torch.cuda.stream(self._copy):
+            host.copy_(dev, non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(self._copy)
+        self._done[k] = self._pool.submit(self._write, ev, host, path)
+
+    def _write(self, ev, host, path: str) -> None:
+        ev.synchronize()
+        a = host.numpy()
+        arrays = {key: np.ascontiguousarray(a[i].transpose(2, 1, 0))
+                  for i, key in enumerate(FRAME_KEYS)}
+        (np.savez_compressed if self.compress else np.savez)(
+            path, ux=arrays["ux"], uy=arrays["uy"], uz=arrays["uz"], rho=arrays["rho"])
+
+    def close(self) -> None:
+        for d in self._done:
+            if d is not None:
+                d.result()
+        self._pool.shutdown()
+
+
 def run(cfg, steps=None, out_dir=None, storage="f32", write_frames=True, record_forces=True,
-        verbose=True, spread_mode="fixed", deviation="weights"):
+        verbose=True, spread_mode="fixed", deviation="weights", async_frames=True,
+        compress=True):
     """Run the coupled loop; returns a dict with the force history and timing.
     ``forces.npz`` is written whenever the forces are recorded, frames and
-    ``meta.npz`` only with ``write_frames``."""
+    ``meta.npz`` only with ``write_frames``.  ``async_frames`` writes frames
+    through ``FrameWriter`` (False: read and write each frame before the next
+    step, as the reference does); ``compress`` False writes frames with
+    np.savez instead of np.savez_compressed (the same arrays for np.load)."""
     import torch
     steps = cfg.N_steps if steps is None else steps
     out_dir = cfg.DATA_DIR if out_dir is None else out_dir
@@ -56,6 +120,7 @@ def run(cfg, steps=None, out_dir=None, storage="f32", write_frames=True, record_
         os.makedirs(out_dir, exist_ok=True)
     lbm = CLBMSolver(cfg, storage=storage, deviation=deviation)
     ibm = DroneMovingIBM(cfg, spread_mode=spread_mode)
+    writer = FrameWriter(lbm, compress=compress) if (write_frames and async_frames) else None
     snaps = dict(xc=[], yc=[], theta=[], step=[])
     frame = 0
     diverged = False
@@ -82,11 +147,17 @@ def run(cfg, steps=None, out_dir=None, storage="f32", write_frames=True, record_
                 break
             if write_frames and step % cfg.vis_interval == 0:
                 frame += 1
-                np.savez_compressed(os.path.join(out_dir, f"frame_{frame:06d}.npz"),
-                                    ux=lbm.ux.to_numpy(), uy=lbm.uy.to_numpy(),
-                                    uz=lbm.uz.to_numpy(), rho=lbm.rho.to_numpy())
+                path = os.path.join(out_dir, f"frame_{frame:06d}.npz")
+                if writer is not None:
+                    writer.submit(path)
+                else:
+                    (np.savez_compressed if compress else np.savez)(
+                        path, ux=lbm.ux.to_numpy(), uy=lbm.uy.to_numpy(),
+                        uz=lbm.uz.to_numpy(), rho=lbm.rho.to_numpy())
                 for k, v in zip(("xc", "yc", "theta", "step"), (xc, yc, th, step)):
                     snaps[k].append(v)
+    if writer is not None:
+        writer.close()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     hist = ibm.force_history()
@@ -122,10 +193,16 @@ def main(argv=None) -> int:
     ap.add_argument("--deviation", default="weights", choices=["weights", "inflow"],
                     help="deviation base of 16-bit storage")
     ap.add_argument("--no-frames", action="store_true")
+    ap.add_argument("--sync-frames", action="store_true",
+                    help="read and write each frame before the next step (as the reference)")
+    ap.add_argument("--no-compress", action="store_true", help="frames with np.savez")
+    ap.add_argument("--vis-interval", type=int, default=None, help="steps between frames")
     a = ap.parse_args(argv)
     cfg = CASES[a.case]()
+    if a.vis_interval is not None:
+        cfg = cfg.with_(vis_interval=a.vis_interval)
     res = run(cfg, a.steps, a.out, a.storage, write_frames=not a.no_frames,
-              deviation=a.deviation)
+              deviation=a.deviation, async_frames=not a.sync_frames, compress=not a.no_compress)
     return 1 if res["diverged"] else 0
 
 

@@ -51,3 +51,27 @@ def test_driver_writes_reference_format_and_matches_oracle(pkg, oracle, gpu, tmp
     np.testing.assert_allclose(fr["ux"], ux, rtol=0, atol=2e-6)
     np.testing.assert_allclose(fr["rho"], rho, rtol=0, atol=2e-6)
     np.testing.assert_allclose(forces["sums"], hist, rtol=0, atol=1e-4 * np.abs(hist).max())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("compress", [True, False])
+def test_async_frames_equal_the_synchronous_ones(pkg, gpu, tmp_path, compress):
+    """Frames written through the pinned copy stream and worker threads hold
+    the same arrays as frames read before the next step; six frames through
+    three snapshot slots, so every slot is reused while steps go on."""
+    drv = load_sub("driver")
+    cfg = pkg.config.naca_case(40, 24, 4, 8.0, name="drv", N_steps=12).with_(vis_interval=2)
+    a = drv.run(cfg, 12, str(tmp_path / "a"), verbose=False, async_frames=True, compress=compress)
+    b = drv.run(cfg, 12, str(tmp_path / "b"), verbose=False, async_frames=False, compress=compress)
+    assert a["steps"] == b["steps"] == 12
+    names = sorted(os.listdir(tmp_path / "a"))
+    assert names == sorted(os.listdir(tmp_path / "b"))
+    assert sum(n.startswith("frame_") for n in names) == 6
+    for n in names:
+        fa, fb = np.load(tmp_path / "a" / n), np.load(tmp_path / "b" / n)
+        assert fa.files == fb.files
+        for k in fa.files:
+            np.testing.assert_array_equal(fa[k], fb[k], err_msg=f"{n}:{k}")
+    # consecutive frames differ: each holds its own step's fields
+    f1, f2 = np.load(tmp_path / "a" / "frame_000001.npz"), np.load(tmp_path / "a" / "frame_000002.npz")
+    assert not np.array_equal(f1["ux"], f2["ux"])
