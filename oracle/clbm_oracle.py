@@ -558,13 +558,14 @@ def motion_state(step: int, x0: float, y0: float, A_heave: float, A_pitch: float
 
 
 def run_coupled(f, X0, Y0, Z0, steps, motion, s_v, s_e, ux_in, uy_in,
-                f_max=0.05, wrap_weight_quirk=True, first_step=1, dV=None, dtype=None):
+                f_max=0.05, wrap_weight_quirk=True, first_step=1, dV=None, dtype=None,
+                inlet=True):
     """The driver loop of main_moving.py:116-122 for ``steps`` steps: per step
     motion -> compute_force(ux, uy, ...) -> lbm step.  ``motion(step)`` returns
     the 6-tuple of ``motion_state``.  ``f`` holds the state the loop starts
     from; the force-free u that the first compute_force reads is K7 of it (at
     step 1 of a fresh run that equals K1's u).  Returns (f, history) with the
-    per-step FP64 force sums."""
+    per-step FP64 force sums.  ``inlet`` as in ``lbm_step``."""
     dtype = dtype or f.dtype.type
     rho, ux, uy, uz = macro_post(f)
     hist = []
@@ -575,7 +576,7 @@ def run_coupled(f, X0, Y0, Z0, steps, motion, s_v, s_e, ux_in, uy_in,
         fx = out["fx"].astype(dtype)
         fy = out["fy"].astype(dtype)
         fz = out["fz"].astype(dtype)
-        f, rho, ux, uy, uz = lbm_step(f, fx, fy, fz, s_v, s_e, ux_in, uy_in)
+        f, rho, ux, uy, uz = lbm_step(f, fx, fy, fz, s_v, s_e, ux_in, uy_in, inlet=inlet)
         hist.append(force_sum_f64(out["FX"], out["FY"], dV))
     return f, np.array(hist, dtype=np.float64)
 

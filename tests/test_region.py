@@ -191,8 +191,8 @@ def test_step_part_rejects_unnormalised_region(pkg, gpu):
                  C.c_void_p(a.state.data_ptr()), a.state_word, None, None, None, None, bad, 0, None)
 
 
-def _coupled(pkg, cfg, steps, overlap, spread="fixed"):
-    lbm = pkg.CLBMSolver(cfg)
+def _coupled(pkg, cfg, steps, overlap, spread="fixed", inlet=True):
+    lbm = pkg.CLBMSolver(cfg, inlet=inlet)
     ibm = pkg.DroneMovingIBM(cfg, overlap=overlap, spread_mode=spread)
     split = 0
     for n in range(1, steps + 1):
@@ -222,6 +222,30 @@ def test_overlapped_run_equals_single_stream_run(pkg, gpu, where):
     np.testing.assert_array_equal(a_ibm.force_history(), b_ibm.force_history())
     np.testing.assert_array_equal(a_ibm.FX_k.to_numpy(), b_ibm.FX_k.to_numpy())
     assert a_ibm.region_violations() == 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("le_x", [20.0, 1.5])
+def test_hecht_harting_coupled_run(pkg, oracle, gpu, le_x):
+    """The opt-in Hecht-Harting inlet through the coupled loop: split and
+    single-stream runs agree bit for bit, also when the stencils reach the
+    inlet plane (le_x 1.5: the marker kernels read x = 0 through the rule),
+    and both follow the FP64 oracle's coupled loop with the same inlet."""
+    cfg = pkg.config.naca_case(64, 32, 4, 12.0, name="hh", le_x=le_x)
+    a_lbm, a_ibm, split = _coupled(pkg, cfg, 6, True, inlet="hecht-harting")
+    b_lbm, b_ibm, nosplit = _coupled(pkg, cfg, 6, False, inlet="hecht-harting")
+    assert nosplit == 0 and split == 6
+    np.testing.assert_array_equal(a_lbm.f.to_numpy(), b_lbm.f.to_numpy())
+    np.testing.assert_array_equal(a_ibm.force_history(), b_ibm.force_history())
+    assert a_ibm.region_violations() == 0
+    f, *_ = oracle.init_feq(cfg.Nx, cfg.Ny, cfg.Nz, cfg.ux_in, cfg.uy_in)
+    f, hist = oracle.run_coupled(f, cfg.X0, cfg.Y0, cfg.Z0, 6, cfg.motion_state, cfg.s_v, cfg.s_e,
+                                 cfg.ux_in, cfg.uy_in, f_max=cfg.f_max, dV=cfg.dV,
+                                 dtype=np.float64, inlet="hecht-harting")
+    np.testing.assert_allclose(a_lbm.f.to_numpy(), f, rtol=0, atol=2e-6)
+    np.testing.assert_allclose(a_ibm.force_history(), hist, rtol=0, atol=1e-4 * np.abs(hist).max())
+    if le_x < 2:       # stencils (floor(X) - 1 .. floor(X) + 2) reach the inlet plane
+        assert float(a_ibm.X_marker.to_numpy().min()) < 2.0
 
 
 @pytest.mark.gpu
