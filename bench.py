@@ -89,6 +89,17 @@ def cpu_baseline(cfg, seconds: float, threads: int):
                        f"restatement of the 12 kernels, float32 AoS, {threads} threads")}
 
 
+def host_cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def traffic_record(case: str, storage: str):
     """HBM bytes per launch of the fused step kernel, from the rocprofv3
     FETCH_SIZE / WRITE_SIZE runs summarised in profiles/ (null when absent)."""
@@ -205,6 +216,17 @@ def main(argv=None) -> int:
 
     mlups = a.steps * ncell / dt / 1e6
     cpu = cpu_baseline(cfg, a.cpu_seconds, a.cpu_threads)
+    if cpu and cpu.get("value"):
+        # the same restatement on every core this process may use (capped by
+        # OMP_NUM_THREADS where set), on half the sample time
+        host = len(os.sched_getaffinity(0))
+        host = min(host, int(os.environ.get("OMP_NUM_THREADS", host)))
+        cpu["cpu_model"] = host_cpu_model()
+        if host != a.cpu_threads:
+            wide = cpu_baseline(cfg, a.cpu_seconds / 2, host)
+            if wide and wide.get("value"):
+                cpu["all_cores"] = {"value": wide["value"], "unit": "MLUPS", "cores": host,
+                                    "sample": wide["sample"]}
     out = {
         "metric": METRIC, "value": round(mlups, 1), "unit": "MLUPS", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(dt / a.steps * 1e3, 5),
