@@ -41,6 +41,9 @@ def parse(argv=None):
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-MiB device buffer 10 times (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
+    ap.add_argument("--stream-probe", action="store_true",
+                    help="only time clbm_stream_probe (the step's data movement without the "
+                         "collision) on the --case grid with --storage populations")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target duration of the CPU baseline sample (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=8)
@@ -125,6 +128,36 @@ def copy_calibration(mib: int) -> int:
     return 0
 
 
+def stream_probe(pkg, cfg, storage, steps, warmup) -> int:
+    """clbm_stream_probe over the whole lattice, parities alternating as in a
+    run: every cell's 2 x 19 population accesses of a step in the fused
+    kernel's sweep order, without the collision.  Timed with events on the
+    stream it is launched on."""
+    import torch
+    lib = importlib.import_module(PKG + "._lib")
+    lbm = pkg.CLBMSolver(cfg, storage=storage, device="cuda:0")
+    st = torch.cuda.current_stream()
+    h = C.c_void_p(st.cuda_stream)
+    fn = lib.fn("clbm_stream_probe")
+    gref, fp = C.byref(lbm.grid), C.c_void_p(lbm.state.data_ptr())
+    for i in range(warmup):
+        lib.check(fn(gref, fp, i & 1, h), "clbm_stream_probe")
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for i in range(steps):
+        lib.check(fn(gref, fp, (warmup + i) & 1, h), "clbm_stream_probe")
+    e1.record(st)
+    torch.cuda.synchronize()
+    us = e0.elapsed_time(e1) * 1e3 / steps
+    nbytes = 2 * 19 * ELEM[storage] * lbm.ncell
+    print(json.dumps({"probe": "clbm_stream_probe: data movement of one AA step, no collision",
+                      "workload": cfg.name, "storage": storage, "launches": steps,
+                      "us_per_launch": round(us, 2), "bytes_per_launch": nbytes,
+                      "achieved_GBs": round(nbytes / us / 1e3, 1),
+                      "frac_of_8TBs": round(nbytes / us / 1e3 / HBM_PEAK_GBS, 4)}))
+    return 0
+
+
 def measure(pkg, cfg, storage, steps, warmup, dev, tune, deviation="weights"):
     """Warm-up, then ``steps`` timed coupled steps (compute_force + step) of a
     fresh solver/IBM pair with ``storage`` populations."""
@@ -184,6 +217,8 @@ def main(argv=None) -> int:
     cfg = {"c0": cfgmod.case_c0, "c1": cfgmod.case_c1, "c2": cfgmod.case_c2_large}[a.case]()
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if a.stream_probe:
+        return stream_probe(pkg, cfg, a.storage, a.steps, a.warmup)
     ncell = cfg.Nx * cfg.Ny * cfg.Nz
     tune = {"lbm": {}, "ibm": {"overlap": a.overlap}}
     if a.outside_lds is not None:

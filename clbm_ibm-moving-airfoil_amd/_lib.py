@@ -88,6 +88,7 @@ _SIGNATURES = {
     "clbm_import_state": (_I, [_G, _P, _P, _P]),
     "clbm_import_state_f64": (_I, [_G, _P, _P, _P]),
     "clbm_health": (_I, [_G, _LP, _P, _I, _P, _P]),
+    "clbm_stream_probe": (_I, [_G, _P, _I, _P]),
     "ibm_compute_force": (_I, [_G, _IP, _LP, _P, _I, _P, _P, _I, _P, _P, _P, _P, _F, _F, _F, _F,
                                _F, _F, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                                C.c_uint32, _P]),

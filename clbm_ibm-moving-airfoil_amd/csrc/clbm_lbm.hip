@@ -166,6 +166,23 @@ __global__ __launch_bounds__(BLK) void k_step(StepArgs a) {
   step_cell<S, PAR, FORCE, BOX, REG>(a, sweep_block<PAR>(blockIdx.x, gridDim.x) * BLK + threadIdx.x);
 }
 
+// The data movement of one AA step without the collision: every cell's 19
+// post-stream slots loaded and stored into the slots the step writes (the
+// populations permuted, their values kept), in the step's sweep order.  The
+// ceiling that the fused step kernel is measured against.
+template <int S, int PAR>
+__global__ __launch_bounds__(BLK) void k_stream_probe(Geo g, void* f) {
+  const int c = sweep_block<PAR>(blockIdx.x, gridDim.x) * BLK + threadIdx.x;
+  if (c >= g.ncell) return;
+  int x, y, z;
+  cell_xyz(g, c, x, y, z);
+  const Pop<S> pop((typename Store<S>::T*)f, g);
+  const Nbr nb = make_nbr(g, x, y, z);
+  typename Store<S>::R fq[Q];
+  pop_load<PAR>(pop, c, nb, fq);
+  pop_store<PAR>(pop, c, nb, fq);
+}
+
 // Many lattices in one launch: block b (in sweep order) belongs to
 // instance inst[b] and is that instance's block b - first[inst[b]]; every
 // instance reads its force inside its force box.
@@ -378,6 +395,16 @@ template <int S> struct StepRun {
     else
       launch_par<1>(g, p, pl, f, fx, fy, fz, box, rg, part, st);
     return check_launch(part < 0 ? "clbm_step" : "clbm_step_part");
+  }
+};
+
+template <int S> struct ProbeRun {
+  static int run(const Geo& g, void* f, int parity, hipStream_t st) {
+    if (parity == 0)
+      hipLaunchKernelGGL((k_stream_probe<S, 0>), dim3(nblocks(g.ncell)), dim3(BLK), 0, st, g, f);
+    else
+      hipLaunchKernelGGL((k_stream_probe<S, 1>), dim3(nblocks(g.ncell)), dim3(BLK), 0, st, g, f);
+    return check_launch("clbm_stream_probe");
   }
 };
 
@@ -683,6 +710,13 @@ int clbm_import_state_f64(const clbm_grid_t* g, void* f, const double* in, void*
   if (int e = check_grid(g, false)) return e;
   if (!f || !in) return fail(-1, "clbm_import_state_f64: NULL buffer");
   return dispatch_storage<ImportRun>(g->storage, make_geo(*g), f, in, (hipStream_t)stream);
+}
+
+int clbm_stream_probe(const clbm_grid_t* g, void* f, int parity, void* stream) {
+  if (int e = check_grid(g, false)) return e;
+  if (!f) return fail(-1, "clbm_stream_probe: f is NULL");
+  if (parity != 0 && parity != 1) return fail(-1, "clbm_stream_probe: parity must be 0 or 1");
+  return dispatch_storage<ProbeRun>(g->storage, make_geo(*g), f, parity, (hipStream_t)stream);
 }
 
 int clbm_health(const clbm_grid_t* g, const clbm_params_t* p, const void* f, int state,

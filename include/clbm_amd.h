@@ -211,6 +211,13 @@ int clbm_import_state(const clbm_grid_t* g, void* f, const float* in, void* stre
 /* The same from float64 planes. */
 int clbm_import_state_f64(const clbm_grid_t* g, void* f, const double* in, void* stream);
 
+/* No counterpart in the reference: a measurement probe.  Moves the data of one
+ * AA step of clbm_step from parity `parity` without the collision: every
+ * cell's 19 post-stream slots are loaded and stored into the slots the step
+ * writes, in the step's sweep order, so that the populations end up permuted
+ * (values kept).  Its duration is the ceiling clbm_step is measured against. */
+int clbm_stream_probe(const clbm_grid_t* g, void* f, int parity, void* stream);
+
 /* Replaces the driver's divergence guard (main_moving.py:130-145): out2[0] =
  * min rho over all cells, out2[1] = 1.0 if every population is finite else 0.0. */
 int clbm_health(const clbm_grid_t* g, const clbm_params_t* p, const void* f, int state,

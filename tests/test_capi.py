@@ -92,6 +92,8 @@ def test_solver_rejects_an_unknown_inlet(pkg):
     (lambda lib: lib.call("clbm_step", C.byref(lib.ClbmGrid(8, 4, 4, 0)),
                           C.byref(lib.ClbmParams(1.7, 1.0, 1.0, 0.0, 2, 0)), C.c_void_p(1), 0,
                           None, None, None, None, None), "ux_in < 1"),
+    (lambda lib: lib.call("clbm_stream_probe", C.byref(lib.ClbmGrid(8, 4, 4, 0)), C.c_void_p(1),
+                          2, None), "parity"),
     (lambda lib: lib.call("clbm_stream_after", None, None, None), "NULL event"),
     (lambda lib: lib.call("clbm_event_create", None), "NULL"),
 ])

@@ -238,3 +238,31 @@ def test_step_rejects_bad_arguments(pkg, gpu):
     with pytest.raises(lib.ClbmError, match="needs the params"):
         lib.call("clbm_macro", C.byref(g), None, C.c_void_p(1), lib.STATE_PLANES, None, None, None,
                  None, None)
+
+
+@pytest.mark.parametrize("parity", [0, 1])
+def test_stream_probe_moves_the_step_slots(pkg, oracle, gpu, parity):
+    """clbm_stream_probe (the measurement probe) loads each cell's slots and
+    stores them where a step from that parity stores: parity 0 new[opp q][c] =
+    old[q][c]; parity 1 new[q][c + e_q] = old[opp q][c - e_q] (periodic), i.e.
+    new[q] = old[opp q] shifted by 2 e_q.  Exact."""
+    import ctypes as C
+    import torch
+    lib = load_sub("_lib")
+    nx, ny, nz = 9, 7, 5
+    s = pkg.CLBMSolver(_case(pkg, nx, ny, nz))
+    old = np.random.default_rng(3).random((19, nz, ny, nx), dtype=np.float32)
+    s.state.copy_(torch.from_numpy(old.reshape(-1).view(np.uint8)).cuda())
+    lib.call("clbm_stream_probe", C.byref(s.grid), C.c_void_p(s.state.data_ptr()), parity, None)
+    torch.cuda.synchronize()
+    new = s.state.cpu().numpy().view(np.float32).reshape(19, nz, ny, nx)
+    e = np.stack([oracle.EX, oracle.EY, oracle.EZ], 1).astype(int)
+    opp = [int(np.flatnonzero((e == -e[q]).all(1))[0]) for q in range(19)]
+    exp = np.empty_like(old)
+    for q in range(19):
+        if parity == 0:
+            exp[opp[q]] = old[q]
+        else:
+            exp[q] = np.roll(old[opp[q]], shift=(2 * e[q, 2], 2 * e[q, 1], 2 * e[q, 0]),
+                             axis=(0, 1, 2))
+    np.testing.assert_array_equal(new, exp)
