@@ -128,11 +128,12 @@ def copy_calibration(mib: int) -> int:
     return 0
 
 
-def stream_probe(pkg, cfg, storage, steps, warmup) -> int:
+def stream_probe(pkg, cfg, storage, steps, warmup, quiet=False):
     """clbm_stream_probe over the whole lattice, parities alternating as in a
     run: every cell's 2 x 19 population accesses of a step in the fused
     kernel's sweep order, without the collision.  Timed with events on the
-    stream it is launched on."""
+    stream it is launched on.  Prints the record (returns 0), or returns it
+    with ``quiet``."""
     import torch
     lib = importlib.import_module(PKG + "._lib")
     lbm = pkg.CLBMSolver(cfg, storage=storage, device="cuda:0")
@@ -150,11 +151,16 @@ def stream_probe(pkg, cfg, storage, steps, warmup) -> int:
     torch.cuda.synchronize()
     us = e0.elapsed_time(e1) * 1e3 / steps
     nbytes = 2 * 19 * ELEM[storage] * lbm.ncell
-    print(json.dumps({"probe": "clbm_stream_probe: data movement of one AA step, no collision",
-                      "workload": cfg.name, "storage": storage, "launches": steps,
-                      "us_per_launch": round(us, 2), "bytes_per_launch": nbytes,
-                      "achieved_GBs": round(nbytes / us / 1e3, 1),
-                      "frac_of_8TBs": round(nbytes / us / 1e3 / HBM_PEAK_GBS, 4)}))
+    rec = {"probe": "clbm_stream_probe: data movement of one AA step, no collision",
+           "workload": cfg.name, "storage": storage, "launches": steps,
+           "us_per_launch": round(us, 2), "bytes_per_launch": nbytes,
+           "achieved_GBs": round(nbytes / us / 1e3, 1),
+           "frac_of_8TBs": round(nbytes / us / 1e3 / HBM_PEAK_GBS, 4)}
+    del lbm
+    torch.cuda.empty_cache()
+    if quiet:
+        return rec
+    print(json.dumps(rec))
     return 0
 
 
@@ -250,6 +256,12 @@ def main(argv=None) -> int:
                           "finite": o["finite"]}
 
     mlups = a.steps * ncell / dt / 1e6
+    # the dominant kernel against its data-movement ceiling: the same loads and
+    # stores in the same order without the collision (whole lattice)
+    probe = stream_probe(pkg, cfg, a.storage, 200, 10, quiet=True)
+    ceiling = {"kernel": "k_stream_probe", "achieved": probe["achieved_GBs"], "unit": "GB/s",
+               "us_per_launch": probe["us_per_launch"],
+               "kernel_frac_of_probe": round(achieved / probe["achieved_GBs"], 4)}
     cpu = cpu_baseline(cfg, a.cpu_seconds, a.cpu_threads)
     if cpu and cpu.get("value"):
         # the same restatement on every core this process may use (capped by
@@ -285,6 +297,7 @@ def main(argv=None) -> int:
                      "bytes_per_launch": round(b_alg),
                      "step_frac": round(mlups * 1e6 * 2 * 19 * ELEM[a.storage] / 1e9 / HBM_PEAK_GBS, 4)},
         "host_ms_per_step": round(r["host_ms"], 5),
+        "ceiling_probe": ceiling,
         "cpu_baseline": cpu,
         "health": {"rho_min": rho_min, "finite": finite},
         "storage_16bit": {k: dict(v, ratio_vs_f32=round(v["value"] / mlups, 3))
