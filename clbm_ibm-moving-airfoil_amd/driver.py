@@ -58,10 +58,16 @@ class FrameWriter:
     into a pinned host buffer, and a worker thread turns it into the
     reference's (Nx, Ny, Nz) arrays and writes the .npz, while the GPU steps
     on.  A slot is reused only after its file is written, so a writer that
-    falls behind holds the loop back instead of dropping frames."""
+    falls behind holds the loop back instead of dropping frames.  ``depth``
+    slots (default: one per two usable cores, 2 to 8) each hold a frame on
+    the device and in pinned host memory."""
 
-    def __init__(self, lbm, depth: int = 3, compress: bool = True):
+    def __init__(self, lbm, depth=None, compress: bool = True):
         import torch
+        if depth is None:
+            # compression takes about 3 s of one core per 96 MB frame: one slot
+            # (and one writer thread) per two usable cores, from 2 to 8
+            depth = min(8, max(2, len(os.sched_getaffinity(0)) // 2))
         self.lbm = lbm
         self.compress = compress
         shape = (4, lbm.Nz, lbm.Ny, lbm.Nx)
