@@ -48,6 +48,20 @@ def grid_meta(cfg) -> np.ndarray:
 FRAME_KEYS = ("rho", "ux", "uy", "uz")
 
 
+def save_frame(path: str, level, **arrays) -> None:
+    """An .npz as np.savez / np.savez_compressed write it (one .npy member per
+    array), deflated at zlib ``level`` (None: stored).  np.load reads every
+    level alike; level 1 deflates float fields several times faster than
+    np.savez_compressed's default level 6, at a slightly larger file."""
+    import zipfile
+    mode = zipfile.ZIP_STORED if level is None else zipfile.ZIP_DEFLATED
+    with zipfile.ZipFile(path, "w", compression=mode, compresslevel=level,
+                         allowZip64=True) as zf:
+        for name, a in arrays.items():
+            with zf.open(name + ".npy", "w", force_zip64=True) as fh:
+                np.lib.format.write_array(fh, np.asanyarray(a), allow_pickle=False)
+
+
 class FrameWriter:
     """Frames leave the device without stopping the step loop.
 
@@ -62,7 +76,7 @@ class FrameWriter:
     slots (default: one per two usable cores, 2 to 8) each hold a frame on
     the device and in pinned host memory."""
 
-    def __init__(self, lbm, depth=None, compress: bool = True):
+    def __init__(self, lbm, depth=None, compress=1):
         import torch
         if depth is None:
             # compression takes about 3 s of one core per 96 MB frame: one slot
@@ -100,8 +114,8 @@ class FrameWriter:
         a = host.numpy()
         arrays = {key: np.ascontiguousarray(a[i].transpose(2, 1, 0))
                   for i, key in enumerate(FRAME_KEYS)}
-        (np.savez_compressed if self.compress else np.savez)(
-            path, ux=arrays["ux"], uy=arrays["uy"], uz=arrays["uz"], rho=arrays["rho"])
+        save_frame(path, _level(self.compress), ux=arrays["ux"], uy=arrays["uy"],
+                   uz=arrays["uz"], rho=arrays["rho"])
 
     def close(self) -> None:
         for d in self._done:
@@ -110,15 +124,24 @@ class FrameWriter:
         self._pool.shutdown()
 
 
+def _level(compress):
+    """zlib level of a ``compress`` argument: False -> stored, True -> 6
+    (np.savez_compressed's), an int -> that level."""
+    if compress is False or compress is None:
+        return None
+    return 6 if compress is True else int(compress)
+
+
 def run(cfg, steps=None, out_dir=None, storage="f32", write_frames=True, record_forces=True,
         verbose=True, spread_mode="fixed", deviation="weights", async_frames=True,
-        compress=True):
+        compress=1):
     """Run the coupled loop; returns a dict with the force history and timing.
     ``forces.npz`` is written whenever the forces are recorded, frames and
     ``meta.npz`` only with ``write_frames``.  ``async_frames`` writes frames
     through ``FrameWriter`` (False: read and write each frame before the next
-    step, as the reference does); ``compress`` False writes frames with
-    np.savez instead of np.savez_compressed (the same arrays for np.load)."""
+    step, as the reference does); ``compress`` is the frames' zlib level (1 by
+    default, True for np.savez_compressed's 6, False for stored members as
+    np.savez writes them; np.load returns the same arrays for all)."""
     import torch
     steps = cfg.N_steps if steps is None else steps
     out_dir = cfg.DATA_DIR if out_dir is None else out_dir
@@ -157,9 +180,9 @@ def run(cfg, steps=None, out_dir=None, storage="f32", write_frames=True, record_
                 if writer is not None:
                     writer.submit(path)
                 else:
-                    (np.savez_compressed if compress else np.savez)(
-                        path, ux=lbm.ux.to_numpy(), uy=lbm.uy.to_numpy(),
-                        uz=lbm.uz.to_numpy(), rho=lbm.rho.to_numpy())
+                    save_frame(path, _level(compress), ux=lbm.ux.to_numpy(),
+                               uy=lbm.uy.to_numpy(), uz=lbm.uz.to_numpy(),
+                               rho=lbm.rho.to_numpy())
                 for k, v in zip(("xc", "yc", "theta", "step"), (xc, yc, th, step)):
                     snaps[k].append(v)
     if writer is not None:
@@ -201,14 +224,17 @@ def main(argv=None) -> int:
     ap.add_argument("--no-frames", action="store_true")
     ap.add_argument("--sync-frames", action="store_true",
                     help="read and write each frame before the next step (as the reference)")
-    ap.add_argument("--no-compress", action="store_true", help="frames with np.savez")
+    ap.add_argument("--no-compress", action="store_true", help="frames stored, as np.savez")
+    ap.add_argument("--compress-level", type=int, default=1,
+                    help="zlib level of the frames (6 = np.savez_compressed's)")
     ap.add_argument("--vis-interval", type=int, default=None, help="steps between frames")
     a = ap.parse_args(argv)
     cfg = CASES[a.case]()
     if a.vis_interval is not None:
         cfg = cfg.with_(vis_interval=a.vis_interval)
     res = run(cfg, a.steps, a.out, a.storage, write_frames=not a.no_frames,
-              deviation=a.deviation, async_frames=not a.sync_frames, compress=not a.no_compress)
+              deviation=a.deviation, async_frames=not a.sync_frames,
+              compress=False if a.no_compress else a.compress_level)
     return 1 if res["diverged"] else 0
 
 

@@ -53,8 +53,29 @@ def test_driver_writes_reference_format_and_matches_oracle(pkg, oracle, gpu, tmp
     np.testing.assert_allclose(forces["sums"], hist, rtol=0, atol=1e-4 * np.abs(hist).max())
 
 
+@pytest.mark.parametrize("level", [None, 1, 6])
+def test_save_frame_is_an_npz_np_load_reads(pkg, tmp_path, level):
+    """save_frame writes what np.load returns unchanged, at every level, and
+    level 6 gives the member layout of np.savez_compressed."""
+    import zipfile
+    drv = load_sub("driver")
+    rng = np.random.default_rng(0)
+    arrs = {k: rng.random((5, 4, 3), dtype=np.float32) for k in ("ux", "uy", "uz", "rho")}
+    drv.save_frame(str(tmp_path / "f.npz"), level, **arrs)
+    got = np.load(tmp_path / "f.npz")
+    assert sorted(got.files) == sorted(arrs)
+    for k, v in arrs.items():
+        assert got[k].dtype == np.float32
+        np.testing.assert_array_equal(got[k], v)
+    np.savez_compressed(tmp_path / "g.npz", **arrs)
+    names = lambda p: sorted(i.filename for i in zipfile.ZipFile(p).infolist())
+    assert names(tmp_path / "f.npz") == names(tmp_path / "g.npz")
+    kinds = {i.compress_type for i in zipfile.ZipFile(tmp_path / "f.npz").infolist()}
+    assert kinds == {zipfile.ZIP_STORED if level is None else zipfile.ZIP_DEFLATED}
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("compress", [True, False])
+@pytest.mark.parametrize("compress", [1, False])
 def test_async_frames_equal_the_synchronous_ones(pkg, gpu, tmp_path, compress):
     """Frames written through the pinned copy stream and worker threads hold
     the same arrays as frames read before the next step; six frames through
