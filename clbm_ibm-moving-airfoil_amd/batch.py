@@ -133,7 +133,7 @@ class BatchSweep:
     def __init__(self, cfgs, storage: str = "f32", device=None, steps: int = 1000,
                  block: int = 256, group: int = 8, first_step: int = 1,
                  cache_bytes: int = 192 << 20, deviation: str = "weights",
-                 solo_min_cells: int = DroneMovingIBM.OVERLAP_MIN_CELLS):
+                 solo_min_cells: int = DroneMovingIBM.OVERLAP_MIN_CELLS, inlet=True):
         import torch
         self.cfgs = list(cfgs)
         self.device = torch.device(device if device is not None else "cuda")
@@ -141,8 +141,8 @@ class BatchSweep:
         self.deviation = deviation
         self.first_step = first_step
         self.capacity = int(steps)
-        self.lbms = [CLBMSolver(c, storage=storage, device=self.device, deviation=deviation)
-                     for c in self.cfgs]
+        self.lbms = [CLBMSolver(c, storage=storage, device=self.device, deviation=deviation,
+                                inlet=inlet) for c in self.cfgs]
         self.ibms = [DroneMovingIBM(c, device=self.device, overlap=False) for c in self.cfgs]
         self._kin = [torch.from_numpy(kinematics_table(c, first_step, self.capacity)).to(self.device)
                      for c in self.cfgs]
@@ -386,7 +386,7 @@ class BatchSweep:
 
 
 def sequential(cfgs, steps: int, storage: str = "f32", device=None, first_step: int = 1,
-               deviation: str = "weights"):
+               deviation: str = "weights", inlet=True):
     """Each instance on its own (its CLBMSolver and DroneMovingIBM with their
     default per-instance launches: the overlapped split step on the large
     lattices), one after the other; returns the objects and the elapsed
@@ -394,7 +394,7 @@ def sequential(cfgs, steps: int, storage: str = "f32", device=None, first_step: 
     import torch
     objs, total = [], 0.0
     for c in cfgs:
-        lbm = CLBMSolver(c, storage=storage, device=device, deviation=deviation)
+        lbm = CLBMSolver(c, storage=storage, device=device, deviation=deviation, inlet=inlet)
         ibm = DroneMovingIBM(c, device=device)
         torch.cuda.synchronize()
         t0 = time.perf_counter()

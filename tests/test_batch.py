@@ -106,6 +106,20 @@ def test_16bit_batch_equals_each_instance_alone(pkg, gpu, storage, deviation):
 
 
 @pytest.mark.gpu
+def test_hecht_harting_batch_equals_each_instance_alone(pkg, gpu):
+    """The opt-in inlet reaches the batched kernels (the rule travels in each
+    instance's argument record) and matches the per-instance path bit for bit."""
+    bm = load_sub("batch")
+    cfgs = _small(pkg)[:3]
+    sweep = bm.BatchSweep(cfgs, steps=16, inlet="hecht-harting")
+    sweep.run(7, graph=True)
+    seq, _ = bm.sequential(cfgs, 7, inlet="hecht-harting")
+    _assert_same(list(zip(sweep.lbms, sweep.ibms)), seq)
+    ref, _ = bm.sequential(cfgs[:1], 7)           # the reference rule gives other populations
+    assert not np.array_equal(ref[0][0].f.to_numpy(), seq[0][0].f.to_numpy())
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("cache_bytes,ngroups", [(192 << 20, 1), (800_000, 2), (1, 3)])
 def test_graph_replay_equals_eager(pkg, gpu, cache_bytes, ngroups):
     bm = load_sub("batch")
