@@ -134,20 +134,21 @@ def _level(compress):
 
 def run(cfg, steps=None, out_dir=None, storage="f32", write_frames=True, record_forces=True,
         verbose=True, spread_mode="fixed", deviation="weights", async_frames=True,
-        compress=1):
+        compress=1, inlet=True):
     """Run the coupled loop; returns a dict with the force history and timing.
     ``forces.npz`` is written whenever the forces are recorded, frames and
     ``meta.npz`` only with ``write_frames``.  ``async_frames`` writes frames
     through ``FrameWriter`` (False: read and write each frame before the next
     step, as the reference does); ``compress`` is the frames' zlib level (1 by
     default, True for np.savez_compressed's 6, False for stored members as
-    np.savez writes them; np.load returns the same arrays for all)."""
+    np.savez writes them; np.load returns the same arrays for all).  ``inlet``
+    as CLBMSolver's (True: the reference's rule, "hecht-harting")."""
     import torch
     steps = cfg.N_steps if steps is None else steps
     out_dir = cfg.DATA_DIR if out_dir is None else out_dir
     if write_frames or record_forces:
         os.makedirs(out_dir, exist_ok=True)
-    lbm = CLBMSolver(cfg, storage=storage, deviation=deviation)
+    lbm = CLBMSolver(cfg, storage=storage, deviation=deviation, inlet=inlet)
     ibm = DroneMovingIBM(cfg, spread_mode=spread_mode)
     writer = FrameWriter(lbm, compress=compress) if (write_frames and async_frames) else None
     snaps = dict(xc=[], yc=[], theta=[], step=[])
@@ -228,13 +229,16 @@ def main(argv=None) -> int:
     ap.add_argument("--compress-level", type=int, default=1,
                     help="zlib level of the frames (6 = np.savez_compressed's)")
     ap.add_argument("--vis-interval", type=int, default=None, help="steps between frames")
+    ap.add_argument("--inlet", default="reference", choices=["reference", "hecht-harting"],
+                    help="inlet rule at x = 0 (the reference's, or the opt-in Hecht-Harting one)")
     a = ap.parse_args(argv)
     cfg = CASES[a.case]()
     if a.vis_interval is not None:
         cfg = cfg.with_(vis_interval=a.vis_interval)
     res = run(cfg, a.steps, a.out, a.storage, write_frames=not a.no_frames,
               deviation=a.deviation, async_frames=not a.sync_frames,
-              compress=False if a.no_compress else a.compress_level)
+              compress=False if a.no_compress else a.compress_level,
+              inlet=True if a.inlet == "reference" else a.inlet)
     return 1 if res["diverged"] else 0
 
 
