@@ -14,8 +14,14 @@ rounding spread).  Citations are to the reference repository
 Pinning (what checks each function, in ``tests/test_oracle_*.py``):
 
 * lattice constants, ``M_raw``/``M_inv``        -- exact identities (M M^-1 = I, dyadic entries)
+* ``relaxation_rates`` / ``inlet_velocity``     -- closed forms at every configured (U, c, Re)
+                                                   (tau 0.53, 0.56, 0.62, 0.518, 0.5024) and the
+                                                   tabulated cos/sin of the 8 degree tilt
 * ``init_feq``                                  -- closed-form moments
-* ``macro_with_force`` / ``macro_post``         -- closed form
+* ``macro_with_force``                          -- closed form
+* ``macro_post``                                -- rho and u recovered from hand-built second-order
+                                                   equilibria; fsum of the definition per cell;
+                                                   planted x/y swap and dropped direction caught
 * ``collide``                                   -- invariants computed by brute-force moment sums:
                                                    mass, momentum gain (1.5 - s_v/2) F, relaxed
                                                    central moments, rest fixed point, Guo moments
@@ -28,11 +34,20 @@ Pinning (what checks each function, in ``tests/test_oracle_*.py``):
 * ``update_markers``                            -- closed form (rotation, rigid velocity)
 * ``interpolate`` / ``spread``                  -- adjointness, partition of unity, seam weight 0.75,
                                                    brute-force stencil enumeration
-* ``marker_forces`` / ``clip_forces``           -- closed form
+* ``marker_forces``                             -- closed form
+* ``clip_forces``                               -- a table of known inputs (both bounds, exact
+                                                   f_max, in-range values), fz zero; planted
+                                                   one-sided clamp caught
+* ``trig_f32``                                  -- nearest float32 of FP64 cos/sin of the float32
+                                                   angle over 20,000 angles; planted truncation
+                                                   and unrounded angle caught
 * ``force_sum_f64``                             -- exact FP64 sum (math.fsum)
 * ``f16c_encode`` / ``f16c_decode``             -- known codes, full round trip, monotonicity,
                                                    brute-force nearest rounding of |v| + 2^-15
 * ``motion_state``                              -- closed form derivatives
+* ``run_coupled``                               -- two steps composed by hand (step 1 reads K7 of
+                                                   the initial state, step n of step n-1's state);
+                                                   a stale or half-force velocity differs
 
 Taichi itself is not installed here, so bit-level agreement with a Taichi
 run is unpinned; the only deliberate choice below is the FP32 marker
