@@ -24,6 +24,7 @@ BOX_WORDS = 2 * 6 * BOX_SUBSLOTS + 4        # uint64 words of a force-box record
 FORCE_BOX_OFFSET = 2 * 6 * BOX_SUBSLOTS     # word offset of the decoded int32[6] box
 REGION_FLAG_OFFSET = 2 * 6 * BOX_SUBSLOTS + 3   # word offset of the region-violation flag
 ABI_VERSION = 7
+CLOSURE_GRAD_NOALLOC_MAX_BLOCKS = 1024   # rows of the library's own partial-gradient array
 
 
 class ClbmGrid(C.Structure):
@@ -115,6 +116,7 @@ _SIGNATURES = {
     "clbm_step_closure": (_I, [_G, _LP, _P, _I, _P, _P, _P, _P, _P, _P, _F, _F, _P]),
     "closure_partial_bytes": (C.c_size_t, [_I]),
     "closure_grad": (_I, [_P, _P, _I, _P, _P, _P, _I, _F, _P, _I, _P, _P]),
+    "closure_grad_noalloc": (_I, [_P, _P, _I, _P, _P, _P, _I, _F, _I, _P, _P]),
     "closure_adam": (_I, [_P, _P, _P, _P, _F, _F, _F, _F, _I, _P]),
 }
 STATE_PARITY, STATE_PLANES = 1, 2

@@ -26,6 +26,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 namespace clbm {
@@ -641,6 +642,47 @@ template <int S> struct ClosureStepRun {
   }
 };
 
+// ---------------------------------------------------------------------------
+// closure_grad_noalloc: the per-workgroup partial gradients in a file-scope
+// device array instead of a caller's buffer.  Each GPU holds its own copy of
+// the array (at its own address) from the load of this code object until the
+// process ends.  Host code never passes the array's name to a kernel: the
+// name is a host-side stand-in, so the device address is looked up once per
+// device with hipGetSymbolAddress and kept.  Every workgroup of
+// k_closure_grad overwrites all GPART entries of its own row before
+// k_closure_reduce (same stream) reads them, so nothing needs clearing
+// between calls; calls that overlap on one GPU (other streams, other
+// threads) share the rows and race.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) float scratch[(size_t)CLOSURE_GRAD_NOALLOC_MAX_BLOCKS * GPART];
+
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static float* g_scratch_ptr[MAX_DEVICES] = {};
+
+// Device address of `scratch` on the current device into *out (looked up on
+// the first call on that device); 0, or the error code with the message set.
+static int scratch_on_current_device(float** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess)
+    return fail((int)e, std::string("closure_grad_noalloc: hipGetDevice: ") + hipGetErrorString(e));
+  if (dev < 0 || dev >= MAX_DEVICES)
+    return fail(-1, "closure_grad_noalloc: device ordinal beyond the library's table of 64");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess)
+      return fail((int)e, std::string("closure_grad_noalloc: hipGetSymbolAddress(scratch): ") +
+                              hipGetErrorString(e));
+    if (!ptr) return fail(-1, "closure_grad_noalloc: hipGetSymbolAddress(scratch) gave NULL");
+    g_scratch_ptr[dev] = (float*)ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
 }  // namespace clbm
 
 using namespace clbm;
@@ -715,6 +757,29 @@ int closure_grad(const void* packed, const float* params, int count, const float
   hipLaunchKernelGGL(k_closure_reduce, dim3((GPART + 255) / 256), dim3(256), 0, st, partial,
                      blocks, grad);
   return check_launch("closure_grad");
+}
+
+int closure_grad_noalloc(const void* packed, const float* params, int count, const float* X,
+                         const float* T, const float* W, int wdim, float wsum, int blocks,
+                         float* grad, void* stream) {
+  if (!packed || !params || !X || !T || !W || !grad)
+    return fail(-1, "closure_grad_noalloc: NULL buffer");
+  if (count <= 0 || blocks <= 0)
+    return fail(-1, "closure_grad_noalloc: count and blocks must be > 0");
+  if (blocks > CLOSURE_GRAD_NOALLOC_MAX_BLOCKS)
+    return fail(-1, "closure_grad_noalloc: blocks exceeds CLOSURE_GRAD_NOALLOC_MAX_BLOCKS (" +
+                        std::to_string(CLOSURE_GRAD_NOALLOC_MAX_BLOCKS) +
+                        "), the rows of the library's partial-gradient array");
+  if (wdim != 1 && wdim != 4) return fail(-1, "closure_grad_noalloc: wdim must be 1 or 4");
+  if (!(wsum > 0.0f)) return fail(-1, "closure_grad_noalloc: weight sum must be > 0");
+  float* partial = nullptr;
+  if (int e = scratch_on_current_device(&partial)) return e;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_closure_grad, dim3(blocks), dim3(64), 0, st, (const bf16x8*)packed, params,
+                     count, X, T, W, wdim, wsum, partial);
+  hipLaunchKernelGGL(k_closure_reduce, dim3((GPART + 255) / 256), dim3(256), 0, st, partial,
+                     blocks, grad);
+  return check_launch("closure_grad_noalloc");
 }
 
 int closure_adam(float* params, float* m, float* v, const float* grad, float lr, float beta1,

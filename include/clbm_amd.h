@@ -4,7 +4,9 @@
  * Plain pointers and sizes only.  Every pointer argument named as a device
  * buffer is device memory the caller owns (in the Python layer: torch-ROCm
  * tensors, passed as tensor.data_ptr()); `stream` is a hipStream_t or NULL
- * for the null stream.  The library allocates no long-lived device memory.
+ * for the null stream.  The library allocates no device memory; it holds one
+ * fixed array per GPU, the 7.1 MiB partial-gradient rows of
+ * closure_grad_noalloc, which come with its code object.
  *
  * Return codes: 0 = ok, < 0 = argument / configuration error, > 0 = a
  * hipError_t from a launch.  The message of the last failure on the calling
@@ -392,6 +394,21 @@ size_t closure_partial_bytes(int blocks);
 int closure_grad(const void* packed, const float* params, int count, const float* X,
                  const float* T, const float* W, int wdim, float wsum, float* partial, int blocks,
                  float* grad, void* stream);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, training or serving loops that forbid
+ * hipMalloc): closure_grad without the `partial` workspace.  The partial
+ * gradients go to a device array inside the library of
+ * CLOSURE_GRAD_NOALLOC_MAX_BLOCKS x (closure_param_count() + 1) floats
+ * (7.1 MiB), which each GPU the library runs on holds until the process ends;
+ * blocks > CLOSURE_GRAD_NOALLOC_MAX_BLOCKS is rejected with an error.  Same
+ * results as closure_grad with the same blocks, bit for bit.  Allocates
+ * nothing; the first call on a device looks the array's address up on that
+ * (current) device.  All calls on one GPU share the array: calls that overlap
+ * on one GPU (different streams or threads) race. */
+#define CLOSURE_GRAD_NOALLOC_MAX_BLOCKS 1024
+int closure_grad_noalloc(const void* packed, const float* params, int count, const float* X,
+                         const float* T, const float* W, int wdim, float wsum, int blocks,
+                         float* grad, void* stream);
 /* One Adam step (bias-corrected, step number t >= 1) on params with moments m, v. */
 int closure_adam(float* params, float* m, float* v, const float* grad, float lr, float beta1,
                  float beta2, float eps, int t, void* stream);
