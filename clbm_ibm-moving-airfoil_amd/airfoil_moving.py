@@ -286,6 +286,20 @@ class DroneMovingIBM:
         self._join()
         return int(self.box[_lib.REGION_FLAG_OFFSET].item())
 
+    def check_region(self) -> None:
+        """Raise ClbmError when the stencils of some compute_force call left
+        the region predicted for the split step (the marker kernel marks the
+        generation in the box record; reading it synchronises).  A split step
+        whose markers left their region ran them concurrently with the update
+        of cells they read, so the results after that generation are not
+        trustworthy."""
+        g = self.region_violations()
+        if g:
+            raise _lib.ClbmError(
+                f"the stencils of compute_force call {g} (generation {g}) left the region "
+                f"predicted for the split step; the marker kernels raced the lattice update "
+                f"away from the body (use overlap=False for this body)")
+
     def force_box(self):
         """The (x0, y0, z0, x1, y1, z1) box of the cells the last compute_force reached."""
         self._join()

@@ -302,6 +302,8 @@ class BatchSweep:
                 left -= 1
         self.state = self._state_after(k)
         self._advance_objects(k)
+        for i in self._solo:   # the split steps' stencils stayed in their regions
+            self.ibms[i].check_region()
 
     def _state_after(self, k: int) -> int:
         s = self.state
@@ -334,8 +336,8 @@ class BatchSweep:
         for l in self.lbms:
             l._init_feq()
             l.steps = 0
-        for i in self.ibms:
-            i.gen += 2     # generations stay increasing; the stale boxes are cleared by clip
+        # the IBM generations carry on: the next one's clamp reads the last
+        # one's box as its previous box and clears the forces left there
         self.steps_done = 0
         self.state = 0
         self.build(self.block, self.group)

@@ -24,6 +24,7 @@ Differences of mechanism (not of results):
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 
 import numpy as np
 
@@ -154,6 +155,9 @@ class CLBMSolver:
         # immersed-boundary objects reading the populations on their own
         # stream: {id: (ibm, region int32[6], event after their last read)}
         self._readers = {}
+        # the immersed-boundary objects whose forces a split step has used:
+        # health() checks that their stencils stayed in the predicted regions
+        self._split_ibms = weakref.WeakSet()
         # optional (start, end) events recorded around the first kernel of the
         # next step (the whole step, or its outside part); cells that kernel updated
         self.kernel_events = None
@@ -182,6 +186,7 @@ class CLBMSolver:
         state = self._state_ptr
         if split is not None:
             ibm, region = split
+            self._split_ibms.add(ibm)
             part = _lib.fn("clbm_step_part")
             if ev is not None:
                 ev[0].record(main)
@@ -292,13 +297,19 @@ class CLBMSolver:
 
     def health(self):
         """(rho_min, all_finite) of the current state, computed on the device
-        (the divergence guard of main_moving.py:143-145)."""
+        (the divergence guard of main_moving.py:143-145).  Also checks, with
+        the same synchronisation, that the stencils of every immersed-boundary
+        object whose forces a split step used stayed in the region predicted
+        for it (DroneMovingIBM.check_region raises ClbmError otherwise: the
+        marker kernels then raced the lattice update away from the body)."""
         import torch
         out = torch.empty(2, dtype=torch.float32, device=self.device)
         _lib.call("clbm_health", C.byref(self.grid), C.byref(self.params),
                   C.c_void_p(self.state.data_ptr()), self.state_word, C.c_void_p(out.data_ptr()),
                   self._stream())
         h = out.cpu().numpy()
+        for ibm in list(self._split_ibms):
+            ibm.check_region()
         return float(h[0]), bool(h[1] > 0.5)
 
     def _stream(self) -> C.c_void_p:

@@ -39,6 +39,7 @@ typedef struct {
   size_t ncell;
   float s_v, s_e, ux_in, uy_in, f_max;
   int quirk;
+  int inlet_rule; /* 0: the reference's rule (K5); 1: Hecht-Harting (opt-in, no counterpart) */
   float minv[Q * Q];
   float *f, *f_post, *rho, *ux, *uy, *uz;  /* lattice */
   float *fx, *fy, *fz;                     /* IBM Eulerian force */
@@ -245,6 +246,31 @@ static void k5_bc_inlet(ref_sim* s) {
     }
 }
 
+/* The opt-in Hecht-Harting velocity inlet (J. Stat. Mech. 2010, P01018; no
+ * reference counterpart), as clbm_oracle.bc_inlet_hecht_harting states it. */
+static void k5_bc_inlet_hecht_harting(ref_sim* s) {
+  const float ux_in = s->ux_in, uy_in = s->uy_in;
+  const float half = 0.5f, third = 1.0f / 3.0f, sixth = 1.0f / 6.0f;
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < s->ny; ++j)
+    for (int k = 0; k < s->nz; ++k) {
+      float* g = s->f + cidx(s, 0, j, k) * Q;
+      const float s_known = g[0] + g[3] + g[4] + g[5] + g[6] + g[15] + g[16] + g[17] + g[18];
+      const float s_out = g[2] + g[8] + g[10] + g[12] + g[14];
+      const float rho_in = (s_known + 2.0f * s_out) / (1.0f - ux_in);
+      const float n_y = half * ((g[3] - g[4]) + (g[15] - g[16]) + (g[17] - g[18])) -
+                        third * rho_in * uy_in;
+      const float n_z = half * ((g[5] - g[6]) + (g[15] + g[16]) - (g[17] + g[18]));
+      g[1] = g[2] + third * rho_in * ux_in;
+      g[7] = g[10] + sixth * rho_in * (ux_in + uy_in) - n_y;
+      g[9] = g[8] + sixth * rho_in * (ux_in - uy_in) + n_y;
+      g[11] = g[14] + sixth * rho_in * ux_in - n_z;
+      g[13] = g[12] + sixth * rho_in * ux_in + n_z;
+    }
+}
+
+void ref_set_inlet(ref_sim* s, int rule) { s->inlet_rule = rule ? 1 : 0; }
+
 /* K6 _bc_outlet, clbm_moving.py:296-300 */
 static void k6_bc_outlet(ref_sim* s) {
 #pragma omp parallel for schedule(static)
@@ -278,7 +304,10 @@ void ref_step(ref_sim* s) {
   k2_macro_with_force(s);
   k3_collide(s);
   k4_stream(s);
-  k5_bc_inlet(s);
+  if (s->inlet_rule)
+    k5_bc_inlet_hecht_harting(s);
+  else
+    k5_bc_inlet(s);
   k6_bc_outlet(s);
   k7_macro_post(s);
 }

@@ -32,6 +32,7 @@ def _load():
         lib.ref_set_threads.argtypes = [I]
         lib.ref_init_feq.argtypes = [P]
         lib.ref_step.argtypes = [P]
+        lib.ref_set_inlet.argtypes = [P, I]
         lib.ref_compute_force.argtypes = [P, F, F, F, F, F, F]
         for name in ("ref_get_f", "ref_set_f"):
             getattr(lib, name).argtypes = [P, P]
@@ -56,7 +57,7 @@ class RefSim:
     """One case (a CaseConfig-like object with Nx, Ny, Nz, s_v, s_e, ux_in,
     uy_in, X0, Y0, Z0, dV, f_max, wrap_weight_quirk, motion_state)."""
 
-    def __init__(self, cfg, threads: int = 8):
+    def __init__(self, cfg, threads: int = 8, inlet=True):
         lib = _load()
         self.cfg = cfg
         self.shape = (cfg.Nx, cfg.Ny, cfg.Nz)
@@ -70,6 +71,8 @@ class RefSim:
         self.dV = (None if getattr(cfg, "dV", None) is None
                    else np.ascontiguousarray(cfg.dV, dtype=np.float32))
         self.n = len(self._keep[0])
+        # inlet: True for the reference's rule, "hecht-harting" for the opt-in one
+        lib.ref_set_inlet(self.h, 1 if inlet == "hecht-harting" else 0)
         lib.ref_init_feq(self.h)
 
     def __del__(self):

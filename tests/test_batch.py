@@ -163,7 +163,12 @@ def test_autotune_restores_the_initial_state(pkg, gpu, tmp_path):
     sweep = bm.BatchSweep(cfgs, steps=64)
     times = sweep.autotune(steps=2, cache=str(tmp_path / "tune.json"))
     assert set(times) == {f"{b}/{g}" for b, g in bm.CANDIDATES}
-    sweep.step(5)
+    # one step after tuning (which took three per candidate): the forces of the
+    # tuning's last step lie in another box and must be gone from fx, fy
+    sweep.step(1)
+    seq, _ = bm.sequential(cfgs, 1)
+    _assert_same(list(zip(sweep.lbms, sweep.ibms)), seq)
+    sweep.step(4)
     seq, _ = bm.sequential(cfgs, 5)
     _assert_same(list(zip(sweep.lbms, sweep.ibms)), seq)
     # the cached choice is reused

@@ -1,0 +1,216 @@
+"""GPU tests at the configured workloads (each configuration of the build has
+one here or in test_gpu_fullsize.py).
+
+* The 300 x 100 x 24 case (chord 40, tau 0.53), 2,000 coupled steps of the
+  reference's loop (main_moving.py:116-122) against the C restatement of the
+  reference step (oracle/clbm_ref.c, float32, OpenMP), for the reference's
+  inlet rule and for the opt-in Hecht-Harting inlet.  Compared: the C_D / C_L
+  histories (drag and lift coefficients of the FP64 marker force sums).  Two
+  float32 evaluations of the same formulas in different orders drift apart
+  over a run; the tolerances below are stated in the test and the FP64 error
+  of the float32 run (against float64 storage and arithmetic of the same
+  kernels) is printed beside them.
+* 16-bit deviation storage at 600 x 200 x 50 (f16c, f16, bf16 with inflow
+  deviations), 50 coupled steps against float64 storage (within 4 roundings
+  of the largest deviation in the format's mantissa), and f16c at
+  1200 x 400 x 100 (tau 0.62) for a few steps, plus its mass conservation with
+  x periodic.
+* The batched sweep on its configured draw: 64 seeded instances over the four
+  configured grids, graph-replayed; one instance of each grid size equals its
+  sequential run bit for bit, and every instance stays finite.
+"""
+
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, load_sub
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ref_c():
+    so = os.path.join(ROOT, "oracle", "libclbm_ref.so")
+    if not os.path.exists(so):
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle")], check=True,
+                       stdout=subprocess.DEVNULL)
+    import clbm_ref_c
+    return clbm_ref_c
+
+
+def _gpu_run(pkg, cfg, steps, storage="f32", inlet=True, gpu=None):
+    lbm = pkg.CLBMSolver(cfg, storage=storage, inlet=inlet, device=gpu)
+    ibm = pkg.DroneMovingIBM(cfg, device=gpu)
+    for n in range(1, steps + 1):
+        fx, fy, fz = ibm.compute_force(lbm.ux, lbm.uy, *cfg.motion_state(n))
+        ibm.record_force()
+        lbm.step(fx, fy, fz)
+    rho_min, finite = lbm.health()
+    return ibm, ibm.force_history(), rho_min, finite
+
+
+# ---------------------------------------------------------------------------
+# 2,000 coupled steps of the 300 x 100 x 24 case
+# ---------------------------------------------------------------------------
+END_STEPS = 2000
+WINDOW = 1000          # means over the last 1,000 steps
+# FP32 tolerances (GPU float32 vs the C restatement's float32): the mean
+# C_D / C_L over the window within 1e-4 relative, every step's coefficients
+# within 1e-4 of the coefficient scale (max |C| of the run).  Measured on
+# MI355X: 3e-6 to 8e-6 for both, the same size as the float32 run's own
+# distance from float64 storage and arithmetic.
+MEAN_RTOL = 1e-4
+STEP_TOL = 1e-4
+
+
+@pytest.mark.parametrize("inlet", [True, "hecht-harting"])
+def test_c0_two_thousand_coupled_steps_match_the_c_restatement(pkg, gpu, ref_c, inlet):
+    cfg = pkg.config.case_c0()
+    assert (cfg.Nx, cfg.Ny, cfg.Nz) == (300, 100, 24) and abs(cfg.tau - 0.53) < 1e-12
+    sim = ref_c.RefSim(cfg, threads=min(16, len(os.sched_getaffinity(0))), inlet=inlet)
+    hist_c = sim.run(END_STEPS, record=True)
+    ibm, h32, rho32, fin32 = _gpu_run(pkg, cfg, END_STEPS, "f32", inlet, gpu)
+    _, h64, rho64, fin64 = _gpu_run(pkg, cfg, END_STEPS, "f64", inlet, gpu)
+    assert fin32 and fin64 and rho32 > 0.9
+    cd = {k: ibm.coefficients(h)[0] for k, h in (("c", hist_c), ("32", h32), ("64", h64))}
+    cl = {k: ibm.coefficients(h)[1] for k, h in (("c", hist_c), ("32", h32), ("64", h64))}
+    w = slice(END_STEPS - WINDOW, END_STEPS)
+    rows = []
+    for name, c in (("C_D", cd), ("C_L", cl)):
+        m = {k: float(np.mean(v[w])) for k, v in c.items()}
+        scale = float(np.abs(c["64"]).max())
+        rel_c = abs(m["32"] - m["c"]) / abs(m["c"])
+        rel_64 = abs(m["32"] - m["64"]) / abs(m["64"])
+        step_c = float(np.abs(c["32"] - c["c"]).max()) / scale
+        step_64 = float(np.abs(c["32"] - c["64"]).max()) / scale
+        rows.append(f"{name}: mean gpu32 {m['32']:.6f} c32 {m['c']:.6f} gpu64 {m['64']:.6f} | "
+                    f"mean rel gpu32-c32 {rel_c:.2e}, gpu32-gpu64 {rel_64:.2e} | "
+                    f"max step |gpu32-c32|/scale {step_c:.2e}, |gpu32-gpu64|/scale {step_64:.2e}")
+        assert rel_c < MEAN_RTOL, rows[-1]
+        assert step_c < STEP_TOL, rows[-1]
+        assert rel_64 < MEAN_RTOL, rows[-1]
+    print(f"\n[c0 {END_STEPS} steps, inlet {inlet}] " + "\n  ".join(rows))
+
+
+# ---------------------------------------------------------------------------
+# 16-bit deviation storage at size
+# ---------------------------------------------------------------------------
+MANTISSA = {"f16": 10, "bf16": 7, "f16c": 11}
+# after 50 coupled steps: populations within 4 roundings of the largest
+# deviation in the format's mantissa of the float64-storage run, force sums
+# within 1e-4 (f16c, f16) / 1e-3 (bf16) of the force scale (measured on
+# MI355X: 5.2e-6 / 8.6e-6 / 6.8e-5 on populations against bounds of 3.2e-5 /
+# 6.3e-5 / 5.1e-4, and 3e-6 / 7e-6 / 5e-5 on force sums)
+C2_STEPS = 50
+SUM_TOL = {"f16c": 1e-4, "f16": 1e-4, "bf16": 1e-3}
+
+
+def _coupled_state(pkg, cfg, steps, storage, deviation="inflow", inlet=True):
+    lbm = pkg.CLBMSolver(cfg, storage=storage, deviation=deviation, inlet=inlet)
+    ibm = pkg.DroneMovingIBM(cfg)
+    for n in range(1, steps + 1):
+        fx, fy, fz = ibm.compute_force(lbm.ux, lbm.uy, *cfg.motion_state(n))
+        ibm.record_force()
+        lbm.step(fx, fy, fz)
+    return lbm, ibm
+
+
+@pytest.fixture(scope="module")
+def c1_f64(pkg, gpu):
+    cfg = pkg.config.case_c1()
+    lbm, ibm = _coupled_state(pkg, cfg, C2_STEPS, "f64")
+    out = (lbm.f.to_numpy(), ibm.force_history())
+    del lbm, ibm
+    return out
+
+
+@pytest.mark.parametrize("storage", ["f16c", "f16", "bf16"])
+def test_16bit_storage_at_600x200x50_tracks_fp64(pkg, gpu, oracle, c1_f64, storage):
+    import torch
+    cfg = pkg.config.case_c1()
+    f64, h64 = c1_f64
+    lbm, ibm = _coupled_state(pkg, cfg, C2_STEPS, storage)
+    f16 = lbm.f.to_numpy()
+    assert lbm.health()[1]
+    base = oracle.init_feq(1, 1, 1, cfg.ux_in, cfg.uy_in)[0][0, 0, 0]
+    dev = float(np.abs(f64 - base).max())
+    err = float(np.abs(f16 - f64).max())
+    tol = dev * 2.0 ** -MANTISSA[storage] * 4
+    scale = float(np.abs(h64).max())
+    serr = float(np.abs(ibm.force_history() - h64).max()) / scale
+    print(f"\n[{storage} 600x200x50, {C2_STEPS} steps] max |f - f64| {err:.3e} "
+          f"(bound {tol:.3e}, largest deviation {dev:.3e}); force sums {serr:.2e} of scale")
+    assert err < tol
+    assert serr < SUM_TOL[storage]
+    del lbm, ibm, f16
+    torch.cuda.empty_cache()
+
+
+def test_f16c_at_1200x400x100_tracks_fp64_and_conserves_mass(pkg, gpu, oracle):
+    import torch
+    cfg = pkg.config.case_c2_large()
+    assert (cfg.Nx, cfg.Ny, cfg.Nz) == (1200, 400, 100) and abs(cfg.tau - 0.62) < 1e-12
+    steps = 4
+    a, ai = _coupled_state(pkg, cfg, steps, "f64")
+    f64 = a.f.to_numpy()
+    h64 = ai.force_history()
+    del a, ai
+    torch.cuda.empty_cache()
+    b, bi = _coupled_state(pkg, cfg, steps, "f16c")
+    f16 = b.f.to_numpy()
+    base = oracle.init_feq(1, 1, 1, cfg.ux_in, cfg.uy_in)[0][0, 0, 0]
+    dev = float(np.abs(f64 - base).max())
+    err = float(np.abs(f16 - f64).max())
+    print(f"\n[f16c 1200x400x100, {steps} steps] max |f - f64| {err:.3e} (largest deviation "
+          f"{dev:.3e})")
+    assert err < dev * 2.0 ** -MANTISSA["f16c"] * 4
+    np.testing.assert_allclose(bi.force_history(), h64, rtol=0,
+                               atol=1e-3 * float(np.abs(h64).max()))
+    del b, bi, f16, f64
+    torch.cuda.empty_cache()
+    # x periodic: the body's Guo forcing adds no mass; the 16-bit rounding of
+    # every deviation, unbiased, keeps the total to 1e-6 relative over 20 steps
+    lbm = pkg.CLBMSolver(cfg, storage="f16c", deviation="inflow", inlet=False)
+    ibm = pkg.DroneMovingIBM(cfg)
+    m0 = float(lbm.f.to_numpy().sum(dtype=np.float64))
+    for n in range(1, 21):
+        fx, fy, fz = ibm.compute_force(lbm.ux, lbm.uy, *cfg.motion_state(n))
+        lbm.step(fx, fy, fz)
+    m1 = float(lbm.f.to_numpy().sum(dtype=np.float64))
+    print(f"[f16c 1200x400x100 periodic] mass drift {abs(m1 - m0) / m0:.2e} over 20 steps")
+    assert float(np.abs(ibm.FX_k.to_numpy()).max()) > 0
+    assert abs(m1 - m0) / m0 < 1e-6
+
+
+# ---------------------------------------------------------------------------
+# the configured batched sweep
+# ---------------------------------------------------------------------------
+def test_configured_64_instance_sweep(pkg, gpu):
+    import torch
+    bm = load_sub("batch")
+    cfgs = bm.draw_instances(64, seed=0)
+    sizes = sorted({(c.Nx, c.Ny, c.Nz) for c in cfgs})
+    assert sizes == sorted(bm.GRIDS)
+    steps = 10
+    sweep = bm.BatchSweep(cfgs, steps=steps + 2)
+    sweep.run(steps, graph=True)
+    assert all(l.steps == steps for l in sweep.lbms)
+    for l in sweep.lbms:
+        rho_min, finite = l.health()
+        assert finite and rho_min > 0.9
+    # one instance of each grid size equals its own sequential run
+    pick = [next(i for i, c in enumerate(cfgs) if (c.Nx, c.Ny, c.Nz) == s) for s in sizes]
+    got = [(sweep.lbms[i].f.to_numpy(), sweep.ibms[i].FX_k.to_numpy(),
+            sweep.ibms[i].fx.to_numpy()) for i in pick]
+    solo = [sum(g.solo for g in sweep.groups), len(sweep.groups)]
+    del sweep
+    torch.cuda.empty_cache()
+    seq, _ = bm.sequential([cfgs[i] for i in pick], steps)
+    for (f, FX, fx), (lbm, ibm) in zip(got, seq):
+        np.testing.assert_array_equal(f, lbm.f.to_numpy())
+        np.testing.assert_array_equal(FX, ibm.FX_k.to_numpy())
+        np.testing.assert_array_equal(fx, ibm.fx.to_numpy())
+    print(f"\n[64-instance sweep] {solo[1]} groups ({solo[0]} solo), grids {sizes}")

@@ -1,7 +1,8 @@
 """Parity at full size, where the NumPy oracle would take minutes per step: the
 benchmarked 600 x 200 x 50 case (chord 80, 8,200 markers, tau 0.56), the
 reference's own airplane case (600 x 200 x 160, 74,201 markers, inflow tilted
-by the mean angle of attack) and its fish case (800 x 250 x 25, 7,700 markers).
+by the mean angle of attack), its high-Re airplane (800 x 260 x 220, 166,765
+markers, Re 1000, tau 0.518) and its fish case (800 x 250 x 25, 7,700 markers).
 
 * Against the C restatement of the reference's step (oracle/clbm_ref.c,
   float32, OpenMP): marker positions and stencil base indices bit for bit
@@ -48,6 +49,9 @@ def ref_c():
     # the reference's own airplane (config_moving.py): tilted inflow, finite span, interior fill
     ("reference_airplane", (600, 200, 160, 74201)),
     ("reference_fish", (800, 250, 25, 7700)),         # fish/config_fish.py
+    # the high-Re airplane of main_highRe.py:26-27 (config_highRe.py:33-44): x1.5, Re 1000,
+    # tau 0.518
+    ("reference_highre", (800, 260, 220, 166765)),
 ])
 def test_full_size_coupled_steps_match_the_c_restatement(pkg, gpu, ref_c, case, size):
     cfg = getattr(pkg.config, case)()

@@ -44,6 +44,25 @@ def test_lbm_step_bit_exact(oracle, ref_c):
     assert np.array_equal(sim.f(), g)
 
 
+def test_hecht_harting_lbm_step_bit_exact(oracle, ref_c):
+    """The C restatement's opt-in Hecht-Harting inlet is the NumPy oracle's
+    float32 arithmetic, bit for bit (and differs from the reference's rule)."""
+    cfg = _case()
+    sim = ref_c.RefSim(cfg, threads=2, inlet="hecht-harting")
+    rng = np.random.default_rng(4)
+    f, *_ = oracle.init_feq(cfg.Nx, cfg.Ny, cfg.Nz, cfg.ux_in, cfg.uy_in, dtype=np.float32)
+    f = (f * (1 + 0.02 * rng.standard_normal(f.shape))).astype(np.float32)
+    sim.set_f(f)
+    ref_c._load().ref_step(sim.h)
+    z = np.zeros(f.shape[:3], np.float32)
+    g, *_ = oracle.lbm_step(f, z, z, z, np.float32(cfg.s_v), np.float32(cfg.s_e), cfg.ux_in,
+                            cfg.uy_in, inlet="hecht-harting")
+    assert np.array_equal(sim.f(), g)
+    h, *_ = oracle.lbm_step(f, z, z, z, np.float32(cfg.s_v), np.float32(cfg.s_e), cfg.ux_in,
+                            cfg.uy_in)
+    assert not np.array_equal(sim.f(), h)
+
+
 def test_init_and_coupled_steps_close(oracle, ref_c):
     cfg = _case()
     sim = ref_c.RefSim(cfg, threads=1)

@@ -268,3 +268,50 @@ def test_overlap_splits_the_benchmark_shape(pkg, gpu):
     lbm, ibm, split = _coupled(pkg, cfg, 6, True)
     assert split == 6
     assert ibm.region_violations() == 0
+
+
+@pytest.mark.gpu
+def test_region_violation_stops_the_run_at_the_health_check(pkg, gpu, monkeypatch):
+    """A predicted region that misses part of the stencils is reported by the
+    run loop's health check (ClbmError naming the generation), with no
+    synchronisation per step.  Here the marker kernel checks a region that is
+    too small while the solver is kept off the split path, so that no outside
+    part runs beside the marker kernels (the check itself is what is tested)."""
+    am = load_sub("airfoil_moving")
+    cm = load_sub("clbm_moving")
+    cfg = pkg.config.naca_case(96, 48, 4, 12.0, name="violation")
+    lbm = pkg.CLBMSolver(cfg)
+    ibm = pkg.DroneMovingIBM(cfg, overlap=True)
+    true_region = am.DroneMovingIBM._region
+
+    def shrunk(self, solver, xc, yc, theta):
+        r = true_region(self, solver, xc, yc, theta)
+        if r is None:
+            return None
+        r = list(r)
+        r[3] = r[0] + 2                 # three x planes: the stencils reach further
+        return (C.c_int32 * 6)(*r)
+
+    def one_ibm_split(self, fx, fy, fz):  # the IBM is remembered, no split runs
+        if len(self._readers) == 1:
+            self._split_ibms.add(next(iter(self._readers.values()))[0])
+        return None
+
+    monkeypatch.setattr(am.DroneMovingIBM, "_region", shrunk)
+    monkeypatch.setattr(cm.CLBMSolver, "_split_source", one_ibm_split)
+    for n in range(1, 4):
+        fx, fy, fz = ibm.compute_force(lbm.ux, lbm.uy, *cfg.motion_state(n))
+        lbm.step(fx, fy, fz)
+        assert lbm.last_kernel_cells == lbm.ncell          # one full step, one stream
+    with pytest.raises(load_sub("_lib").ClbmError, match="generation 3"):
+        lbm.health()
+    assert ibm.region_violations() == 3
+
+
+@pytest.mark.gpu
+def test_health_passes_when_the_split_regions_hold(pkg, gpu):
+    cfg = pkg.config.naca_case(96, 48, 4, 12.0, name="holds")
+    lbm, ibm, split = _coupled(pkg, cfg, 5, True)
+    assert split == 5 and ibm in lbm._split_ibms
+    rho_min, finite = lbm.health()
+    assert finite and rho_min > 0.9
