@@ -73,15 +73,28 @@ class FrameWriter:
     reference's (Nx, Ny, Nz) arrays and writes the .npz, while the GPU steps
     on.  A slot is reused only after its file is written, so a writer that
     falls behind holds the loop back instead of dropping frames.  ``depth``
-    slots (default: one per two usable cores, 2 to 8) each hold a frame on
-    the device and in pinned host memory."""
+    slots each hold a frame (4 float32 fields, 16 bytes a cell) on the device
+    and in pinned host memory.  By default: one per two usable cores, 2 to 8,
+    and no more than fit in ``pinned_budget`` bytes of page-locked host
+    memory (1 GiB; at least one slot): 8 slots at 600 x 200 x 50 (96 MB a
+    frame), 1 at the 800 x 260 x 220 airplane (732 MB a frame)."""
 
-    def __init__(self, lbm, depth=None, compress=1):
+    PINNED_BUDGET = 1 << 30
+
+    @staticmethod
+    def default_depth(ncell: int, cores: int, budget: int) -> int:
+        """Compression takes about 3 s of one core per 96 MB frame: one slot
+        (and one writer thread) per two cores, from 2 to 8, and no more slots
+        than fit in ``budget`` bytes of pinned memory (at least one)."""
+        depth = min(8, max(2, cores // 2))
+        return max(1, min(depth, budget // (16 * ncell)))
+
+    def __init__(self, lbm, depth=None, compress=1, pinned_budget=None):
         import torch
         if depth is None:
-            # compression takes about 3 s of one core per 96 MB frame: one slot
-            # (and one writer thread) per two usable cores, from 2 to 8
-            depth = min(8, max(2, len(os.sched_getaffinity(0)) // 2))
+            budget = self.PINNED_BUDGET if pinned_budget is None else int(pinned_budget)
+            depth = self.default_depth(lbm.Nx * lbm.Ny * lbm.Nz, len(os.sched_getaffinity(0)),
+                                       budget)
         self.lbm = lbm
         self.compress = compress
         shape = (4, lbm.Nz, lbm.Ny, lbm.Nx)

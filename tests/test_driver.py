@@ -96,3 +96,47 @@ def test_async_frames_equal_the_synchronous_ones(pkg, gpu, tmp_path, compress):
     # consecutive frames differ: each holds its own step's fields
     f1, f2 = np.load(tmp_path / "a" / "frame_000001.npz"), np.load(tmp_path / "a" / "frame_000002.npz")
     assert not np.array_equal(f1["ux"], f2["ux"])
+
+
+def test_main_maps_the_inlet_flag(pkg, monkeypatch):
+    """--inlet reaches run(): "reference" as the default rule (True), and
+    "hecht-harting" as itself."""
+    drv = load_sub("driver")
+    seen = []
+    monkeypatch.setattr(drv, "run", lambda *a, **k: seen.append(k["inlet"]) or {"diverged": False})
+    assert drv.main(["--case", "c0", "--steps", "1", "--no-frames"]) == 0
+    assert drv.main(["--case", "c0", "--steps", "1", "--no-frames", "--inlet", "hecht-harting"]) == 0
+    assert seen == [True, "hecht-harting"]
+
+
+@pytest.mark.gpu
+def test_driver_hecht_harting_inlet_is_the_solver_option(pkg, gpu, tmp_path):
+    """run(inlet="hecht-harting") steps a CLBMSolver(inlet="hecht-harting"):
+    its force history equals a hand loop with that solver, bit for bit, and
+    differs from the reference rule's."""
+    drv = load_sub("driver")
+    cfg = pkg.config.naca_case(40, 24, 4, 8.0, le_x=3.0, name="drv-hh", N_steps=5)
+    res = drv.run(cfg, 5, str(tmp_path), write_frames=False, verbose=False, inlet="hecht-harting")
+    lbm = pkg.CLBMSolver(cfg, inlet="hecht-harting")
+    ibm = pkg.DroneMovingIBM(cfg)
+    for n in range(1, 6):
+        fx, fy, fz = ibm.compute_force(lbm.ux, lbm.uy, *cfg.motion_state(n))
+        lbm.step(fx, fy, fz)
+        ibm.record_force()
+    np.testing.assert_array_equal(res["history"], ibm.force_history())
+    ref = drv.run(cfg, 5, str(tmp_path / "ref"), write_frames=False, verbose=False)
+    assert not np.array_equal(ref["history"], res["history"])
+
+
+def test_frame_writer_depth_respects_the_pinned_budget(pkg):
+    """The default number of snapshot slots: one per two cores (2 to 8),
+    capped by the pinned host memory a frame needs (16 bytes a cell), at
+    least one."""
+    drv = load_sub("driver")
+    d = drv.FrameWriter.default_depth
+    assert d(600 * 200 * 50, cores=16, budget=1 << 30) == 8       # 96 MB frames
+    assert d(600 * 200 * 50, cores=6, budget=1 << 30) == 3
+    assert d(600 * 200 * 50, cores=1, budget=1 << 30) == 2
+    assert d(800 * 260 * 220, cores=16, budget=1 << 30) == 1      # 732 MB frames
+    assert d(800 * 260 * 220, cores=16, budget=4 << 30) == 5
+    assert d(1200 * 400 * 100, cores=16, budget=1 << 30) == 1     # over budget: still one
