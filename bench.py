@@ -57,6 +57,10 @@ def parse(argv=None):
                     help="skip the FP16/BF16-storage runs of the same workload")
     ap.add_argument("--profile-only", action="store_true",
                     help="run warmup + timed steps only (for rocprofv3)")
+    ap.add_argument("--coupled", default=None, metavar="IL,SPREAD,SPINS",
+                    help="step through coupled.step_coupled (markers and lattice in one launch "
+                         "on one stream) with these interleave, inside-spread and wait-spins "
+                         "values instead of the two-stream split step")
     return ap.parse_args(argv)
 
 
@@ -164,12 +168,16 @@ def stream_probe(pkg, cfg, storage, steps, warmup, quiet=False):
     return 0
 
 
-def measure(pkg, cfg, storage, steps, warmup, dev, tune, deviation="weights"):
+def measure(pkg, cfg, storage, steps, warmup, dev, tune, deviation="weights", coupled=None):
     """Warm-up, then ``steps`` timed coupled steps (compute_force + step) of a
-    fresh solver/IBM pair with ``storage`` populations."""
+    fresh solver/IBM pair with ``storage`` populations.  ``coupled``: a
+    (interleave, inside_spread, wait_spins) triple to step through
+    coupled.step_coupled instead (its two kernels timed as the step kernel)."""
     import torch
     lbm = pkg.CLBMSolver(cfg, storage=storage, device=dev, deviation=deviation, **tune["lbm"])
     ibm = pkg.DroneMovingIBM(cfg, device=dev, **tune["ibm"])
+    if coupled is not None:
+        return measure_coupled(pkg, cfg, lbm, ibm, steps, warmup, coupled)
     step = 1
     for _ in range(warmup):
         fx, fy, fz = ibm.compute_force(lbm.ux, lbm.uy, *cfg.motion_state(step))
@@ -209,6 +217,44 @@ def measure(pkg, cfg, storage, steps, warmup, dev, tune, deviation="weights"):
     return out
 
 
+def measure_coupled(pkg, cfg, lbm, ibm, steps, warmup, knobs):
+    """measure() through coupled.step_coupled; HIP events bracket its two
+    launches on every 4th timed step."""
+    import torch
+    cm = importlib.import_module(PKG + ".coupled")
+    il, spread, spins = knobs
+    step = 1
+    for _ in range(warmup):
+        cm.step_coupled(lbm, ibm, *cfg.motion_state(step), interleave=il, inside_spread=spread,
+                        wait_spins=spins)
+        step += 1
+    st = torch.cuda.current_stream()
+    ev = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        timed = i % 4 == 0
+        if timed:
+            e = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+            e[0].record(st)
+        cm.step_coupled(lbm, ibm, *cfg.motion_state(step), interleave=il, inside_spread=spread,
+                        wait_spins=spins)
+        if timed:
+            e[1].record(st)
+            ev.append(e)
+        step += 1
+    t_host = time.perf_counter() - t0
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    rho_min, finite = lbm.health()
+    k_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    out = dict(dt=dt, k_ms=k_ms, k_cells=float(lbm.ncell), rho_min=rho_min, finite=finite,
+               host_ms=t_host / steps * 1e3)
+    del lbm, ibm
+    torch.cuda.empty_cache()
+    return out
+
+
 def main(argv=None) -> int:
     a = parse(argv)
     if a.gpus != 1:
@@ -231,7 +277,16 @@ def main(argv=None) -> int:
         tune["lbm"]["outside_lds"] = a.outside_lds
     if a.group is not None:
         tune["ibm"]["group"] = a.group
-    r = measure(pkg, cfg, a.storage, a.steps, a.warmup, dev, tune, deviation=a.deviation)
+    knobs = tuple(int(v) for v in a.coupled.split(",")) if a.coupled else None
+    r = measure(pkg, cfg, a.storage, a.steps, a.warmup, dev, tune, deviation=a.deviation,
+                coupled=knobs)
+    if knobs is not None:
+        print(json.dumps({"coupled": list(knobs), "workload": cfg.name, "storage": a.storage,
+                          "steps": a.steps,
+                          "mlups": round(a.steps * ncell / r["dt"] / 1e6, 1),
+                          "ms_per_step": round(r["dt"] / a.steps * 1e3, 5),
+                          "kernels_ms": round(r["k_ms"], 5), "finite": r["finite"]}))
+        return 0
     if a.profile_only:
         print(json.dumps({"steps": a.steps, "seconds": r["dt"], "finite": r["finite"]}))
         return 0

@@ -20,10 +20,10 @@ ELEM_BYTES = {STORE_F32: 4, STORE_F16: 2, STORE_BF16: 2, STORE_F64: 8, STORE_F16
 SPREAD_FIXED, SPREAD_ATOMIC_F32 = 0, 1
 SPREAD_MODES = {"fixed": SPREAD_FIXED, "atomic": SPREAD_ATOMIC_F32}
 BOX_SUBSLOTS = 64
-BOX_WORDS = 2 * 6 * BOX_SUBSLOTS + 4        # uint64 words of a force-box record
+BOX_WORDS = 2 * 6 * BOX_SUBSLOTS + 16       # uint64 words of a force-box record
 FORCE_BOX_OFFSET = 2 * 6 * BOX_SUBSLOTS     # word offset of the decoded int32[6] box
 REGION_FLAG_OFFSET = 2 * 6 * BOX_SUBSLOTS + 3   # word offset of the region-violation flag
-ABI_VERSION = 7
+ABI_VERSION = 8
 CLOSURE_GRAD_NOALLOC_MAX_BLOCKS = 1024   # rows of the library's own partial-gradient array
 
 
@@ -101,6 +101,9 @@ _SIGNATURES = {
     "ibm_spread": (_I, [_G, _IP, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
     "ibm_clip_forces": (_I, [_G, _IP, _P, _P, _P, _P, _P, C.c_uint32, _P]),
     "ibm_force_sum_f64": (_I, [_I, _P, _P, _P, _P, _P]),
+    "clbm_ibm_step": (_I, [_G, _LP, _P, _I, _IP, _I, _P, _P, _P, _P, _F, _F, _F, _F, _F, _F,
+                           _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, C.c_uint32, _P,
+                           _I, _I, _I, _P]),
     "clbm_batch_bytes": (C.c_size_t, [_I, _G, _I]),
     "clbm_batch_build": (_I, [_I, _G, _LP, _P, _P, _P, _P, _I, _P, C.c_size_t,
                               C.POINTER(ClbmBatch)]),

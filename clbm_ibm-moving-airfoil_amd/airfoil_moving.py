@@ -134,12 +134,14 @@ class DroneMovingIBM:
         self.box = torch.zeros(_lib.BOX_WORDS, dtype=torch.int64, device=dev)
         self.gen = 0
         box_ptr = self.box.data_ptr() + 8 * _lib.FORCE_BOX_OFFSET
+        # True after coupled.step_coupled, which leaves fx, fy unwritten
+        self._fields_stale = False
         self.fx = Field(shape, torch.zeros((self.Nz, self.Ny, self.Nx), dtype=f32, device=dev),
-                        refresh=self._join, owner=self, box=box_ptr)
+                        refresh=self._refresh_fields, owner=self, box=box_ptr)
         self.fy = Field(shape, torch.zeros((self.Nz, self.Ny, self.Nx), dtype=f32, device=dev),
-                        refresh=self._join, owner=self, box=box_ptr)
+                        refresh=self._refresh_fields, owner=self, box=box_ptr)
         self.fz = Field(shape, torch.zeros((self.Nz, self.Ny, self.Nx), dtype=f32, device=dev),
-                        refresh=self._join, owner=self, box=box_ptr)
+                        refresh=self._refresh_fields, owner=self, box=box_ptr)
         self._acc = torch.zeros((2, ncell), dtype=torch.int64, device=dev)
         # what the solver's split step reads: the force fields and their box
         self._force_ptrs = (C.c_void_p(self.fx._t.data_ptr()), C.c_void_p(self.fy._t.data_ptr()),
@@ -183,6 +185,12 @@ class DroneMovingIBM:
         fields (read straight from its populations) or any [Nz,Ny,Nx] float32
         device Field/tensor.  Returns (fx, fy, fz), this object's own fields."""
         import torch
+        if self._fields_stale:
+            # the last coupled step left fx, fy unwritten: clear them so that the
+            # clamp of this call (which clears the last two boxes) leaves them exact
+            self._fields_stale = False
+            self.fx._t.zero_()
+            self.fy._t.zero_()
         pre, post, clip, solver, state = self._call_args(ux, uy)
         region = self._region(solver, xc, yc, theta) if solver is not None else None
         main_h = torch.cuda.current_stream(self.device).cuda_stream
@@ -274,6 +282,29 @@ class DroneMovingIBM:
             _lib.check(_lib.fn("clbm_stream_after")(main_h, self._side_h, self._ev[1]),
                        "clbm_stream_after")
             self._pending = False
+
+    def _refresh_fields(self) -> None:
+        """Before fx, fy, fz are read: order after pending marker kernels and,
+        after a coupled step (which leaves the fields unwritten), rebuild
+        them from the markers: spread of F_k into the accumulators, then the
+        clamp of this generation."""
+        self._join()
+        if not self._fields_stale:
+            return
+        self._fields_stale = False
+        P = C.c_void_p
+        st = self._stream()
+        grid = _lib.ClbmGrid(self.Nx, self.Ny, self.Nz, _lib.STORE_F32)
+        self.fx._t.zero_()
+        self.fy._t.zero_()
+        acc0, acc1 = P(self._acc[0].data_ptr()), P(self._acc[1].data_ptr())
+        _lib.call("ibm_spread", C.byref(grid), C.byref(self.ip), self.N,
+                  P(self.X_marker._t.data_ptr()), P(self.Y_marker._t.data_ptr()),
+                  P(self.Z_marker._t.data_ptr()), P(self.FX_k._t.data_ptr()),
+                  P(self.FY_k._t.data_ptr()), acc0, acc1, st)
+        _lib.call("ibm_clip_forces", C.byref(grid), C.byref(self.ip), acc0, acc1,
+                  P(self.fx._t.data_ptr()), P(self.fy._t.data_ptr()), P(self.box.data_ptr()),
+                  self.gen, st)
 
     @property
     def stencil_base(self):

@@ -65,7 +65,6 @@ int check_state(const clbm_grid_t* g, const clbm_params_t* p, int state, Planes&
   return 0;
 }
 
-constexpr int BLK = 256;
 
 // ---------------------------------------------------------------------------
 // K1: equilibrium initialisation (clbm_moving.py:75-91)
@@ -94,76 +93,10 @@ __global__ __launch_bounds__(BLK) void k_init_feq(Geo g, typename Store<S>::T* _
 // PAR: parity of the incoming state.  FORCE: force fields present.  BOX: the
 // force is read only inside box[0..5].
 // ---------------------------------------------------------------------------
-// One thread of the fused step: t is the thread's index in the launch over
-// this lattice.  REG selects the cells: 0 every cell; 1 the cells outside
-// region rg (one thread per cell of the grid); 2 the cells inside it (one
-// thread per cell of the region).
-template <int S, int PAR, bool FORCE, bool BOX, int REG>
-__device__ __forceinline__ void step_cell(const StepArgs& a, int t) {
-  using R = typename Store<S>::R;
-  const Geo& g = a.g;
-  const Region& rg = a.rg;
-  const Planes& pl = a.pl;
-  int c, x, y, z;
-  if (REG == 2) {
-    if (t >= rg.vol) return;
-    const uint32_t row = fdiv((uint32_t)t, rg.drx);
-    const int iz = (int)fdiv(row, rg.dry);
-    x = rg.lo[0] + (t - (int)row * rg.rx);
-    y = rg.lo[1] + ((int)row - iz * rg.ry);
-    z = rg.lo[2] + iz;
-    c = (z * g.ny + y) * g.nx + x;
-  } else {
-    c = t;
-    if (c >= g.ncell) return;
-    cell_xyz(g, c, x, y, z);
-    if (REG == 1 && in_region(rg, x, y, z)) return;
-  }
-  const Pop<S> pop((typename Store<S>::T*)a.f, g);
-  // With pending planes, cell x = nx-1 collides the outlet copy of cell nx-2's
-  // post-stream populations, but those slots are overwritten by the thread of
-  // x = nx-2 during this pass, so that thread handles both cells (cell nx-1's
-  // own thread stands aside).  When the two cells see the same force (always,
-  // unless the body reaches the outlet), their post-collision populations are
-  // equal and the thread stores its result a second time, into cell nx-1's
-  // slots.  Otherwise it first writes what it loaded into cell nx-1's slots
-  // (the outlet copy; no other thread touches them), and after its own cell
-  // reads them back and collides them with cell nx-1's force.
-  if (pl.on && x == g.nx - 1) return;
-  R fq[Q], Fx, Fy, Fz, Gx, Gy, Gz;
-  const Nbr nb = make_nbr(g, x, y, z);
-  pop_load<PAR>(pop, c, nb, fq);
-  force_at<R, FORCE, BOX>(c, x, y, z, a.fx, a.fy, a.fz, a.box, Fx, Fy, Fz);
-  const bool outlet = pl.on && x == g.nx - 2;
-  // The stores' cell index and the outlet cell's are kept opaque: a thread
-  // stores into the slots it loaded from (in both AA parities), and left to
-  // itself the compiler forms full 64-bit addresses once to share them
-  // between the loads and the stores (and the outlet cell's stores), at a
-  // vector add per population instead of the scalar-base form.
-  int c1 = c + 1;
-  asm("" : "+v"(c1));
-  bool twin = true;
-  if (outlet) {
-    force_at<R, FORCE, BOX>(c1, x + 1, y, z, a.fx, a.fy, a.fz, a.box, Gx, Gy, Gz);
-    twin = (Gx == Fx) & (Gy == Fy) & (Gz == Fz);
-    if (!twin) pop_store_in<PAR>(pop, c1, make_nbr(g, x + 1, y, z), fq);
-  }
-  if (pl.on && x == 0) inlet_rule<R>(fq, R(pl.ux_in), R(pl.uy_in), pl.rule);
-  collide_cell<R, FORCE>(fq, Fx, Fy, Fz, R(a.s_v), R(a.s_e));
-  pop_store<PAR>(pop, c, nb, fq);
-  if (outlet) {
-    const Nbr nb1 = make_nbr(g, x + 1, y, z);
-    if (!twin) {
-      pop_load<PAR>(pop, c1, nb1, fq);
-      collide_cell<R, FORCE>(fq, Gx, Gy, Gz, R(a.s_v), R(a.s_e));
-    }
-    pop_store<PAR>(pop, c1, nb1, fq);
-  }
-}
-
 template <int S, int PAR, bool FORCE, bool BOX, int REG>
 __global__ __launch_bounds__(BLK) void k_step(StepArgs a) {
-  step_cell<S, PAR, FORCE, BOX, REG>(a, sweep_block<PAR>(blockIdx.x, gridDim.x) * BLK + threadIdx.x);
+  step_cell<S, PAR, REG>(a, sweep_block<PAR>(blockIdx.x, gridDim.x) * BLK + threadIdx.x,
+                         FieldForce<FORCE, BOX>{a.fx, a.fy, a.fz, a.box});
 }
 
 // The data movement of one AA step without the collision: every cell's 19
@@ -195,7 +128,8 @@ __global__ __launch_bounds__(BK) void k_step_batch(const StepArgs* __restrict__ 
   const int i = inst[lb];
   StepArgs a = args[i];
   a.pl.on &= planes_on;
-  step_cell<S, PAR, true, true, 0>(a, (lb - first[i]) * BK + threadIdx.x);
+  step_cell<S, PAR, 0>(a, (lb - first[i]) * BK + threadIdx.x,
+                      FieldForce<true, true>{a.fx, a.fy, a.fz, a.box});
 }
 
 // ---------------------------------------------------------------------------

@@ -51,7 +51,7 @@
 extern "C" {
 #endif
 
-#define CLBM_ABI_VERSION 7
+#define CLBM_ABI_VERSION 8
 
 /* population storage formats */
 #define CLBM_STORE_F32 0  /* float32 populations, float32 arithmetic (the drop-in) */
@@ -74,12 +74,17 @@ extern "C" {
 #define CLBM_STATE_PLANES 2
 
 #define IBM_BOX_SUBSLOTS 64
-#define IBM_BOX_WORDS (2 * 6 * IBM_BOX_SUBSLOTS + 4)
+#define IBM_BOX_WORDS (2 * 6 * IBM_BOX_SUBSLOTS + 16)
 /* the decoded int32[6] box inside the uint64 box record */
 #define IBM_FORCE_BOX(box) ((int32_t*)((box) + 2 * 6 * IBM_BOX_SUBSLOTS))
 /* the word of the box record where ibm_compute_force marks (as an atomic
  * maximum of gen) a generation whose stencils left ibm_params_t.region */
 #define IBM_REGION_FLAG(box) ((box) + 2 * 6 * IBM_BOX_SUBSLOTS + 3)
+/* words of the box record used by clbm_ibm_step within one call: the inside
+ * workgroups' generation-tagged decision, and per generation parity eight
+ * uint32 counts of finished marker workgroups */
+#define IBM_STEP_DECISION(box) ((box) + 2 * 6 * IBM_BOX_SUBSLOTS + 4)
+#define IBM_STEP_DONE(box) ((uint32_t*)((box) + 2 * 6 * IBM_BOX_SUBSLOTS + 8))
 
 typedef struct {
   int32_t nx, ny, nz; /* lattice size, Nx >= 3; one population plane (nx*ny*nz elements of
@@ -284,6 +289,37 @@ int ibm_spread(const clbm_grid_t* g, const ibm_params_t* ip, int n, const float*
  * to IBM_FORCE_BOX(box) for clbm_step. */
 int ibm_clip_forces(const clbm_grid_t* g, const ibm_params_t* ip, void* acc_x, void* acc_y,
                     float* fx, float* fy, uint64_t* box, uint32_t gen, void* stream);
+
+/* Replaces the loop body of main_moving.py:116-122 (DroneMovingIBM.
+ * compute_force, airfoil_moving.py:230-255, then CLBMSolver.step,
+ * clbm_moving.py:326-343) for one body on one lattice: the same results as
+ * ibm_compute_force (velocity from f at `state`) + ibm_clip_forces +
+ * clbm_step with the clamped fields, bit for bit, in one launch and a short
+ * tail on one stream.  The marker workgroups and the update of the cells
+ * outside `region` share the launch (the cells the markers read lie inside
+ * it and no update touches another cell's populations); the region's cells
+ * take the force straight from the accumulators, clamped as ibm_clip_forces
+ * would write it, and clear them (so the clamp has no launch of its own).
+ * fx/fy fields are not written (ibm_spread + ibm_clip_forces rebuild them
+ * from the markers when a reader wants them); the box of generation gen goes
+ * to IBM_FORCE_BOX(box).  `region`: normalised by clbm_step_region and
+ * holding every stencil cell (ibm_step_region predicts it; a stencil outside
+ * it marks IBM_REGION_FLAG and then results are not trustworthy).  Needs the
+ * fixed-point spread mode; ip->group and ip->check_region are not used (four
+ * markers per workgroup, the region always checked).  `interleave`: one
+ * super-block of eight marker workgroups every `interleave` super-blocks
+ * from the start of the launch (1: markers first).  `wait_spins`: 0 steps the
+ * region in the tail kernel; > 0 steps it in the launch's last super-blocks
+ * (one every `inside_spread`), after a wait of at most that many polls for the
+ * marker workgroups, else in the tail; < 0 gives up at once (a test of the
+ * tail path).  No reference counterpart as one call. */
+int clbm_ibm_step(const clbm_grid_t* g, const clbm_params_t* p, void* f, int state,
+                  const ibm_params_t* ip, int n, const int32_t* order, const float* X0,
+                  const float* Y0, const float* Z0, float xc, float yc, float theta, float vx,
+                  float vy, float omega, float* X, float* Y, float* Z, float* VX, float* VY,
+                  float* U, float* V, float* FX, float* FY, int32_t* stencil_base, void* acc_x,
+                  void* acc_y, uint64_t* box, uint32_t gen, const int32_t* region,
+                  int interleave, int inside_spread, int wait_spins, void* stream);
 
 /* No reference counterpart (the reference never reduces forces): out2[0..1]
  * = sum_k FX_k dV_k, sum_k FY_k dV_k in float64 with a fixed summation order
