@@ -47,6 +47,10 @@ def parse(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target duration of the CPU baseline sample (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=8)
+    ap.add_argument("--device-warmup", type=float, default=0.25, metavar="SECONDS",
+                    help="before each storage's warm-up steps, keep the GPU busy with device "
+                         "copies for this long (its clocks leave their idle state; nothing of "
+                         "it is timed or counted as a step; 0 = off)")
     ap.add_argument("--group", type=int, default=None,
                     help="markers per workgroup of the marker kernel (4, 8, 16)")
     ap.add_argument("--outside-lds", type=int, default=None,
@@ -168,7 +172,28 @@ def stream_probe(pkg, cfg, storage, steps, warmup, quiet=False):
     return 0
 
 
-def measure(pkg, cfg, storage, steps, warmup, dev, tune, deviation="weights", coupled=None):
+def device_warmup(seconds: float) -> None:
+    """Keep the GPU busy with 256 MiB device-to-device copies for ``seconds``
+    of wall time.  After the idle seconds of a process start or of building
+    the next storage's objects the GPU sits in a low clock state, which a run
+    of a few milliseconds (the 20 steps of a short bench) does not leave: on
+    MI355X the fused step kernel ran 2-3 % slower for hundreds of steps after
+    an idle gap than after 0.2 s of load.  Untimed, not a step."""
+    import torch
+    if seconds <= 0:
+        return
+    a = torch.empty(64 << 20, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        for _ in range(8):
+            b.copy_(a)
+        torch.cuda.synchronize()
+    del a, b
+
+
+def measure(pkg, cfg, storage, steps, warmup, dev, tune, deviation="weights", coupled=None,
+            device_warmup_s=0.0):
     """Warm-up, then ``steps`` timed coupled steps (compute_force + step) of a
     fresh solver/IBM pair with ``storage`` populations.  ``coupled``: a
     (interleave, inside_spread, wait_spins) triple to step through
@@ -176,6 +201,7 @@ def measure(pkg, cfg, storage, steps, warmup, dev, tune, deviation="weights", co
     import torch
     lbm = pkg.CLBMSolver(cfg, storage=storage, device=dev, deviation=deviation, **tune["lbm"])
     ibm = pkg.DroneMovingIBM(cfg, device=dev, **tune["ibm"])
+    device_warmup(device_warmup_s)
     if coupled is not None:
         return measure_coupled(pkg, cfg, lbm, ibm, steps, warmup, coupled)
     step = 1
@@ -191,6 +217,9 @@ def measure(pkg, cfg, storage, steps, warmup, dev, tune, deviation="weights", co
     EVERY = 4
     pool = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
             for _ in range(steps // EVERY + 1)]
+    for e0, e1 in pool:        # created now (a first record), so the step records them itself
+        e0.record()
+        e1.record()
     ev, cells = [], []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -279,7 +308,7 @@ def main(argv=None) -> int:
         tune["ibm"]["group"] = a.group
     knobs = tuple(int(v) for v in a.coupled.split(",")) if a.coupled else None
     r = measure(pkg, cfg, a.storage, a.steps, a.warmup, dev, tune, deviation=a.deviation,
-                coupled=knobs)
+                coupled=knobs, device_warmup_s=a.device_warmup)
     if knobs is not None:
         print(json.dumps({"coupled": list(knobs), "workload": cfg.name, "storage": a.storage,
                           "steps": a.steps,
@@ -299,7 +328,8 @@ def main(argv=None) -> int:
     others = {}
     if a.storage == "f32" and a.with_16bit:
         for st in ("f16c", "f16", "bf16"):
-            o = measure(pkg, cfg, st, a.steps, a.warmup, dev, tune, deviation="inflow")
+            o = measure(pkg, cfg, st, a.steps, a.warmup, dev, tune, deviation="inflow",
+                        device_warmup_s=a.device_warmup)
             m = a.steps * ncell / o["dt"] / 1e6
             others[st] = {"value": round(m, 1), "deviation": "inflow",
                           "ms_per_step": round(o["dt"] / a.steps * 1e3, 5),
@@ -352,6 +382,7 @@ def main(argv=None) -> int:
                      "bytes_per_launch": round(b_alg),
                      "step_frac": round(mlups * 1e6 * 2 * 19 * ELEM[a.storage] / 1e9 / HBM_PEAK_GBS, 4)},
         "host_ms_per_step": round(r["host_ms"], 5),
+        "device_warmup_s": a.device_warmup,
         "ceiling_probe": ceiling,
         "cpu_baseline": cpu,
         "health": {"rho_min": rho_min, "finite": finite},

@@ -43,6 +43,16 @@ class IbmParams(C.Structure):
                 ("region", C.c_int32 * 6)]
 
 
+class IbmSplitPlan(C.Structure):
+    _fields_ = [("grid", ClbmGrid), ("params", ClbmParams), ("ip", IbmParams), ("f", C.c_void_p),
+                ("n", C.c_int32), ("order", C.c_void_p)] + [
+        (k, C.c_void_p) for k in ("X0", "Y0", "Z0", "X", "Y", "Z", "VX", "VY", "U", "V", "FX",
+                                  "FY", "stencil_base", "acc_x", "acc_y", "box", "fx", "fy",
+                                  "hx", "hy")] + [
+        ("nh", C.c_int32), ("z0", C.c_int32), ("z1", C.c_int32), ("side", C.c_void_p),
+        ("fork_ev", C.c_void_p)]
+
+
 class ClbmBatch(C.Structure):
     _fields_ = [("n", C.c_int32), ("storage", C.c_int32), ("block", C.c_int32),
                 ("blocks", C.c_int32), ("args", C.c_void_p), ("inst", C.c_void_p),
@@ -78,6 +88,9 @@ _SIGNATURES = {
     "clbm_step": (_I, [_G, _LP, _P, _I, _P, _P, _P, _P, _P]),
     "clbm_step_part": (_I, [_G, _LP, _P, _I, _P, _P, _P, _P, _P, _I, _P]),
     "clbm_step_region": (_I, [_G, _LP, _P, _P]),
+    "clbm_step_split": (_I, [_G, _LP, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "ibm_compute_force_split": (_I, [C.POINTER(IbmSplitPlan), _I, C.c_uint32, _F, _F, _F, _F, _F,
+                                     _F, _P, _P]),
     "ibm_step_region": (_I, [_G, _LP, _P, _P, _I, _I, _I, C.c_double, C.c_double, C.c_double,
                              _P]),
     "clbm_event_create": (_I, [C.POINTER(C.c_void_p)]),

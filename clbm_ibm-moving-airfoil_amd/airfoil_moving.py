@@ -175,6 +175,11 @@ class DroneMovingIBM:
         self._argcache = None
         self._hist = None
         self._hist_n = 0
+        # with a solver as the velocity source, compute_force is one library call
+        # over a launch plan (False: the calls one by one, as _region et al. do)
+        self.fast_host = True
+        self._plan = None
+        self._plan_solver = None
         self._sums = torch.zeros(2, dtype=torch.float64, device=dev)
 
     # -- reference surface ------------------------------------------------
@@ -192,6 +197,8 @@ class DroneMovingIBM:
             self.fx._t.zero_()
             self.fy._t.zero_()
         pre, post, clip, solver, state = self._call_args(ux, uy)
+        if solver is not None and self.fast_host:
+            return self._compute_force_plan(solver, xc, yc, theta, vx, vy, omega)
         region = self._region(solver, xc, yc, theta) if solver is not None else None
         main_h = torch.cuda.current_stream(self.device).cuda_stream
         if region is not None:
@@ -218,6 +225,70 @@ class DroneMovingIBM:
             self._pending = True
             solver._add_reader(self, region)
         return self.fx, self.fy, self.fz
+
+    def _compute_force_plan(self, solver, xc, yc, theta, vx, vy, omega):
+        """compute_force with the velocity from ``solver``'s populations as one
+        library call (ibm_compute_force_split) over a launch plan built once:
+        the step region, the fork of the marker stream and both launches."""
+        import torch
+        plan = self._plan
+        ip = self.ip
+        if (plan is None or self._plan_solver() is not solver or
+                (plan.ip.f_max, plan.ip.wrap_weight_quirk, plan.ip.spread_mode,
+                 plan.ip.fixed_shift, plan.ip.group) !=
+                (ip.f_max, ip.wrap_weight_quirk, ip.spread_mode, ip.fixed_shift, ip.group)):
+            plan = self._build_plan(solver)
+        if self._pending:
+            self._join()
+        main_h = torch.cuda.current_stream(self.device).cuda_stream
+        self.gen += 1
+        region = self._region_buf
+        rc = self._split_fn(C.byref(plan), solver.state_word, self.gen, float(xc), float(yc),
+                            float(theta), float(vx), float(vy), float(omega), main_h, region)
+        if rc < 0:
+            _lib.check(rc, "ibm_compute_force_split")
+        if rc == 1:
+            self._pending = True
+            solver._add_reader(self, region)
+        return self.fx, self.fy, self.fz
+
+    def _build_plan(self, solver):
+        """The ibm_split_plan_t of this body on ``solver``'s lattice (and the
+        marker stream with its two events when overlapping)."""
+        import torch
+        if (solver.Nx, solver.Ny, solver.Nz) != (self.Nx, self.Ny, self.Nz):
+            raise ValueError("solver and IBM lattices differ")
+        if self.overlap and self.N and self._side is None:
+            self._side = torch.cuda.Stream(device=self.device)
+            self._side_h = C.c_void_p(self._side.cuda_stream)
+            self._ev = [self._new_event(), self._new_event()]
+        pl = _lib.IbmSplitPlan()
+        pl.grid = solver.grid
+        pl.params = solver.params
+        pl.ip = self.ip
+        pl.f = solver.state.data_ptr()
+        pl.n = self.N
+        pl.order = self.order.data_ptr() if self.order is not None else None
+        for name, fld in (("X0", self.X0), ("Y0", self.Y0), ("Z0", self.Z0), ("X", self.X_marker),
+                          ("Y", self.Y_marker), ("Z", self.Z_marker), ("VX", self.VX_body),
+                          ("VY", self.VY_body), ("U", self.U_k), ("V", self.V_k),
+                          ("FX", self.FX_k), ("FY", self.FY_k)):
+            setattr(pl, name, fld._t.data_ptr())
+        pl.stencil_base = (self._stencil_base.data_ptr() if self._stencil_base is not None
+                           else None)
+        pl.acc_x, pl.acc_y = self._acc[0].data_ptr(), self._acc[1].data_ptr()
+        pl.box = self.box.data_ptr()
+        pl.fx, pl.fy = self.fx._t.data_ptr(), self.fy._t.data_ptr()
+        pl.hx, pl.hy = self._hx.ctypes.data, self._hy.ctypes.data
+        pl.nh, pl.z0, pl.z1 = len(self._hx), int(self._zr[0]), int(self._zr[1])
+        on = self.overlap and self.N > 0
+        pl.side = self._side_h.value if on else None
+        pl.fork_ev = self._ev[0].value if on else None
+        self._plan = pl
+        self._plan_solver = weakref.ref(solver)
+        self._region_buf = (C.c_int32 * 6)()
+        self._split_fn = _lib.fn("ibm_compute_force_split")
+        return pl
 
     def _call_args(self, ux, uy):
         """ctypes arguments of ibm_compute_force / ibm_clip_forces around the

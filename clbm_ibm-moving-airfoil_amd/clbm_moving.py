@@ -184,7 +184,21 @@ class CLBMSolver:
         main = torch.cuda.current_stream(self.device)
         st = C.c_void_p(main.cuda_stream)
         state = self._state_ptr
-        if split is not None:
+        if split is not None and (ev is None or (ev[0].cuda_event and ev[1].cuda_event)):
+            # part 0, the join of the marker stream and part 1 in one call
+            ibm, region = split
+            self._split_ibms.add(ibm)
+            fxp, fyp, boxp = ibm._force_ptrs
+            _lib.check(_lib.fn("clbm_step_split")(
+                self._gref, self._pref, state, self.state_word, region, fxp, fyp, boxp, st,
+                ibm._side_h, ibm._ev[1], ev[0].cuda_event if ev is not None else None,
+                ev[1].cuda_event if ev is not None else None), "clbm_step_split")
+            ibm._pending = False
+            self._readers.clear()
+            r = list(region)
+            self.last_kernel_cells = self.ncell - ((r[3] - r[0] + 1) * (r[4] - r[1] + 1)
+                                                   * (r[5] - r[2] + 1))
+        elif split is not None:
             ibm, region = split
             self._split_ibms.add(ibm)
             part = _lib.fn("clbm_step_part")

@@ -749,6 +749,34 @@ int ibm_step_region(const clbm_grid_t* g, const clbm_params_t* p, const float* h
   return 2 * vol > ncell ? 0 : 1;  // a region over half the lattice does not pay
 }
 
+int ibm_compute_force_split(ibm_split_plan_t* plan, int state, uint32_t gen, float xc, float yc,
+                            float theta, float vx, float vy, float omega, void* main,
+                            int32_t* region_out) {
+  if (!plan || !region_out) return fail(-1, "ibm_compute_force_split: NULL argument");
+  ibm_split_plan_t& P = *plan;
+  int split = ibm_step_region(&P.grid, &P.params, P.hx, P.hy, P.nh, P.z0, P.z1, (double)xc,
+                              (double)yc, (double)theta, region_out);
+  if (split < 0) return split;
+  if (split && (!P.side || !P.fork_ev)) split = 0;
+  void* st = main;
+  if (split) {
+    if (int e = clbm_stream_after(P.side, main, P.fork_ev)) return e;
+    st = P.side;
+    P.ip.check_region = 1;
+    for (int i = 0; i < 6; ++i) P.ip.region[i] = region_out[i];
+  } else {
+    P.ip.check_region = 0;
+  }
+  if (int e = ibm_compute_force(&P.grid, &P.ip, &P.params, P.f, state, nullptr, nullptr, P.n,
+                                P.order, P.X0, P.Y0, P.Z0, xc, yc, theta, vx, vy, omega, P.X,
+                                P.Y, P.Z, P.VX, P.VY, P.U, P.V, P.FX, P.FY, P.stencil_base,
+                                P.acc_x, P.acc_y, P.box, gen, st))
+    return e;
+  if (int e = ibm_clip_forces(&P.grid, &P.ip, P.acc_x, P.acc_y, P.fx, P.fy, P.box, gen, st))
+    return e;
+  return split;
+}
+
 int ibm_force_sum_f64(int n, const float* FX, const float* FY, const float* dV, double* out2,
                       void* stream) {
   if (n < 0) return fail(-1, "ibm_force_sum_f64: n < 0");

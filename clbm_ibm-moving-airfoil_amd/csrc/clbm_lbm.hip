@@ -604,6 +604,26 @@ int clbm_step_part(const clbm_grid_t* g, const clbm_params_t* p, void* f, int st
                                    (hipStream_t)stream);
 }
 
+int clbm_step_split(const clbm_grid_t* g, const clbm_params_t* p, void* f, int state,
+                    const int32_t* region, const float* fx, const float* fy,
+                    const int32_t* force_box, void* main, void* side, void* join_ev,
+                    void* start_ev, void* end_ev) {
+  if (!join_ev) return fail(-1, "clbm_step_split: NULL join event");
+  hipStream_t m = (hipStream_t)main;
+  if (start_ev) {
+    const hipError_t r = hipEventRecord((hipEvent_t)start_ev, m);
+    if (r != hipSuccess) return fail((int)r, std::string("clbm_step_split: ") + hipGetErrorString(r));
+  }
+  if (int e = clbm_step_part(g, p, f, state, nullptr, nullptr, nullptr, nullptr, region, 0, main))
+    return e;
+  if (end_ev) {
+    const hipError_t r = hipEventRecord((hipEvent_t)end_ev, m);
+    if (r != hipSuccess) return fail((int)r, std::string("clbm_step_split: ") + hipGetErrorString(r));
+  }
+  if (int e = clbm_stream_after(main, side, join_ev)) return e;
+  return clbm_step_part(g, p, f, state, fx, fy, nullptr, force_box, region, 1, main);
+}
+
 int clbm_macro(const clbm_grid_t* g, const clbm_params_t* p, const void* f, int state, float* rho,
                float* ux, float* uy, float* uz, void* stream) {
   if (int e = check_grid(g, false)) return e;

@@ -189,6 +189,17 @@ int ibm_step_region(const clbm_grid_t* g, const clbm_params_t* p, const float* h
                     const float* hy, int nh, int z0, int z1, double xc, double yc, double theta,
                     int32_t* out);
 
+/* clbm_step_part 0, then the marker stream's join, then clbm_step_part 1, as
+ * one call (the split step of a coupled loop whose marker kernels ran on
+ * `side`): part 0 on `main`; `join_ev` recorded on `side` and waited for by
+ * `main`; part 1 on `main` with the force.  start_ev / end_ev (nullable:
+ * timing events) are recorded on `main` around part 0.  No reference
+ * counterpart (the reference runs one queue). */
+int clbm_step_split(const clbm_grid_t* g, const clbm_params_t* p, void* f, int state,
+                    const int32_t* region, const float* fx, const float* fy,
+                    const int32_t* force_box, void* main, void* side, void* join_ev,
+                    void* start_ev, void* end_ev);
+
 /* Host-only stream ordering for running the marker kernels beside the
  * lattice step (no reference counterpart; Taichi runs one queue):
  * clbm_event_create makes a synchronisation-only HIP event, clbm_event_destroy
@@ -251,6 +262,38 @@ int ibm_compute_force(const clbm_grid_t* g, const ibm_params_t* ip, const clbm_p
                       float* Y, float* Z, float* VX, float* VY, float* U, float* V, float* FX,
                       float* FY, int32_t* stencil_base, void* acc_x, void* acc_y, uint64_t* box,
                       uint32_t gen, void* stream);
+
+/* The launch plan of one body on one lattice for ibm_compute_force_split:
+ * what ibm_compute_force + ibm_clip_forces take that stays fixed from call to
+ * call, built once by the caller (hx, hy: host memory). */
+typedef struct {
+  clbm_grid_t grid;
+  clbm_params_t params;
+  ibm_params_t ip;            /* check_region and region are set per call */
+  const void* f;              /* populations (the velocity source) */
+  int32_t n;
+  const int32_t* order;
+  const float *X0, *Y0, *Z0;
+  float *X, *Y, *Z, *VX, *VY, *U, *V, *FX, *FY;
+  int32_t* stencil_base;
+  void *acc_x, *acc_y;
+  uint64_t* box;
+  float *fx, *fy;
+  const float *hx, *hy;       /* template hull vertices (see ibm_step_region) */
+  int32_t nh, z0, z1;
+  void *side, *fork_ev;       /* the marker stream and its fork event */
+} ibm_split_plan_t;
+
+/* Replaces DroneMovingIBM.compute_force (airfoil_moving.py:230-255) in the
+ * split coupled loop, as one call: the step region (ibm_step_region); when
+ * it pays, `side` waits for `main` (fork_ev) and ibm_compute_force (region
+ * checked) + ibm_clip_forces run on `side`, the region goes to region_out
+ * and the call returns 1; otherwise both run on `main` and it returns 0.
+ * Velocity from the populations in state `state`.  < 0 on errors.  No
+ * reference counterpart as one call. */
+int ibm_compute_force_split(ibm_split_plan_t* plan, int state, uint32_t gen, float xc, float yc,
+                            float theta, float vx, float vy, float omega, void* main,
+                            int32_t* region_out);
 
 /* Replaces DroneMovingIBM._update_markers (airfoil_moving.py:100-127) on its
  * own (same arithmetic as in ibm_compute_force).  box (nullable) collects the

@@ -191,9 +191,10 @@ def test_step_part_rejects_unnormalised_region(pkg, gpu):
                  C.c_void_p(a.state.data_ptr()), a.state_word, None, None, None, None, bad, 0, None)
 
 
-def _coupled(pkg, cfg, steps, overlap, spread="fixed", inlet=True):
+def _coupled(pkg, cfg, steps, overlap, spread="fixed", inlet=True, fast_host=True):
     lbm = pkg.CLBMSolver(cfg, inlet=inlet)
     ibm = pkg.DroneMovingIBM(cfg, overlap=overlap, spread_mode=spread)
+    ibm.fast_host = fast_host
     split = 0
     for n in range(1, steps + 1):
         fx, fy, fz = ibm.compute_force(lbm.ux, lbm.uy, *cfg.motion_state(n))
@@ -282,6 +283,7 @@ def test_region_violation_stops_the_run_at_the_health_check(pkg, gpu, monkeypatc
     cfg = pkg.config.naca_case(96, 48, 4, 12.0, name="violation")
     lbm = pkg.CLBMSolver(cfg)
     ibm = pkg.DroneMovingIBM(cfg, overlap=True)
+    ibm.fast_host = False        # the calls one by one, through _region
     true_region = am.DroneMovingIBM._region
 
     def shrunk(self, solver, xc, yc, theta):
@@ -315,3 +317,20 @@ def test_health_passes_when_the_split_regions_hold(pkg, gpu):
     assert split == 5 and ibm in lbm._split_ibms
     rho_min, finite = lbm.health()
     assert finite and rho_min > 0.9
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("overlap", [True, False])
+def test_one_call_host_path_equals_the_calls_one_by_one(pkg, gpu, overlap):
+    """compute_force through ibm_compute_force_split and the split step
+    through clbm_step_split equal the per-call host path (ibm_step_region,
+    the fork, ibm_compute_force, ibm_clip_forces, clbm_step_part x 2), bit for
+    bit, with and without the marker stream."""
+    cfg = pkg.config.naca_case(96, 48, 4, 12.0, le_x=60.0, name="one-call")
+    a_lbm, a_ibm, a_split = _coupled(pkg, cfg, 9, overlap, fast_host=True)
+    b_lbm, b_ibm, b_split = _coupled(pkg, cfg, 9, overlap, fast_host=False)
+    assert a_split == b_split == (9 if overlap else 0)
+    np.testing.assert_array_equal(a_lbm.f.to_numpy(), b_lbm.f.to_numpy())
+    np.testing.assert_array_equal(a_ibm.force_history(), b_ibm.force_history())
+    np.testing.assert_array_equal(a_ibm.fx.to_numpy(), b_ibm.fx.to_numpy())
+    assert a_ibm.region_violations() == 0 and a_ibm.gen == b_ibm.gen == 9
