@@ -25,7 +25,12 @@ namespace clbm {
 
 template <typename R, bool FORCE = true>
 __device__ __forceinline__ void collide_cell(R* f, R Fx, R Fy, R Fz, R s_v, R s_e) {
-  const R half = R(0.5), third = R(1) / R(3), quarter = R(0.25);
+  const R half = R(0.5), third = R(1) / R(3);
+  // 1/4 is no inline constant: kept in a register of its own, so that the
+  // 16-bit formats' f_q - base_q (one FMA with the scalar base) needs no copy
+  // of the base into a vector register per direction
+  R quarter = R(0.25);
+  asm("" : "+v"(quarter));
   // ---- raw moments of the post-stream populations ----
   const R sxy = (f[7] + f[8]) + (f[9] + f[10]);
   const R sxz = (f[11] + f[12]) + (f[13] + f[14]);

@@ -88,14 +88,20 @@ template <> struct Store<CLBM_STORE_BF16> {
 // code, shifted by 12, under the bias 0x78000000, so exponent code 15 lands on
 // float32 exponent 0xff by itself; an ldexp by -128 undoes the bias.  Both
 // directions are branch-free.
-__device__ __forceinline__ uint16_t f16c_encode(float v) {
+// The code in bits 31:16 of the result (bits 15:0 unspecified), for a store
+// of the high half: R = fl32(|v| + 2^-15) - bits(2^-15), rounded at bit 12,
+// is clamped to 0x7ffffff (the codes above 0x7fff: too large, inf, NaN) and
+// shifted up by 4, so that bits 30:16 hold R >> 12 and bit 31 takes v's sign.
+__device__ __forceinline__ uint32_t f16c_encode_hi(float v) {
   const uint32_t b = __float_as_uint(v);
-  const uint32_t ab = b & 0x7fffffffu;
-  const uint32_t t = __float_as_uint(__uint_as_float(ab) + 3.0517578125e-05f);  // + 2^-15
-  const uint32_t r = t + 0x7ffu + ((t >> 12) & 1u);
-  const uint32_t h = min((r - (112u << 23)) >> 12, 0x7fffu);  // too large, inf, NaN -> 0x7fff
-  return (uint16_t)(((b >> 16) & 0x8000u) | h);
+  const uint32_t t = __float_as_uint(fabsf(v) + 3.0517578125e-05f);  // + 2^-15
+  const uint32_t r = min(t + ((t >> 12) & 1u) + (0x7ffu - (112u << 23)), 0x7ffffffu);
+  return (b & 0x80000000u) | (r << 4);
 }
+__device__ __forceinline__ uint16_t f16c_encode(float v) {
+  return (uint16_t)(f16c_encode_hi(v) >> 16);
+}
+__device__ __forceinline__ void f16c_store(uint16_t* p, float v) { *p = f16c_encode(v); }
 // The code is read sign-extended: shifted by 12, its sign copies fill bits
 // 27-31, of which the bias covers 27-30, so the OR gives +-F * 2^128 with the
 // code's sign.  One FMA on its magnitude scales it back and removes the offset
@@ -116,7 +122,7 @@ template <> struct Store<CLBM_STORE_F16C> {
   using R = float;
   static constexpr bool dev = true;
   __device__ static R ld(const T* p, float b) { return b + f16c_decode(*p); }
-  __device__ static void st(T* p, R v, float b) { *p = f16c_encode(v - b); }
+  __device__ static void st(T* p, R v, float b) { f16c_store(p, v - b); }
 };
 
 inline size_t elem_bytes(int storage) {
