@@ -269,10 +269,13 @@ class BatchSweep:
         self.state = self._state_after(k)
         self._advance_objects(k)
 
-    def run(self, k: int, graph: bool = True) -> None:
+    def run(self, k: int, graph: bool = True, times=None) -> None:
         """k steps, group by group; with ``graph``, a group's steps after the
         first go through its captured two-step graph (the inlet/outlet planes
-        are pending from step 2 on, so the state word cycles with period two)."""
+        are pending from step 2 on, so the state word cycles with period two).
+        ``times``: a list that receives (group instance indices, start event,
+        end event) per group, events on the current stream around the group's
+        work (read them after a synchronisation)."""
         import torch
         if not graph:
             self.step(k)
@@ -284,8 +287,14 @@ class BatchSweep:
             k -= 1
         st = C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
         for gr in self.groups:
+            if times is not None:
+                ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+                ev[0].record()
+                times.append((gr.idx, ev[0], ev[1]))
             if gr.solo:
                 self._solo_steps(gr, k)
+                if times is not None:
+                    ev[1].record()
                 continue
             s, left = self.state, k
             if left >= 2 and gr.graph is not None and gr.graph_state != s:
@@ -300,6 +309,8 @@ class BatchSweep:
             while left:
                 s = self._launch_step(gr, s, st)
                 left -= 1
+            if times is not None:
+                ev[1].record()
         self.state = self._state_after(k)
         self._advance_objects(k)
         for i in self._solo:   # the split steps' stencils stayed in their regions
@@ -388,11 +399,11 @@ class BatchSweep:
 
 
 def sequential(cfgs, steps: int, storage: str = "f32", device=None, first_step: int = 1,
-               deviation: str = "weights", inlet=True):
+               deviation: str = "weights", inlet=True, per=None):
     """Each instance on its own (its CLBMSolver and DroneMovingIBM with their
     default per-instance launches: the overlapped split step on the large
     lattices), one after the other; returns the objects and the elapsed
-    seconds."""
+    seconds (``per``: a list that receives each instance's seconds)."""
     import torch
     objs, total = [], 0.0
     for c in cfgs:
@@ -404,7 +415,10 @@ def sequential(cfgs, steps: int, storage: str = "f32", device=None, first_step: 
             fx, fy, fz = ibm.compute_force(lbm.ux, lbm.uy, *c.motion_state(n))
             lbm.step(fx, fy, fz)
         torch.cuda.synchronize()
-        total += time.perf_counter() - t0
+        dt = time.perf_counter() - t0
+        total += dt
+        if per is not None:
+            per.append(dt)
         objs.append((lbm, ibm))
     return objs, total
 
@@ -434,27 +448,56 @@ def main(argv=None) -> int:
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     done = 0
+    group_ms = {}                      # group (instance indices) -> milliseconds in the run
     while done < a.steps:
         k = min(500, a.steps - done)
-        sweep.run(k)
+        times = []
+        sweep.run(k, times=times)
         done += k
         torch.cuda.synchronize()
+        for idx, e0, e1 in times:
+            group_ms[tuple(idx)] = group_ms.get(tuple(idx), 0.0) + e0.elapsed_time(e1)
         print(f"[batch] {done} steps", file=sys.stderr, flush=True)
     dt = time.perf_counter() - t0
+    # the group times cover all steps but the first (an eager step of the
+    # whole batch when the sweep starts); per grid size: cell updates / time
+    by_grid = {}
+    steps_timed = a.steps - 1
+    for idx, ms in group_ms.items():
+        for i in idx:
+            c = cfgs[i]
+            key = f"{c.Nx}x{c.Ny}x{c.Nz}"
+            share = c.Nx * c.Ny * c.Nz / sum(cfgs[j].Nx * cfgs[j].Ny * cfgs[j].Nz for j in idx)
+            cells, sec = by_grid.get(key, (0, 0.0))
+            by_grid[key] = (cells + c.Nx * c.Ny * c.Nz, sec + share * ms / 1e3)
     res = {"instances": len(cfgs), "cells": sweep.cells, "steps": a.steps, "storage": a.storage,
            "deviation": a.deviation,
            "block": sweep.block, "group": sweep.group, "cache_groups": len(sweep.groups),
            "solo_groups": sum(g.solo for g in sweep.groups),
            "cache_mib": a.cache_mib, "tuning_ms_per_step": tune,
            "seconds": dt, "mlups": sweep.cells * a.steps / dt / 1e6,
-           "finite": all(l.health()[1] for l in sweep.lbms)}
+           "finite": all(l.health()[1] for l in sweep.lbms),
+           "by_grid": {k: {"instances": sum(f"{c.Nx}x{c.Ny}x{c.Nz}" == k for c in cfgs),
+                           "mlups": cells * steps_timed / sec / 1e6}
+                       for k, (cells, sec) in by_grid.items()}}
     if a.sequential_steps:
         del sweep
         torch.cuda.empty_cache()
-        _, ts = sequential(cfgs, a.sequential_steps, a.storage, deviation=a.deviation)
+        per = []
+        _, ts = sequential(cfgs, a.sequential_steps, a.storage, deviation=a.deviation, per=per)
         res["sequential_steps"] = a.sequential_steps
         res["sequential_mlups"] = sum(c.Nx * c.Ny * c.Nz for c in cfgs) * a.sequential_steps / ts / 1e6
         res["speedup_vs_sequential"] = res["mlups"] / res["sequential_mlups"]
+        seq = {}
+        for c, t in zip(cfgs, per):
+            key = f"{c.Nx}x{c.Ny}x{c.Nz}"
+            cells, sec = seq.get(key, (0, 0.0))
+            seq[key] = (cells + c.Nx * c.Ny * c.Nz, sec + t)
+        for k, (cells, sec) in seq.items():
+            row = res["by_grid"].setdefault(k, {})
+            row["sequential_mlups"] = cells * a.sequential_steps / sec / 1e6
+            if "mlups" in row:
+                row["speedup"] = row["mlups"] / row["sequential_mlups"]
     text = json.dumps(res, indent=1)
     if a.json:
         with open(a.json, "w") as fh:

@@ -254,6 +254,19 @@ def _central_moments(fT, u, v, w):
     return out
 
 
+def _noneq_moments(fT):
+    """Populations (19, n) -> (features X, non-equilibrium moments D,
+    (rho, u, v, w)): D is the normalised central moments minus their
+    equilibrium values (PQR order); X equals D with X[0] = rho - 1 (the
+    network's input, no force)."""
+    rho = fT.sum(0)
+    u, v, w = (EX @ fT) / rho, (EY @ fT) / rho, (EZ @ fT) / rho
+    D = _central_moments(fT, u, v, w) / rho - FEATURE_EQ[:, None]
+    X = D.copy()
+    X[0] = rho - 1.0
+    return X, D, (rho, u, v, w)
+
+
 def features_and_targets(f_now: np.ndarray, f_next: np.ndarray, mask=None,
                          per_output: bool = False):
     """Features (n, 19), target outputs y = s - 1 (n, 4) and weights from two
@@ -275,12 +288,8 @@ def features_and_targets(f_now: np.ndarray, f_next: np.ndarray, mask=None,
     fT = np.stack([f[..., q][mask] for q in range(19)])
     fsT = np.stack([np.roll(fn[..., q], shift=(-EX[q], -EY[q], -EZ[q]), axis=(0, 1, 2))[mask]
                     for q in range(19)])
-    rho = fT.sum(0)
-    u, v, w = (EX @ fT) / rho, (EY @ fT) / rho, (EZ @ fT) / rho
-    Dpre = _central_moments(fT, u, v, w) / rho - FEATURE_EQ[:, None]
+    X, Dpre, (rho, u, v, w) = _noneq_moments(fT)
     Dpost = _central_moments(fsT, u, v, w) / rho - FEATURE_EQ[:, None]
-    X = Dpre.copy()
-    X[0] = rho - 1.0
     T = np.zeros((4, len(rho)))
     D = np.zeros((4, len(rho)))
     for j, idx in enumerate(GROUPS):
@@ -314,6 +323,72 @@ def one_step_error(cfg, f_now, f_next, net=None, mask=None, device=None):
 
 
 # ---------------------------------------------------------------------------
+# evaluation diagnostics
+# ---------------------------------------------------------------------------
+ORDERS = {2: list(range(4, 10)), 3: list(range(10, 16)), 4: list(range(16, 19))}
+GROUP_NAMES = ["k210,k120", "k201,k021,k102,k012", "k220", "k202,k022"]
+
+
+def diagnostics(f, net=None, mask=None, chunk: int = 1 << 20) -> dict:
+    """What the closure does to populations f (X, Y, Z, 19), over the cells
+    of ``mask``:
+
+    * ``energy``: per order (2, 3, 4) the mean over cells of the summed
+      squares of the normalised non-equilibrium central moments (those the
+      rates of that order relax; order 2 also holds the trace);
+    * with ``net``: ``on_network``, the fraction of cells whose four rates
+      s = 1 + y all lie in [net.lo, net.hi] (the others fall back to s = 1);
+      per rate group the mean and max |s - 1| over all cells (fallback cells
+      count as 0); and ``correction_energy``, the mean summed squares of
+      what the network changes in the post-collision moments, (s - 1) times
+      the pre-collision non-equilibrium moment of each group (the plain
+      cascade relaxes them to equilibrium: post-collision non-equilibrium 0).
+
+    The features are computed as the fused closure step computes them from
+    a cell's loaded populations (no force: far from the body they equal it)."""
+    a = np.asarray(f)
+    flat = a.reshape(-1, 19)
+    cells = np.flatnonzero(np.ones(a.shape[:3], bool) if mask is None else np.asarray(mask).ravel())
+    e = {k: 0.0 for k in ORDERS}
+    n = len(cells)
+    n_ok, corr = 0, 0.0
+    abs_sum, abs_max = np.zeros(4), np.zeros(4)
+    for lo in range(0, n, chunk):
+        fT = flat[cells[lo:lo + chunk]].T.astype(np.float64)
+        X, D, _ = _noneq_moments(fT)
+        for k, idx in ORDERS.items():
+            e[k] += float((D[idx] ** 2).sum())
+        if net is None:
+            continue
+        Y = net.infer(np.ascontiguousarray(X.T)).cpu().numpy().astype(np.float64)
+        S = 1.0 + Y
+        ok = np.all(np.isfinite(S), 1) & np.all((S >= net.lo) & (S <= net.hi), 1)
+        n_ok += int(ok.sum())
+        Ye = np.where(ok[:, None], Y, 0.0)
+        abs_sum += np.abs(Ye).sum(0)
+        abs_max = np.maximum(abs_max, np.abs(Ye).max(0) if len(Ye) else 0.0)
+        for j, idx in enumerate(GROUPS):
+            corr += float(((Ye[:, j] * D[idx]) ** 2).sum())
+    out = {"cells": int(n), "energy": {str(k): v / max(n, 1) for k, v in e.items()}}
+    if net is not None:
+        out["on_network"] = n_ok / max(n, 1)
+        out["mean_abs_s_minus_1"] = {g: float(v / max(n, 1)) for g, v in zip(GROUP_NAMES, abs_sum)}
+        out["max_abs_s_minus_1"] = {g: float(v) for g, v in zip(GROUP_NAMES, abs_max)}
+        out["correction_energy"] = corr / max(n, 1)
+    return out
+
+
+def body_mask(cfg, ibm_box, margin: int = 3):
+    """Cells within ``margin`` of the force box (all z), the complement of the
+    far-field mask inside the x planes."""
+    far = _far_mask(cfg, ibm_box, margin)
+    m = ~far
+    m[:4] = False
+    m[-5:] = False
+    return m
+
+
+# ---------------------------------------------------------------------------
 # the study: refined runs -> data -> training -> evaluation
 # ---------------------------------------------------------------------------
 def case_for(seed: int, coarse, chord: float, Re: float, refine: int = 1):
@@ -341,10 +416,14 @@ def _far_mask(cfg_coarse, ibm_box, margin: int = 3):
 
 
 def fine_pairs(seed: int, coarse, chord: float, Re: float, warm: int, pairs: int, gap: int,
-               storage: str = "f32", device=None, log=None):
+               storage: str = "f32", device=None, log=None, region: str = "far"):
     """Run the 2x case for ``warm`` fine steps, then take ``pairs`` filtered
     (now, one coarse step later) pairs every ``gap`` fine steps.  Returns the
-    pairs and a mask of coarse cells away from the body and the x planes."""
+    pairs and a mask of coarse cells away from the x planes and, with
+    ``region`` "far", away from the body ("all": the body's cells too, whose
+    targets then absorb the immersed boundary's forcing)."""
+    if region not in ("far", "all"):
+        raise ValueError("region must be 'far' or 'all'")
     from .airfoil_moving import DroneMovingIBM
     from .clbm_moving import CLBMSolver
     cf = case_for(seed, coarse, chord, Re, refine=2)
@@ -370,7 +449,8 @@ def fine_pairs(seed: int, coarse, chord: float, Re: float, warm: int, pairs: int
         b = box_filter(lbm.f.to_numpy())
         out.append((a, b))
         coarse_box = tuple(int(v) // 2 for v in box)
-        masks.append(_far_mask(cc, coarse_box))
+        masks.append(_far_mask(cc, coarse_box) if region == "far" else
+                     _far_mask(cc, coarse_box) | body_mask(cc, coarse_box))
         if log:
             log(f"seed {seed}: pair at fine step {n}, finite {lbm.health()[1]}")
     return out, masks
@@ -400,16 +480,18 @@ def run_forces(cfg, steps: int, net=None, storage: str = "f32", device=None, ski
 
 def study(coarse=(300, 100, 24), chord=None, Re=1000.0, train_seeds=range(8), test_seeds=range(100, 104),
           warm=4000, pairs=3, gap=400, samples=600000, epochs=40, eval_steps=4000, device=None,
-          log=None, net=None, apriori=True, aposteriori=True):
+          log=None, net=None, apriori=True, aposteriori=True, train_region: str = "far"):
     """The full pipeline at a given scale; returns (dict of results, network).
     With ``net`` given the training stage is skipped; ``apriori`` and
-    ``aposteriori`` switch the two evaluation stages."""
+    ``aposteriori`` switch the two evaluation stages; ``train_region``: the
+    cells the training data come from (fine_pairs' ``region``)."""
     chord = coarse[1] / 2.5 if chord is None else chord
     log = log or (lambda s: print(f"[closure] {s}", file=sys.stderr, flush=True))
     res = {"coarse": list(coarse), "fine": [2 * v for v in coarse], "chord": chord, "Re": Re}
     if net is None:
+        res["train_region"] = train_region
         net = _train_stage(res, coarse, chord, Re, train_seeds, warm, pairs, gap, samples, epochs,
-                           device, log)
+                           device, log, train_region)
     if apriori:
         _apriori_stage(res, net, coarse, chord, Re, test_seeds, warm, gap, device, log)
     if aposteriori:
@@ -418,11 +500,12 @@ def study(coarse=(300, 100, 24), chord=None, Re=1000.0, train_seeds=range(8), te
 
 
 def _train_stage(res, coarse, chord, Re, train_seeds, warm, pairs, gap, samples, epochs, device,
-                 log):
+                 log, region="far"):
     Xs, Ts, Ws = [], [], []
     rng = np.random.default_rng(0)
     for s in train_seeds:
-        pr, masks = fine_pairs(s, coarse, chord, Re, warm, pairs, gap, device=device, log=log)
+        pr, masks = fine_pairs(s, coarse, chord, Re, warm, pairs, gap, device=device, log=log,
+                               region=region)
         for (a, b), m in zip(pr, masks):
             X, T, W = features_and_targets(a, b, m, per_output=True)
             Xs.append(X)
@@ -489,6 +572,77 @@ def _aposteriori_stage(res, net, coarse, chord, Re, test_seeds, eval_steps, devi
         np.mean([r["fine"][2] / r["closure"][2] for r in post]))
 
 
+def diagnose(net, coarse=(600, 200, 50), chord=None, Re=1000.0, seed: int = 100, warm: int = 4000,
+             gap: int = 400, steps: int = 3000, device=None, log=None) -> dict:
+    """Evaluation diagnostics (``diagnostics``) of a held-out case: on the
+    filtered fine state the network was trained on (far field), and on the
+    coarse run with the network after ``steps`` steps, near the body and in
+    the far field."""
+    from .airfoil_moving import DroneMovingIBM
+    from .clbm_moving import CLBMSolver
+    chord = coarse[1] / 2.5 if chord is None else chord
+    log = log or (lambda s: print(f"[closure] {s}", file=sys.stderr, flush=True))
+    res = {"seed": seed, "coarse": list(coarse), "Re": Re}
+    pr, masks = fine_pairs(seed, coarse, chord, Re, warm, 1, gap, device=device, log=log)
+    res["filtered_fine_far"] = diagnostics(pr[0][0], net, masks[0])
+    log(f"filtered fine: {res['filtered_fine_far']}")
+    cfg = case_for(seed, coarse, chord, Re)
+    lbm = CLBMSolver(cfg, device=device)
+    ibm = DroneMovingIBM(cfg, device=device)
+    lbm.closure = net
+    for n in range(1, steps + 1):
+        fx, fy, fz = ibm.compute_force(lbm.ux, lbm.uy, *cfg.motion_state(n))
+        lbm.step(fx, fy, fz)
+    res["coarse_finite"] = bool(lbm.health()[1])
+    f = lbm.f.to_numpy()
+    box = tuple(int(v) for v in ibm.force_box())
+    res["coarse_far"] = diagnostics(f, net, _far_mask(cfg, box))
+    res["coarse_body"] = diagnostics(f, net, body_mask(cfg, box))
+    log(f"coarse far: {res['coarse_far']}")
+    log(f"coarse body: {res['coarse_body']}")
+    return res
+
+
+def stability_probe(coarse=(600, 200, 50), chord=None, Re_list=(5000.0, 1000.0), steps: int = 10000,
+                    check: int = 100, seed: int = 100, refine=(1, 2), net=None, device=None,
+                    log=None) -> list:
+    """The plain coupled method (or, on the coarse grid, the closure ``net``)
+    at each Re on the coarse grid and its 2x refinement (``steps`` coarse
+    steps, twice as many fine ones): the step at which the divergence guard
+    of main_moving.py:143-145 (non-finite populations or rho_min < 0.3) first
+    trips, checked every ``check`` coarse steps, or None if it never does."""
+    import torch
+    from .airfoil_moving import DroneMovingIBM
+    from .clbm_moving import CLBMSolver
+    chord = coarse[1] / 2.5 if chord is None else chord
+    log = log or (lambda s: print(f"[closure] {s}", file=sys.stderr, flush=True))
+    rows = []
+    for Re in Re_list:
+        for r in refine:
+            cfg = case_for(seed, coarse, chord, Re, refine=r)
+            lbm = CLBMSolver(cfg, device=device)
+            ibm = DroneMovingIBM(cfg, device=device)
+            lbm.closure = net if r == 1 else None
+            t0 = time.perf_counter()
+            diverged, rho_min = None, 1.0
+            for n in range(1, r * steps + 1):
+                fx, fy, fz = ibm.compute_force(lbm.ux, lbm.uy, *cfg.motion_state(n))
+                lbm.step(fx, fy, fz)
+                if n % (r * check) == 0 or n == r * steps:
+                    rho_min, finite = lbm.health()
+                    if not finite or rho_min < 0.3:
+                        diverged = n
+                        break
+            torch.cuda.synchronize()
+            row = dict(Re=Re, refine=r, grid=[cfg.Nx, cfg.Ny, cfg.Nz], chord=cfg.chord,
+                       tau=cfg.tau, closure=lbm.closure is not None, steps=r * steps,
+                       diverged_at=diverged, rho_min=rho_min, seconds=time.perf_counter() - t0)
+            rows.append(row)
+            log(f"probe {row}")
+            del lbm, ibm
+    return rows
+
+
 def main(argv=None) -> int:
     import argparse
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
@@ -504,17 +658,41 @@ def main(argv=None) -> int:
     ap.add_argument("--json", default=None)
     ap.add_argument("--save", default=None, help="write the trained network (.npz)")
     ap.add_argument("--load", default=None, help="use this network (.npz) instead of training one")
-    ap.add_argument("--stage", default="all", choices=["all", "train", "eval"],
+    ap.add_argument("--stage", default="all", choices=["all", "train", "eval", "diagnose", "probe"],
                     help="train: data, training and the a-priori check; eval: the a-posteriori "
-                         "runs only (needs --load)")
+                         "runs only (needs --load); diagnose: the evaluation diagnostics of the "
+                         "first held-out case (needs --load); probe: the stability probe "
+                         "(with --load, the coarse runs use the network)")
+    ap.add_argument("--train-region", default="far", choices=["far", "all"],
+                    help="cells the training data come from: away from the body, or all")
+    ap.add_argument("--probe-re", default="5000,4000,3000,2000,1000")
+    ap.add_argument("--probe-steps", type=int, default=10000)
+    ap.add_argument("--probe-refine", default="1,2", help="grids of the probe: 1 coarse, 2 fine")
     a = ap.parse_args(argv)
     coarse = tuple(int(v) for v in a.coarse.split(","))
-    if a.stage == "eval" and not a.load:
-        ap.error("--stage eval needs --load")
+    if a.stage in ("eval", "diagnose") and not a.load:
+        ap.error(f"--stage {a.stage} needs --load")
     net = ClosureNet.load(a.load) if a.load else None
+    if a.stage in ("diagnose", "probe"):
+        if a.stage == "diagnose":
+            res = diagnose(net, coarse, Re=a.re, seed=100, warm=a.warm, gap=a.gap,
+                           steps=a.eval_steps)
+        else:
+            res = {"probe": stability_probe(coarse, Re_list=[float(v) for v in
+                                                             a.probe_re.split(",")],
+                                            steps=a.probe_steps, net=net,
+                                            refine=[int(v) for v in a.probe_refine.split(",")])}
+        res["stage"] = a.stage
+        text = json.dumps(res, indent=1)
+        if a.json:
+            with open(a.json, "w") as fh:
+                fh.write(text)
+        print(text, flush=True)
+        return 0
     res, net = study(coarse, Re=a.re, train_seeds=range(a.train_seeds),
                      test_seeds=range(100, 100 + a.test_seeds), warm=a.warm, pairs=a.pairs,
                      gap=a.gap, epochs=a.epochs, eval_steps=a.eval_steps, net=net,
+                     train_region=a.train_region,
                      apriori=a.stage != "eval", aposteriori=a.stage != "train")
     res["stage"] = a.stage
     if a.save:

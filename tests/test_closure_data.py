@@ -60,3 +60,42 @@ def test_folded_standardisation_gives_the_same_outputs(oracle):
     y1, _ = oracle.mlp_forward(cl.split_params(p), (X - mean) / std)
     y2, _ = oracle.mlp_forward(cl.split_params(cl.fold_standardization(p, mean, std)), X)
     np.testing.assert_allclose(y1, y2, rtol=1e-10, atol=1e-12)
+
+
+def _rest(shape):
+    """Populations at rest, rho = 1: every non-equilibrium moment is zero."""
+    w = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
+    return np.broadcast_to(w, shape + (19,)).copy()
+
+
+def test_diagnostic_energies_of_known_perturbations():
+    """At rest every order carries zero energy.  Adding d to f(1,1,0) and
+    f(-1,-1,0) and taking it from f(-1,1,0) and f(1,-1,0) changes only k110,
+    by 4d: order-2 energy 16 d^2 in that cell.  Adding d to f(1,1,0) and
+    f(-1,1,0) and taking 2d from f(0,1,0) changes k200, k210 and k220 by 2d
+    each (mass and momentum unchanged): 4 d^2 in each of orders 2, 3, 4."""
+    cl = load_sub("closure")
+    shape = (4, 3, 2)
+    f = _rest(shape)
+    d = cl.diagnostics(f)
+    assert d["cells"] == 24 and all(abs(v) < 1e-30 for v in d["energy"].values())
+    dl = 1e-3
+    g = f.copy()
+    g[1, 1, 0, [7, 10]] += dl
+    g[1, 1, 0, [8, 9]] -= dl
+    e = cl.diagnostics(g)["energy"]
+    np.testing.assert_allclose([e["2"] * 24, e["3"], e["4"]], [16 * dl ** 2, 0, 0], atol=1e-18,
+                               rtol=1e-9)
+    h = f.copy()
+    h[2, 0, 1, [7, 8]] += dl
+    h[2, 0, 1, 3] -= 2 * dl
+    e = cl.diagnostics(h)["energy"]
+    np.testing.assert_allclose([e["2"] * 24, e["3"] * 24, e["4"] * 24], [4 * dl ** 2] * 3,
+                               rtol=1e-9)
+    # the mask selects cells: only the perturbed cell, then none of it
+    m = np.zeros(shape, bool)
+    m[2, 0, 1] = True
+    np.testing.assert_allclose(cl.diagnostics(h, mask=m)["energy"]["3"], 4 * dl ** 2, rtol=1e-9)
+    m[:] = False
+    m[0, 0, 0] = True
+    assert cl.diagnostics(h, mask=m)["energy"]["3"] < 1e-30

@@ -191,3 +191,26 @@ def test_learned_closure_beats_plain_when_the_data_has_one(pkg, oracle, gpu):
     e_plain = cl.one_step_error(cfg, a, b, None)
     e_net = cl.one_step_error(cfg, a, b, net)
     assert e_net < 0.5 * e_plain, (e_plain, e_net)
+
+
+def test_diagnostics_report_the_rates_the_network_gives(pkg, oracle, gpu):
+    """A network with zero weights in its last layer and bias b gives s = 1 + b
+    in every cell: b = 0 is the plain cascade (no correction), b = 0.3 moves
+    every rate by 0.3 and changes the post-collision third/fourth-order
+    moments by 0.3 times their pre-collision values (correction energy
+    0.09 x the order-3 + order-4 energy), and b = 1.5 (s = 2.5, outside
+    [0, 2]) sends every cell to the fallback."""
+    cl = load_sub("closure")
+    f = _state(oracle, (9, 6, 4), 3)
+    for b, on, dev in ((0.0, 1.0, 0.0), (0.3, 1.0, 0.3), (1.5, 0.0, 0.0)):
+        p = cl.init_params(4, out_scale=0.0)
+        p[-4:] = b
+        d = cl.diagnostics(f, cl.ClosureNet(p))
+        assert d["cells"] == 216
+        assert d["on_network"] == on
+        np.testing.assert_allclose(list(d["mean_abs_s_minus_1"].values()), [dev] * 4, atol=1e-6)
+        np.testing.assert_allclose(list(d["max_abs_s_minus_1"].values()), [dev] * 4, atol=1e-6)
+        e = d["energy"]
+        assert e["3"] > 0 and e["4"] > 0
+        np.testing.assert_allclose(d["correction_energy"], dev ** 2 * (e["3"] + e["4"]),
+                                   rtol=1e-5, atol=1e-30)
