@@ -23,7 +23,7 @@ BOX_SUBSLOTS = 64
 BOX_WORDS = 2 * 6 * BOX_SUBSLOTS + 16       # uint64 words of a force-box record
 FORCE_BOX_OFFSET = 2 * 6 * BOX_SUBSLOTS     # word offset of the decoded int32[6] box
 REGION_FLAG_OFFSET = 2 * 6 * BOX_SUBSLOTS + 3   # word offset of the region-violation flag
-ABI_VERSION = 8
+ABI_VERSION = 9
 CLOSURE_GRAD_NOALLOC_MAX_BLOCKS = 1024   # rows of the library's own partial-gradient array
 
 
@@ -56,7 +56,7 @@ class IbmSplitPlan(C.Structure):
 class ClbmBatch(C.Structure):
     _fields_ = [("n", C.c_int32), ("storage", C.c_int32), ("block", C.c_int32),
                 ("blocks", C.c_int32), ("args", C.c_void_p), ("inst", C.c_void_p),
-                ("first", C.c_void_p)]
+                ("first", C.c_void_p), ("lds", C.c_int32)]
 
 
 class IbmBuffers(C.Structure):

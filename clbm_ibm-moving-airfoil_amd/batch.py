@@ -20,7 +20,8 @@ graph of two steps (both AA parities), which is replayed.  Every instance
 computes exactly what it computes when run on its own (fixed-point
 spreading), which ``sequential`` checks and times.
 
-``autotune`` picks the lattice workgroup size and the markers per workgroup
+``autotune`` picks the lattice workgroup size, the markers per workgroup and
+the LDS each lattice workgroup reserves (which caps the workgroups per CU)
 by timing the candidates on the batch itself, then restores the initial
 state.  The choice is cached per batch signature in a JSON file.
 
@@ -46,7 +47,10 @@ from .airfoil_moving import DroneMovingIBM
 from .clbm_moving import CLBMSolver
 
 GRIDS = [(200, 64, 16), (300, 100, 24), (450, 150, 36), (600, 200, 50)]
-CANDIDATES = [(256, 8), (256, 4), (128, 8), (128, 4)]   # (lattice block, markers per group)
+# (lattice block, markers per group, LDS bytes each lattice workgroup reserves: 24 KiB caps a
+# CU at 6 workgroups of 256, 40 KiB at 4)
+CANDIDATES = [(256, 8, 0), (256, 4, 0), (128, 8, 0), (128, 4, 0),
+              (256, 8, 24576), (256, 4, 24576), (256, 8, 40960), (256, 4, 40960)]
 TUNE_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "clbm_amd_batch_tune.json")
 
 
@@ -133,7 +137,8 @@ class BatchSweep:
     def __init__(self, cfgs, storage: str = "f32", device=None, steps: int = 1000,
                  block: int = 256, group: int = 8, first_step: int = 1,
                  cache_bytes: int = 192 << 20, deviation: str = "weights",
-                 solo_min_cells: int = DroneMovingIBM.OVERLAP_MIN_CELLS, inlet=True):
+                 solo_min_cells: int = DroneMovingIBM.OVERLAP_MIN_CELLS, inlet=True,
+                 lanes: int = 1, lds: int = 0):
         import torch
         self.cfgs = list(cfgs)
         self.device = torch.device(device if device is not None else "cuda")
@@ -148,20 +153,25 @@ class BatchSweep:
                      for c in self.cfgs]
         self.steps_done = 0
         self.state = 0
+        if int(lanes) < 1:
+            raise ValueError("lanes must be >= 1")
+        self.lanes = int(lanes)
+        self._lanes = None
         cells = [c.Nx * c.Ny * c.Nz for c in self.cfgs]
+        # the groups in flight together share the cache budget
         self.groups = [_Group(g, solo=len(g) == 1 and cells[g[0]] >= solo_min_cells)
-                       for g in cache_groups(self.cfgs, storage, cache_bytes)]
+                       for g in cache_groups(self.cfgs, storage, cache_bytes // self.lanes)]
         self._solo = set()
         for gr in self.groups:
             if gr.solo:
                 self._solo.add(gr.idx[0])
                 self.ibms[gr.idx[0]].overlap = True
-        self.build(block, group)
+        self.build(block, group, lds)
 
     # -- tables ---------------------------------------------------------------
-    def build(self, block: int, group: int) -> None:
+    def build(self, block: int, group: int, lds: int = 0) -> None:
         import torch
-        self.block, self.group = int(block), int(group)
+        self.block, self.group, self.lds = int(block), int(group), int(lds)
         first = self.ibms[0].ip
         for ibm in self.ibms[1:]:
             if (ibm.ip.f_max, ibm.ip.wrap_weight_quirk, ibm.ip.spread_mode) != \
@@ -192,6 +202,7 @@ class BatchSweep:
         torch.cuda.synchronize(self.device)
         _lib.call("clbm_batch_build", n, grids, params, f, fx, fy, boxes, self.block,
                   C.c_void_p(gr.lwork.data_ptr()), nb, C.byref(gr.lb))
+        gr.lb.lds = self.lds
         bufs = (_lib.IbmBuffers * n)()
         for k, ibm in enumerate(ibms):
             b = bufs[k]
@@ -273,9 +284,12 @@ class BatchSweep:
         """k steps, group by group; with ``graph``, a group's steps after the
         first go through its captured two-step graph (the inlet/outlet planes
         are pending from step 2 on, so the state word cycles with period two).
-        ``times``: a list that receives (group instance indices, start event,
-        end event) per group, events on the current stream around the group's
-        work (read them after a synchronisation)."""
+        With ``lanes`` > 1, that many groups advance at once, each on a stream
+        of its own, their launches interleaved step by step (or pair by pair
+        of graph steps), so that one group's marker kernels run beside
+        another's lattice update.  ``times``: a list that receives (group
+        instance indices, start event, end event) per group, events on the
+        group's stream around its work (read them after a synchronisation)."""
         import torch
         if not graph:
             self.step(k)
@@ -285,36 +299,70 @@ class BatchSweep:
         if self.steps_done == 0 and k > 0:
             self.step(1)
             k -= 1
-        st = C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
-        for gr in self.groups:
-            if times is not None:
-                ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-                ev[0].record()
-                times.append((gr.idx, ev[0], ev[1]))
-            if gr.solo:
-                self._solo_steps(gr, k)
-                if times is not None:
-                    ev[1].record()
-                continue
-            s, left = self.state, k
-            if left >= 2 and gr.graph is not None and gr.graph_state != s:
-                s = self._launch_step(gr, s, st)   # realign with the captured pair
-                left -= 1
-            if left >= 2 and gr.graph is None:
-                self._capture(gr, s)
-            while left >= 2:
-                gr.graph.replay()
-                s = self._next_state(self._next_state(s))
-                left -= 2
-            while left:
-                s = self._launch_step(gr, s, st)
-                left -= 1
-            if times is not None:
-                ev[1].record()
+        main = torch.cuda.current_stream(self.device)
+        if self.lanes > 1 and self._lanes is None:
+            self._lanes = [torch.cuda.Stream(device=self.device) for _ in range(self.lanes)]
+        order = [g for g in self.groups if g.solo] + [g for g in self.groups if not g.solo]
+        for c in range(0, len(order), self.lanes):
+            chunk = order[c:c + self.lanes]
+            streams = [main] if self.lanes == 1 else self._lanes[:len(chunk)]
+            for st in streams:
+                if st is not main:
+                    st.wait_stream(main)
+            work = [self._units(gr, k, st, times) for gr, st in zip(chunk, streams)]
+            while work:
+                for w in list(work):
+                    try:
+                        next(w)
+                    except StopIteration:
+                        work.remove(w)
+            for st in streams:
+                if st is not main:
+                    main.wait_stream(st)
         self.state = self._state_after(k)
         self._advance_objects(k)
         for i in self._solo:   # the split steps' stencils stayed in their regions
             self.ibms[i].check_region()
+
+    def _units(self, gr: "_Group", k: int, stream, times):
+        """Issue a group's k steps on ``stream``, yielding after each unit of
+        work (a step, or a replay of the captured pair of steps)."""
+        import torch
+        if times is not None:
+            ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+            ev[0].record(stream)
+            times.append((gr.idx, ev[0], ev[1]))
+        if gr.solo:
+            i = gr.idx[0]
+            lbm, ibm, cfg = self.lbms[i], self.ibms[i], self.cfgs[i]
+            for j in range(k):
+                with torch.cuda.stream(stream):
+                    fx, fy, fz = ibm.compute_force(
+                        lbm.ux, lbm.uy, *cfg.motion_state(self.first_step + self.steps_done + j))
+                    lbm.step(fx, fy, fz)
+                yield
+        else:
+            st = C.c_void_p(stream.cuda_stream)
+            s, left = self.state, k
+            if left >= 2 and gr.graph is not None and gr.graph_state != s:
+                s = self._launch_step(gr, s, st)   # realign with the captured pair
+                left -= 1
+                yield
+            if left >= 2 and gr.graph is None:
+                with torch.cuda.stream(stream):
+                    self._capture(gr, s)
+            while left >= 2:
+                with torch.cuda.stream(stream):
+                    gr.graph.replay()
+                s = self._next_state(self._next_state(s))
+                left -= 2
+                yield
+            while left:
+                s = self._launch_step(gr, s, st)
+                left -= 1
+                yield
+        if times is not None:
+            ev[1].record(stream)
 
     def _state_after(self, k: int) -> int:
         s = self.state
@@ -351,12 +399,13 @@ class BatchSweep:
         # one's box as its previous box and clears the forces left there
         self.steps_done = 0
         self.state = 0
-        self.build(self.block, self.group)
+        self.build(self.block, self.group, self.lds)
         torch.cuda.synchronize(self.device)
 
     def autotune(self, steps: int = 6, cache: str = TUNE_CACHE, use_cache: bool = True):
-        """Time each (block, group) candidate over ``steps`` eager steps, keep the
-        fastest, restore the initial state; returns {candidate: ms per step}."""
+        """Time each (block, group, lds) candidate over ``steps`` eager steps,
+        keep the fastest, restore the initial state; returns {"block/group/lds":
+        ms per step}."""
         import torch
         sig = self.signature()
         table = {}
@@ -366,24 +415,22 @@ class BatchSweep:
                     table = json.load(fh)
             except (OSError, ValueError):
                 table = {}
-        if sig in table:
-            block, group = table[sig]["best"]
-            self.build(block, group)
+        if sig in table and len(table[sig]["best"]) == 3:
+            self.build(*table[sig]["best"])
             return table[sig]["times"]
         times = {}
-        for block, group in CANDIDATES:
-            self.build(block, group)
+        for block, group, lds in CANDIDATES:
+            self.build(block, group, lds)
             self.step(1)                      # warm-up (first-use costs)
             torch.cuda.synchronize(self.device)
             t0 = time.perf_counter()
             self.step(steps)
             torch.cuda.synchronize(self.device)
-            times[f"{block}/{group}"] = (time.perf_counter() - t0) / steps * 1e3
+            times[f"{block}/{group}/{lds}"] = (time.perf_counter() - t0) / steps * 1e3
             self.reset()
-        best = min(times, key=times.get)
-        block, group = (int(v) for v in best.split("/"))
-        self.build(block, group)
-        table[sig] = {"best": [block, group], "times": times}
+        best = [int(v) for v in min(times, key=times.get).split("/")]
+        self.build(*best)
+        table[sig] = {"best": best, "times": times}
         if cache:
             try:
                 os.makedirs(os.path.dirname(cache), exist_ok=True)
@@ -435,16 +482,21 @@ def main(argv=None) -> int:
                     help="population bytes per cache group in MiB (0 = one group)")
     ap.add_argument("--sequential-steps", type=int, default=200,
                     help="steps of the per-instance sequential timing (0 = skip)")
+    ap.add_argument("--lanes", type=int, default=1,
+                    help="groups advanced at once, each on its own stream")
+    ap.add_argument("--solo-min-mcells", type=float, default=None,
+                    help="lattices of this many million cells or more step on their own objects")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--json", default=None)
     a = ap.parse_args(argv)
     import torch
     cfgs = draw_instances(a.instances, a.seed)
+    kw = {} if a.solo_min_mcells is None else {"solo_min_cells": int(a.solo_min_mcells * 1e6)}
     sweep = BatchSweep(cfgs, storage=a.storage, steps=a.steps + 64, cache_bytes=a.cache_mib << 20,
-                       deviation=a.deviation)
+                       deviation=a.deviation, lanes=a.lanes, **kw)
     tune = None if a.no_tune else sweep.autotune()
     print(f"[batch] {len(cfgs)} instances, {sweep.cells / 1e6:.1f} M cells, block {sweep.block}, "
-          f"group {sweep.group}", file=sys.stderr, flush=True)
+          f"group {sweep.group}, lds {sweep.lds}", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     done = 0
@@ -472,9 +524,10 @@ def main(argv=None) -> int:
             by_grid[key] = (cells + c.Nx * c.Ny * c.Nz, sec + share * ms / 1e3)
     res = {"instances": len(cfgs), "cells": sweep.cells, "steps": a.steps, "storage": a.storage,
            "deviation": a.deviation,
-           "block": sweep.block, "group": sweep.group, "cache_groups": len(sweep.groups),
+           "block": sweep.block, "group": sweep.group, "lds": sweep.lds,
+           "cache_groups": len(sweep.groups),
            "solo_groups": sum(g.solo for g in sweep.groups),
-           "cache_mib": a.cache_mib, "tuning_ms_per_step": tune,
+           "cache_mib": a.cache_mib, "lanes": sweep.lanes, "tuning_ms_per_step": tune,
            "seconds": dt, "mlups": sweep.cells * a.steps / dt / 1e6,
            "finite": all(l.health()[1] for l in sweep.lbms),
            "by_grid": {k: {"instances": sum(f"{c.Nx}x{c.Ny}x{c.Nz}" == k for c in cfgs),

@@ -423,8 +423,8 @@ static int batch_layout(int n, const clbm_grid_t* grids, int block, BatchLayout&
 template <int S> struct BatchRun {
   template <int PAR, int BK>
   static void go(const clbm_batch_t* b, int planes, hipStream_t st) {
-    hipLaunchKernelGGL((k_step_batch<S, PAR, BK>), dim3(b->blocks), dim3(BK), 0, st,
-                       (const StepArgs*)b->args, b->inst, b->first, planes);
+    hipLaunchKernelGGL((k_step_batch<S, PAR, BK>), dim3(b->blocks), dim3(BK), (unsigned)b->lds,
+                       st, (const StepArgs*)b->args, b->inst, b->first, planes);
   }
   static int run(const clbm_batch_t* b, int state, hipStream_t st) {
     const int planes = (state & CLBM_STATE_PLANES) ? 1 : 0;
@@ -491,7 +491,7 @@ int clbm_batch_build(int n, const clbm_grid_t* grids, const clbm_params_t* param
   if (e != hipSuccess) return fail((int)e, std::string("clbm_batch_build: ") + hipGetErrorString(e));
   unsigned char* w = (unsigned char*)work;
   *out = clbm_batch_t{n, grids[0].storage, block, L.blocks, w, (uint16_t*)(w + L.inst),
-                      (int32_t*)(w + L.first)};
+                      (int32_t*)(w + L.first), 0};
   return 0;
 }
 
@@ -501,6 +501,7 @@ int clbm_batch_step(const clbm_batch_t* b, int state, void* stream) {
   if (state & ~(CLBM_STATE_PARITY | CLBM_STATE_PLANES))
     return fail(-1, "clbm_batch_step: state has unknown bits");
   if (b->block != 128 && b->block != 256) return fail(-1, "clbm_batch_step: bad block size");
+  if (b->lds < 0 || b->lds > 64 * 1024) return fail(-1, "clbm_batch_step: lds must be in [0, 65536]");
   return dispatch_storage<BatchRun>(b->storage, b, state, (hipStream_t)stream);
 }
 

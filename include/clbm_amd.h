@@ -51,7 +51,7 @@
 extern "C" {
 #endif
 
-#define CLBM_ABI_VERSION 8
+#define CLBM_ABI_VERSION 9
 
 /* population storage formats */
 #define CLBM_STORE_F32 0  /* float32 populations, float32 arithmetic (the drop-in) */
@@ -389,6 +389,9 @@ typedef struct {
   void* args;         /* device: per-instance step arguments */
   uint16_t* inst;     /* device: instance of each block */
   int32_t* first;     /* device: first block of each instance */
+  int32_t lds;        /* bytes of LDS each workgroup of clbm_batch_step reserves, unused, to cap
+                         the workgroups per CU (0 after clbm_batch_build; a caller may set it,
+                         at most 65536) */
 } clbm_batch_t;
 
 /* Device bytes of the lattice tables for n grids, `block` threads per

@@ -78,11 +78,11 @@ def _assert_same(batch_objs, seq_objs):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("block,group", [(256, 8), (128, 4)])
-def test_batch_equals_each_instance_alone(pkg, gpu, block, group):
+@pytest.mark.parametrize("block,group,lds", [(256, 8, 0), (128, 4, 0), (256, 4, 40960)])
+def test_batch_equals_each_instance_alone(pkg, gpu, block, group, lds):
     bm = load_sub("batch")
     cfgs = _small(pkg)
-    sweep = bm.BatchSweep(cfgs, steps=16, block=block, group=group)
+    sweep = bm.BatchSweep(cfgs, steps=16, block=block, group=group, lds=lds)
     sweep.step(7)
     seq, _ = bm.sequential(cfgs, 7)
     _assert_same(list(zip(sweep.lbms, sweep.ibms)), seq)
@@ -157,12 +157,37 @@ def test_solo_group_steps_its_own_objects_and_equals_each_instance_alone(pkg, gp
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("lanes", [2, 3])
+def test_lanes_step_groups_side_by_side_and_equal_each_instance_alone(pkg, gpu, lanes):
+    """Groups advanced together on streams of their own (two solo lattices on
+    the split step, batched groups replaying their graphs) give the same
+    populations, marker forces and force fields as the instances run one by
+    one, bit for bit; timing events come back for every group."""
+    bm = load_sub("batch")
+    cfgs = _small(pkg)[:3] + [pkg.config.naca_case(96, 48, 4, 12.0, name=f"solo{i}")
+                              for i in range(2)]
+    sweep = bm.BatchSweep(cfgs, steps=32, cache_bytes=lanes, solo_min_cells=10_000, lanes=lanes)
+    assert sum(g.solo for g in sweep.groups) == 2 and len(sweep.groups) == 5
+    times = []
+    sweep.run(9, graph=True, times=times)
+    sweep.step(1)
+    sweep.run(6, graph=True, times=times)
+    assert len(times) == 10
+    import torch
+    torch.cuda.synchronize()
+    assert all(e0.elapsed_time(e1) > 0 for _, e0, e1 in times)
+    assert all(l.steps == 16 and l.state_word == sweep.state for l in sweep.lbms)
+    seq, _ = bm.sequential(cfgs, 16)
+    _assert_same(list(zip(sweep.lbms, sweep.ibms)), seq)
+
+
+@pytest.mark.gpu
 def test_autotune_restores_the_initial_state(pkg, gpu, tmp_path):
     bm = load_sub("batch")
     cfgs = _small(pkg)[:2]
     sweep = bm.BatchSweep(cfgs, steps=64)
     times = sweep.autotune(steps=2, cache=str(tmp_path / "tune.json"))
-    assert set(times) == {f"{b}/{g}" for b, g in bm.CANDIDATES}
+    assert set(times) == {f"{b}/{g}/{l}" for b, g, l in bm.CANDIDATES}
     # one step after tuning (which took three per candidate): the forces of the
     # tuning's last step lie in another box and must be gone from fx, fy
     sweep.step(1)
@@ -187,3 +212,12 @@ def test_batch_rejects_malformed_input(pkg, gpu):
     out = lib.ClbmBatch()
     with pytest.raises(lib.ClbmError, match="not built"):
         lib.call("clbm_batch_step", C.byref(out), 0, None)
+
+
+@pytest.mark.parametrize("lds", [-1, 65537])
+def test_batch_step_rejects_an_lds_reservation_out_of_range(pkg, lds):
+    """Checked on the host before any launch (the tables are never read)."""
+    lib = load_sub("_lib")
+    b = lib.ClbmBatch(1, lib.STORE_F32, 256, 1, 16, 16, 16, lds)
+    with pytest.raises(lib.ClbmError, match="lds must be"):
+        lib.call("clbm_batch_step", C.byref(b), 0, None)
