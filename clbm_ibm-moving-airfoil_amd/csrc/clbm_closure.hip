@@ -396,7 +396,7 @@ __global__ __launch_bounds__(CBLK) void k_step_closure(StepArgs a, ClosureArgs n
   if (!act) return;
   rates_from(yv, n.lo, n.hi, s);
   float out[Q];
-  relax_rebuild(m, s, Fx, Fy, Fz, float(a.s_v), float(a.s_e), out);
+  relax_rebuild(m, s, Fx, Fy, Fz, a.s_v.f, a.s_e.f, out);
 #pragma unroll
   for (int q = 0; q < Q; ++q) fq[q] = R(out[q]);
   pop_store<PAR>(pop, c, nb, fq);
@@ -406,7 +406,7 @@ __global__ __launch_bounds__(CBLK) void k_step_closure(StepArgs a, ClosureArgs n
       // the outlet cell of a row whose force differs from its neighbour's takes
       // the plain cascade (the network runs on the cells of the wavefront only)
       pop_load<PAR>(pop, c + 1, nb1, fq);
-      collide_cell<R>(fq, R(Gx), R(Gy), R(Gz), R(a.s_v), R(a.s_e));
+      collide_cell<R>(fq, R(Gx), R(Gy), R(Gz), a.s_v.template as<R>(), a.s_e.template as<R>());
     }
     pop_store<PAR>(pop, c + 1, nb1, fq);
   }

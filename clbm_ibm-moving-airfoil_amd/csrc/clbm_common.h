@@ -337,11 +337,24 @@ template <int S> struct Pop {
   __device__ __forceinline__ R ld(int plane, int cell, int q) const {
     return Store<S>::ld(base_elem<T>(base[plane], cell), dv[q]);
   }
+  // AGAIN: a store of the value an earlier st of this thread stored (the
+  // outlet copy).  For f16c, the encoding of the earlier store is reused, and
+  // an opaque copy of it gives each store a shift of its own: the compiler
+  // then stores the code straight from the high half of the register
+  // (a shift shared by two stores keeps it from doing so).
+  template <bool AGAIN = false>
   __device__ __forceinline__ void st(int plane, int cell, int q, R v) const {
     uint32_t off = (uint32_t)cell * (uint32_t)sizeof(T);
     asm("" : "+v"(off));
     using GT = __attribute__((address_space(1))) char;
-    Store<S>::st((T*)(reinterpret_cast<GT*>(base[plane]) + off), v, dv[q]);
+    T* p = (T*)(reinterpret_cast<GT*>(base[plane]) + off);
+    if constexpr (S == CLBM_STORE_F16C && AGAIN) {
+      uint32_t h = f16c_encode_hi(v - dv[q]);
+      asm("" : "+v"(h));
+      *p = (uint16_t)(h >> 16);
+    } else {
+      Store<S>::st(p, v, dv[q]);
+    }
   }
 };
 
@@ -369,14 +382,15 @@ __device__ __forceinline__ void pop_store_in(const P& pop, int c, const Nbr& nb,
       pop.st(opp(q), c + nb.off(-ex(q), -ey(q), -ez(q)), q, fq[q]);
   }
 }
-template <int PAR, typename P>
+// AGAIN: the thread stored these values before (Pop::st).
+template <int PAR, bool AGAIN = false, typename P>
 __device__ __forceinline__ void pop_store(const P& pop, int c, const Nbr& nb, const typename P::R* fq) {
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     if (PAR == 0)
-      pop.st(opp(q), c, q, fq[q]);
+      pop.template st<AGAIN>(opp(q), c, q, fq[q]);
     else
-      pop.st(q, c + nb.off(ex(q), ey(q), ez(q)), q, fq[q]);
+      pop.template st<AGAIN>(q, c + nb.off(ex(q), ey(q), ez(q)), q, fq[q]);
   }
 }
 

@@ -79,10 +79,21 @@ template <bool COHERENT> struct AccForce {
   }
 };
 
+// A relaxation rate in both precisions, rounded once on the host, so that a
+// float32 step reads its rate instead of converting it in every thread.
+struct Rate {
+  double d;
+  float f;
+  __host__ __device__ Rate(double v = 0.0) : d(v), f((float)v) {}
+  template <typename R> __device__ __forceinline__ R as() const {
+    if constexpr (sizeof(R) == sizeof(float)) return f; else return R(d);
+  }
+};
+
 // Everything one launch of the fused step needs for one lattice.
 struct StepArgs {
   Geo g;
-  double s_v, s_e;
+  Rate s_v, s_e;
   Planes pl;
   void* f;
   const float *fx, *fy, *fz;
@@ -148,15 +159,15 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, int t, const Force&
     if (!twin) pop_store_in<PAR>(pop, c1, make_nbr(g, x + 1, y, z), fq);
   }
   if (pl.on && x == 0) inlet_rule<R>(fq, R(pl.ux_in), R(pl.uy_in), pl.rule);
-  collide_cell<R, FORCE>(fq, Fx, Fy, Fz, R(a.s_v), R(a.s_e));
+  collide_cell<R, FORCE>(fq, Fx, Fy, Fz, a.s_v.template as<R>(), a.s_e.template as<R>());
   pop_store<PAR>(pop, c, nb, fq);
   if (outlet) {
     const Nbr nb1 = make_nbr(g, x + 1, y, z);
     if (!twin) {
       pop_load<PAR>(pop, c1, nb1, fq);
-      collide_cell<R, FORCE>(fq, Gx, Gy, Gz, R(a.s_v), R(a.s_e));
+      collide_cell<R, FORCE>(fq, Gx, Gy, Gz, a.s_v.template as<R>(), a.s_e.template as<R>());
     }
-    pop_store<PAR>(pop, c1, nb1, fq);
+    pop_store<PAR, true>(pop, c1, nb1, fq);
   }
 }
 
