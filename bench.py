@@ -111,15 +111,26 @@ def host_cpu_model() -> str:
     return "unknown"
 
 
-def traffic_record(case: str, storage: str):
-    """HBM bytes per launch of the fused step kernel, from the rocprofv3
-    FETCH_SIZE / WRITE_SIZE runs summarised in profiles/ (null when absent)."""
+def _traffic_file():
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
-        return None
+        return {}
     with open(path) as fh:
-        rec = json.load(fh)
-    return rec.get(f"{case}/{storage}")
+        return json.load(fh)
+
+
+def traffic_record(case: str, storage: str):
+    """HBM bytes per launch of the fused step kernel, from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs summarised in profiles/ (null when absent).
+    Not measured in this run: the figure holds for the kernels of the commit
+    that traffic_source names."""
+    return _traffic_file().get(f"{case}/{storage}")
+
+
+def traffic_source() -> str:
+    commit = _traffic_file().get("_commit")
+    return ("profiles/traffic.json: rocprofv3 FETCH_SIZE + WRITE_SIZE passes on the kernels of "
+            f"commit {commit}, calibrated by a device copy" if commit else "none")
 
 
 def copy_calibration(mib: int) -> int:
@@ -377,6 +388,7 @@ def main(argv=None) -> int:
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic_record(a.case, a.storage),
+                     "traffic_source": traffic_source(),
                      "kernel_ms": round(k_ms, 5),
                      "cells_per_launch": round(k_cells),
                      "bytes_per_launch": round(b_alg),
