@@ -41,6 +41,9 @@ def parse(argv=None):
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-MiB device buffer 10 times (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
+    ap.add_argument("--spread-sort-probe", action="store_true",
+                    help="only time a sort-based spread of the markers' stencil contributions "
+                         "beside the library's marker computation")
     ap.add_argument("--stream-probe", action="store_true",
                     help="only time clbm_stream_probe (the step's data movement without the "
                          "collision) on the --case grid with --storage populations")
@@ -183,6 +186,75 @@ def stream_probe(pkg, cfg, storage, steps, warmup, quiet=False):
     return 0
 
 
+def spread_sort_probe(pkg, cfg, reps: int = 200) -> int:
+    """The cost of spreading by sorting, measured beside the marker kernel.
+
+    A design that spreads by sorting (cell, contribution) pairs by cell and
+    summing each cell's run in order (deterministic without atomics) sorts,
+    per step, the 64 stencil cells of every marker.  With the markers of a few
+    coupled steps of the case, this times on the GPU (events, ``reps``
+    repetitions): the library's whole marker computation (``compute_force`` on
+    one stream: the fused marker kernel placing the markers, interpolating,
+    forming the forces and spreading them with fixed-point atomics, and the
+    clamp kernel); a stable radix sort of the keys alone (torch.sort, rocPRIM);
+    and the sort with the rest of a sort-based spread (gather of two int64
+    contributions per key, prefix sums, the sum of each cell's run, one store
+    per cell).  Prints one JSON record."""
+    import torch
+    lbm = pkg.CLBMSolver(cfg, device="cuda:0")
+    ibm = pkg.DroneMovingIBM(cfg, device="cuda:0", overlap=False)
+    for n in range(1, 5):
+        fx, fy, fz = ibm.compute_force(lbm.ux, lbm.uy, *cfg.motion_state(n))
+        lbm.step(fx, fy, fz)
+    st = torch.cuda.current_stream()
+
+    def timed(fn):
+        for _ in range(5):
+            fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            fn()
+        e1.record(st)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) * 1e3 / reps
+
+    kin = cfg.motion_state(5)
+    t_ibm = timed(lambda: ibm.compute_force(lbm.ux, lbm.uy, *kin))
+    # the 64 stencil cells of every marker (x clamped, y and z periodic)
+    X, Y, Z = (t._t.float() for t in (ibm.X_marker, ibm.Y_marker, ibm.Z_marker))
+    d = torch.arange(-1, 3, device="cuda:0")
+    ix = (torch.floor(X)[:, None] + d).clamp(0, cfg.Nx - 1).long()
+    iy = torch.remainder(torch.floor(Y)[:, None] + d, cfg.Ny).long()
+    iz = torch.remainder(torch.floor(Z)[:, None] + d, cfg.Nz).long()
+    cells = ((iz[:, :, None, None] * cfg.Ny + iy[:, None, :, None]) * cfg.Nx
+             + ix[:, None, None, :]).reshape(-1).to(torch.int32)
+    vals = torch.randint(-(1 << 40), 1 << 40, (2, cells.numel()), device="cuda:0",
+                         dtype=torch.int64)
+    acc = torch.zeros(2, lbm.ncell, dtype=torch.int64, device="cuda:0")
+
+    def sort_only():
+        torch.sort(cells, stable=True)
+
+    def sort_spread():
+        k, perm = torch.sort(cells, stable=True)
+        v = vals[:, perm].cumsum(1)
+        last = torch.ones_like(k, dtype=torch.bool)
+        last[:-1] = k[1:] != k[:-1]
+        ends = v[:, last]
+        sums = torch.cat([ends[:, :1], ends[:, 1:] - ends[:, :-1]], 1)
+        acc[:, k[last].long()] = sums
+
+    t_sort = timed(sort_only)
+    t_spread = timed(sort_spread)
+    rec = {"probe": "spreading by sort vs the fused marker kernel", "workload": cfg.name,
+           "markers": ibm.N, "keys": int(cells.numel()), "reps": reps,
+           "us_compute_force": round(t_ibm, 2), "us_sort_keys": round(t_sort, 2),
+           "us_sort_based_spread": round(t_spread, 2)}
+    print(json.dumps(rec))
+    return 0
+
+
 def device_warmup(seconds: float) -> None:
     """Keep the GPU busy with 256 MiB device-to-device copies for ``seconds``
     of wall time.  After the idle seconds of a process start or of building
@@ -311,6 +383,8 @@ def main(argv=None) -> int:
     torch.cuda.set_device(dev)
     if a.stream_probe:
         return stream_probe(pkg, cfg, a.storage, a.steps, a.warmup)
+    if a.spread_sort_probe:
+        return spread_sort_probe(pkg, cfg)
     ncell = cfg.Nx * cfg.Ny * cfg.Nz
     tune = {"lbm": {}, "ibm": {"overlap": a.overlap}}
     if a.outside_lds is not None:
