@@ -453,6 +453,18 @@ class DroneMovingIBM:
             return np.zeros((0, 2))
         return self._hist[: self._hist_n].cpu().numpy().copy()
 
+    def restore_history(self, sums) -> None:
+        """Replace the recorded history by ``sums`` ((steps, 2), as
+        force_history returns it), e.g. when a run resumes from a checkpoint;
+        record_force appends after it."""
+        import torch
+        a = np.asarray(sums, dtype=np.float64).reshape(-1, 2)
+        cap = max(1024, 1 << max(0, int(len(a) - 1).bit_length()))
+        h = torch.zeros((cap, 2), dtype=torch.float64, device=self.device)
+        if len(a):
+            h[: len(a)] = torch.from_numpy(a).to(self.device)
+        self._hist, self._hist_n = h, len(a)
+
     def coefficients(self, sums, rho_inf: float = 1.0, span=None, momentum_factor: float = 1.0):
         """Drag and lift coefficients from force sums (array (..., 2)).
 

@@ -24,6 +24,8 @@ Differences of mechanism (not of results):
 from __future__ import annotations
 
 import ctypes as C
+import json
+import os
 import weakref
 
 import numpy as np
@@ -245,6 +247,64 @@ class CLBMSolver:
         self.state_word = _lib.fn("clbm_state_after_step")(self.state_word, self._pref)
         self.steps += 1
         self._macro_valid = False
+
+    # -- checkpoint / resume (no reference counterpart: the reference saves
+    #    only macroscopic frames, main_moving.py:148-155, and cannot resume) --
+    CHECKPOINT_VERSION = 1
+
+    def _checkpoint_meta(self) -> dict:
+        g, p = self.grid, self.params
+        return {"version": self.CHECKPOINT_VERSION, "grid": [self.Nx, self.Ny, self.Nz],
+                "storage": self.storage, "deviation_velocity": [float(g.dev_ux), float(g.dev_uy)],
+                "inlet": int(p.inlet), "s_v": float(p.s_v), "s_e": float(p.s_e),
+                "ux_in": float(p.ux_in), "uy_in": float(p.uy_in)}
+
+    def save_state(self, path: str, **extra) -> None:
+        """Write the populations as they are held (the raw in-place buffer of
+        this storage format, so that 16-bit codes are kept exactly), the state
+        word and the step count to ``path`` (an .npz file, replaced
+        atomically), with ``extra`` arrays beside them.  A solver of the same
+        grid, storage, deviation base, inlet and rates that loads it continues
+        bit for bit."""
+        import torch
+        self._wait_readers()
+        raw = self.state.cpu().numpy()
+        tmp = f"{path}.tmp{os.getpid()}"
+        with open(tmp, "wb") as fh:
+            np.savez(fh, state=raw, state_word=np.int32(self.state_word),
+                     steps=np.int64(self.steps), meta=json.dumps(self._checkpoint_meta()),
+                     **{k: np.asarray(v) for k, v in extra.items()})
+        os.replace(tmp, path)
+        torch.cuda.current_stream(self.device).synchronize()
+
+    def load_state(self, path: str) -> dict:
+        """Load what ``save_state`` wrote (ValueError when the file is for
+        another grid, storage, deviation base, inlet or rates); returns its
+        extra arrays."""
+        import torch
+        with np.load(path, allow_pickle=False) as d:
+            meta = json.loads(str(d["meta"]))
+            mine = self._checkpoint_meta()
+            for k, v in mine.items():
+                if meta.get(k) != v:
+                    raise ValueError(f"checkpoint {path}: {k} is {meta.get(k)!r}, "
+                                     f"this solver's {v!r}")
+            raw = d["state"]
+            word = int(d["state_word"])
+            steps = int(d["steps"])
+            extra = {k: d[k] for k in d.files if k not in ("state", "state_word", "steps", "meta")}
+        if raw.dtype != np.uint8 or raw.size != self.state.numel():
+            raise ValueError(f"checkpoint {path}: population buffer of {raw.size} bytes, "
+                             f"this solver holds {self.state.numel()}")
+        if word & ~(_lib.STATE_PARITY | _lib.STATE_PLANES) or steps < 0:
+            raise ValueError(f"checkpoint {path}: bad state word {word} or step count {steps}")
+        self._wait_readers()
+        self.state.copy_(torch.from_numpy(raw).to(self.device))
+        self.state_word = word
+        self.steps = steps
+        self._macro_valid = False
+        torch.cuda.current_stream(self.device).synchronize()
+        return extra
 
     # -- concurrent readers (immersed-boundary objects on their own stream) --
     def _add_reader(self, ibm, region) -> None:

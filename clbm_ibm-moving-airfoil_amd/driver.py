@@ -147,7 +147,7 @@ def _level(compress):
 
 def run(cfg, steps=None, out_dir=None, storage="f32", write_frames=True, record_forces=True,
         verbose=True, spread_mode="fixed", deviation="weights", async_frames=True,
-        compress=1, inlet=True):
+        compress=1, inlet=True, checkpoint=None, checkpoint_every=None, resume=None):
     """Run the coupled loop; returns a dict with the force history and timing.
     ``forces.npz`` is written whenever the forces are recorded, frames and
     ``meta.npz`` only with ``write_frames``.  ``async_frames`` writes frames
@@ -155,7 +155,15 @@ def run(cfg, steps=None, out_dir=None, storage="f32", write_frames=True, record_
     step, as the reference does); ``compress`` is the frames' zlib level (1 by
     default, True for np.savez_compressed's 6, False for stored members as
     np.savez writes them; np.load returns the same arrays for all).  ``inlet``
-    as CLBMSolver's (True: the reference's rule, "hecht-harting")."""
+    as CLBMSolver's (True: the reference's rule, "hecht-harting").
+
+    ``checkpoint``: a path that receives the solver state (CLBMSolver.save_state:
+    populations as held, state word, step count) with the force history and
+    the frame records every ``checkpoint_every`` steps (default the frame
+    interval) and at the end; ``resume``: such a file to continue from (its
+    step count + 1 up to ``steps``).  The resumed run's populations, forces
+    and frames equal those of one uninterrupted run, bit for bit.  No
+    reference counterpart: the reference cannot resume."""
     import torch
     steps = cfg.N_steps if steps is None else steps
     out_dir = cfg.DATA_DIR if out_dir is None else out_dir
@@ -163,14 +171,30 @@ def run(cfg, steps=None, out_dir=None, storage="f32", write_frames=True, record_
         os.makedirs(out_dir, exist_ok=True)
     lbm = CLBMSolver(cfg, storage=storage, deviation=deviation, inlet=inlet)
     ibm = DroneMovingIBM(cfg, spread_mode=spread_mode)
-    writer = FrameWriter(lbm, compress=compress) if (write_frames and async_frames) else None
     snaps = dict(xc=[], yc=[], theta=[], step=[])
     frame = 0
+    first = 1
+    if resume is not None:
+        extra = lbm.load_state(resume)
+        first = lbm.steps + 1
+        if "force_history" in extra:
+            ibm.restore_history(extra["force_history"])
+        frame = int(extra.get("frame", 0))
+        for k in snaps:
+            if f"snap_{k}" in extra:
+                snaps[k] = list(extra[f"snap_{k}"].tolist())
+    every = checkpoint_every or cfg.vis_interval
+
+    def save_checkpoint():
+        lbm.save_state(checkpoint, force_history=ibm.force_history(), frame=np.int64(frame),
+                       **{f"snap_{k}": np.asarray(v, np.float64) for k, v in snaps.items()})
+
+    writer = FrameWriter(lbm, compress=compress) if (write_frames and async_frames) else None
     diverged = False
     torch.cuda.synchronize()
     t0 = last = time.perf_counter()
-    step = 0
-    for step in range(1, steps + 1):
+    step = first - 1
+    for step in range(first, steps + 1):
         xc, yc, th, vx, vy, om = cfg.motion_state(step)
         fx, fy, fz = ibm.compute_force(lbm.ux, lbm.uy, xc, yc, th, vx, vy, om)
         lbm.step(fx, fy, fz)
@@ -199,6 +223,8 @@ def run(cfg, steps=None, out_dir=None, storage="f32", write_frames=True, record_
                                rho=lbm.rho.to_numpy())
                 for k, v in zip(("xc", "yc", "theta", "step"), (xc, yc, th, step)):
                     snaps[k].append(v)
+        if checkpoint is not None and (step % every == 0 or step == steps):
+            save_checkpoint()
     if writer is not None:
         writer.close()
     torch.cuda.synchronize()
@@ -219,10 +245,11 @@ def run(cfg, steps=None, out_dir=None, storage="f32", write_frames=True, record_
     if record_forces:
         np.savez_compressed(os.path.join(out_dir, "forces.npz"), sums=hist, cd=cd, cl=cl)
     ncell = cfg.Nx * cfg.Ny * cfg.Nz
-    res = dict(steps=step, elapsed=elapsed, mlups=step * ncell / elapsed / 1e6, history=hist,
-               cd=cd, cl=cl, diverged=diverged)
+    ran = step - first + 1
+    res = dict(steps=step, elapsed=elapsed, mlups=ran * ncell / max(elapsed, 1e-9) / 1e6,
+               history=hist, cd=cd, cl=cl, diverged=diverged)
     if verbose:
-        print(f"Done. {step} steps in {elapsed:.1f} s ({step / elapsed:.0f} steps/s, "
+        print(f"Done. {ran} steps in {elapsed:.1f} s ({ran / max(elapsed, 1e-9):.0f} steps/s, "
               f"{res['mlups']:.0f} MLUPS)", flush=True)
     return res
 
@@ -244,6 +271,10 @@ def main(argv=None) -> int:
     ap.add_argument("--vis-interval", type=int, default=None, help="steps between frames")
     ap.add_argument("--inlet", default="reference", choices=["reference", "hecht-harting"],
                     help="inlet rule at x = 0 (the reference's, or the opt-in Hecht-Harting one)")
+    ap.add_argument("--checkpoint", default=None, help="write the run's state to this .npz file")
+    ap.add_argument("--checkpoint-every", type=int, default=None,
+                    help="steps between checkpoints (default: the frame interval)")
+    ap.add_argument("--resume", default=None, help="continue from this checkpoint file")
     a = ap.parse_args(argv)
     cfg = CASES[a.case]()
     if a.vis_interval is not None:
@@ -251,7 +282,8 @@ def main(argv=None) -> int:
     res = run(cfg, a.steps, a.out, a.storage, write_frames=not a.no_frames,
               deviation=a.deviation, async_frames=not a.sync_frames,
               compress=False if a.no_compress else a.compress_level,
-              inlet=True if a.inlet == "reference" else a.inlet)
+              inlet=True if a.inlet == "reference" else a.inlet, checkpoint=a.checkpoint,
+              checkpoint_every=a.checkpoint_every, resume=a.resume)
     return 1 if res["diverged"] else 0
 
 

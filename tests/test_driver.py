@@ -140,3 +140,64 @@ def test_frame_writer_depth_respects_the_pinned_budget(pkg):
     assert d(800 * 260 * 220, cores=16, budget=1 << 30) == 1      # 732 MB frames
     assert d(800 * 260 * 220, cores=16, budget=4 << 30) == 5
     assert d(1200 * 400 * 100, cores=16, budget=1 << 30) == 1     # over budget: still one
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("storage,deviation", [("f32", "weights"), ("f16c", "inflow"),
+                                               ("f64", "weights")])
+def test_checkpoint_resume_continues_bit_for_bit(pkg, gpu, tmp_path, storage, deviation):
+    """save_state after 3 coupled steps, load_state into new objects, 4 more
+    steps: populations, marker forces and force sums equal the uninterrupted
+    run's bit for bit (the raw buffer keeps 16-bit codes and the AA state
+    word); a checkpoint of another storage, grid or rate is refused."""
+    cfg = pkg.config.naca_case(40, 24, 4, 8.0, name="ckpt")
+    a_l = pkg.CLBMSolver(cfg, storage=storage, deviation=deviation)
+    a_i = pkg.DroneMovingIBM(cfg)
+
+    def steps(l, i, n0, n1):
+        for n in range(n0, n1):
+            fx, fy, fz = i.compute_force(l.ux, l.uy, *cfg.motion_state(n))
+            l.step(fx, fy, fz)
+            i.record_force()
+
+    steps(a_l, a_i, 1, 4)
+    path = str(tmp_path / "state.npz")
+    a_l.save_state(path, force_history=a_i.force_history())
+    steps(a_l, a_i, 4, 8)
+    b_l = pkg.CLBMSolver(cfg, storage=storage, deviation=deviation)
+    b_i = pkg.DroneMovingIBM(cfg)
+    extra = b_l.load_state(path)
+    assert b_l.steps == 3 and b_l.state_word == 1 | 2
+    b_i.restore_history(extra["force_history"])
+    steps(b_l, b_i, 4, 8)
+    assert b_l.steps == a_l.steps == 7 and b_l.state_word == a_l.state_word
+    np.testing.assert_array_equal(b_l.f.to_numpy(), a_l.f.to_numpy())
+    np.testing.assert_array_equal(b_i.FX_k.to_numpy(), a_i.FX_k.to_numpy())
+    np.testing.assert_array_equal(b_i.force_history(), a_i.force_history())
+    other = "f32" if storage != "f32" else "f16"
+    for wrong in (pkg.CLBMSolver(cfg, storage=other),
+                  pkg.CLBMSolver(cfg.with_(Nz=6), storage=storage, deviation=deviation),
+                  pkg.CLBMSolver(cfg.with_(s_v=1.7), storage=storage, deviation=deviation)):
+        with pytest.raises(ValueError, match="checkpoint"):
+            wrong.load_state(path)
+
+
+@pytest.mark.gpu
+def test_driver_resumes_from_its_checkpoint(pkg, gpu, tmp_path):
+    """run(8 steps) against run(4 steps, checkpoint) + run(8 steps, resume):
+    the same force history, the same frames and meta records."""
+    drv = load_sub("driver")
+    cfg = pkg.config.naca_case(40, 24, 4, 8.0, name="resume").with_(vis_interval=2)
+    one, two = tmp_path / "one", tmp_path / "two"
+    ref = drv.run(cfg, 8, str(one), verbose=False)
+    ck = str(tmp_path / "run.npz")
+    drv.run(cfg, 4, str(two), verbose=False, checkpoint=ck, checkpoint_every=3)
+    res = drv.run(cfg, 8, str(two), verbose=False, checkpoint=ck, resume=ck)
+    assert res["steps"] == 8 and not res["diverged"]
+    np.testing.assert_array_equal(res["history"], ref["history"])
+    assert sorted(os.listdir(two)) == sorted(os.listdir(one))
+    for name in ("frame_000004.npz", "meta.npz", "forces.npz"):
+        a, b = np.load(one / name), np.load(two / name)
+        assert set(a.files) == set(b.files)
+        for k in a.files:
+            np.testing.assert_array_equal(a[k], b[k], err_msg=f"{name}:{k}")
