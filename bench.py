@@ -41,6 +41,8 @@ def parse(argv=None):
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-MiB device buffer 10 times (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
+    ap.add_argument("--events-every", type=int, default=None,
+                    help="time the step kernel with events on every N-th step (0: none)")
     ap.add_argument("--spread-sort-probe", action="store_true",
                     help="only time a sort-based spread of the markers' stencil contributions "
                          "beside the library's marker computation")
@@ -75,7 +77,8 @@ def cpu_baseline(cfg, seconds: float, threads: int):
     """The reference-structured C restatement (oracle/, AoS double buffer,
     12 kernels per step, float32) timed on the host on a bounded sample of the
     same workload: the same grid and markers for as many steps as fit in
-    ``seconds``."""
+    ``seconds``.  It runs the build with FMA contraction (libclbm_ref_fast.so,
+    flags in the record) when present, else the bit-exact test build."""
     if seconds <= 0:
         return None
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -84,10 +87,13 @@ def cpu_baseline(cfg, seconds: float, threads: int):
     except Exception as e:  # noqa: BLE001
         return {"value": None, "unit": "MLUPS", "cores": threads, "kind": "port",
                 "sample": f"unavailable: {e}"}
-    if not clbm_ref_c.available():
+    fast = clbm_ref_c.available(fast=True)
+    if not fast and not clbm_ref_c.available():
         return {"value": None, "unit": "MLUPS", "cores": threads, "kind": "port",
                 "sample": "oracle/libclbm_ref.so not built"}
-    sim = clbm_ref_c.RefSim(cfg, threads=threads)
+    flags = (clbm_ref_c.FAST_FLAGS if fast else
+             "-O3 -march=x86-64-v3 -ffp-contract=off -fno-fast-math -fopenmp -std=c11")
+    sim = clbm_ref_c.RefSim(cfg, threads=threads, fast=fast)
     t0 = time.perf_counter()
     sim.run(1, first_step=1)
     t1 = time.perf_counter() - t0
@@ -100,7 +106,8 @@ def cpu_baseline(cfg, seconds: float, threads: int):
             "kind": "port",
             "sample": (f"{n} steps of the {cfg.Nx}x{cfg.Ny}x{cfg.Nz} case with its "
                        f"{cfg.N_MARKERS} markers ({dt:.1f} s), reference-structured C/OpenMP "
-                       f"restatement of the 12 kernels, float32 AoS, {threads} threads")}
+                       f"restatement of the 12 kernels, float32 AoS, {threads} threads"),
+            "flags": "gcc " + flags}
 
 
 def host_cpu_model() -> str:
@@ -275,6 +282,14 @@ def device_warmup(seconds: float) -> None:
     del a, b
 
 
+# The step kernel's events are recorded on every 64th timed step: on MI355X a
+# pair of timing events recorded around a launch costs the step loop about
+# 1.5 % (f32) to 3.5 % (16-bit) of its rate when done on every 4th step, and
+# 0.4-0.8 % on every 16th.
+EVENTS_EVERY = 64
+KERNEL_STEPS = 200
+
+
 def measure(pkg, cfg, storage, steps, warmup, dev, tune, deviation="weights", coupled=None,
             device_warmup_s=0.0):
     """Warm-up, then ``steps`` timed coupled steps (compute_force + step) of a
@@ -294,32 +309,42 @@ def measure(pkg, cfg, storage, steps, warmup, dev, tune, deviation="weights", co
         step += 1
     # The dominant kernel is the fused step kernel over the lattice away from
     # the body (the whole lattice when the step is not split).  CLBMSolver
-    # records a pair of HIP events around it on the stream it launches on;
-    # that is done on every EVERY-th step, alternating parities, so that the
-    # event records barely perturb the loop.
-    EVERY = 4
+    # records a pair of HIP events around it on the stream it launches on, on
+    # every EVERY-th timed step, alternating parities.  With EVERY = 0 the timed
+    # steps record none, and the kernel is timed over KERNEL_STEPS further
+    # steps right after them (events on every other step).
+    EVERY = EVENTS_EVERY
+    n_ev = steps // max(EVERY, 1) + 1 if EVERY > 0 else KERNEL_STEPS // 2 + 1
     pool = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-            for _ in range(steps // EVERY + 1)]
+            for _ in range(n_ev)]
     for e0, e1 in pool:        # created now (a first record), so the step records them itself
         e0.record()
         e1.record()
     ev, cells = [], []
+
+    def run_steps(n, every):
+        nonlocal step
+        for i in range(n):
+            xc, yc, th, vx, vy, om = cfg.motion_state(step)
+            fx, fy, fz = ibm.compute_force(lbm.ux, lbm.uy, xc, yc, th, vx, vy, om)
+            timed = every > 0 and i % every == (i // every) % 2
+            if timed:
+                ev.append(pool[len(ev)])
+                lbm.kernel_events = ev[-1]
+            lbm.step(fx, fy, fz)
+            if timed:
+                cells.append(lbm.last_kernel_cells)
+            step += 1
+
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for i in range(steps):
-        xc, yc, th, vx, vy, om = cfg.motion_state(step)
-        fx, fy, fz = ibm.compute_force(lbm.ux, lbm.uy, xc, yc, th, vx, vy, om)
-        timed = i % EVERY == (i // EVERY) % 2
-        if timed:
-            ev.append(pool[len(ev)])
-            lbm.kernel_events = ev[-1]
-        lbm.step(fx, fy, fz)
-        if timed:
-            cells.append(lbm.last_kernel_cells)
-        step += 1
+    run_steps(steps, EVERY)
     t_host = time.perf_counter() - t0      # the host has queued every step
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    if EVERY <= 0:
+        run_steps(KERNEL_STEPS, 2)
+        torch.cuda.synchronize()
     rho_min, finite = lbm.health()
     k_ms = float(np.mean([e0.elapsed_time(e1) for e0, e1 in ev]))
     out = dict(dt=dt, k_ms=k_ms, k_cells=float(np.mean(cells)), rho_min=rho_min, finite=finite,
@@ -375,6 +400,9 @@ def main(argv=None) -> int:
         return 2
     if a.copy_calibration:
         return copy_calibration(a.copy_calibration)
+    global EVENTS_EVERY
+    if a.events_every is not None:
+        EVENTS_EVERY = a.events_every
     import torch
     pkg = importlib.import_module(PKG)
     cfgmod = pkg.config

@@ -14,17 +14,20 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "libclbm_ref.so")
-_lib = None
+# the same source built with FMA contraction (the CPU baseline; not bit-exact)
+LIB_FAST = os.path.join(HERE, "libclbm_ref_fast.so")
+FAST_FLAGS = "-O3 -march=x86-64-v3 -mtune=generic -ffp-contract=fast -fopenmp -std=gnu11"
+_libs = {}
 
 
-def available() -> bool:
-    return os.path.exists(LIB)
+def available(fast: bool = False) -> bool:
+    return os.path.exists(LIB_FAST if fast else LIB)
 
 
-def _load():
-    global _lib
-    if _lib is None:
-        lib = C.CDLL(LIB)
+def _load(fast: bool = False):
+    path = LIB_FAST if fast else LIB
+    if path not in _libs:
+        lib = C.CDLL(path)
         P, F, D, I = C.c_void_p, C.c_float, C.c_double, C.c_int
         lib.ref_create.restype = P
         lib.ref_create.argtypes = [I, I, I, D, D, D, D, P, I, P, P, P, F, I]
@@ -40,8 +43,8 @@ def _load():
         lib.ref_get_forces.argtypes = [P, P, P]
         lib.ref_get_markers.argtypes = [P, P]
         lib.ref_force_sum.argtypes = [P, P, P]
-        _lib = lib
-    return _lib
+        _libs[path] = lib
+    return _libs[path]
 
 
 def _minv() -> np.ndarray:
@@ -57,8 +60,9 @@ class RefSim:
     """One case (a CaseConfig-like object with Nx, Ny, Nz, s_v, s_e, ux_in,
     uy_in, X0, Y0, Z0, dV, f_max, wrap_weight_quirk, motion_state)."""
 
-    def __init__(self, cfg, threads: int = 8, inlet=True):
-        lib = _load()
+    def __init__(self, cfg, threads: int = 8, inlet=True, fast: bool = False):
+        self._fast = fast
+        lib = self._lib()
         self.cfg = cfg
         self.shape = (cfg.Nx, cfg.Ny, cfg.Nz)
         self._keep = [np.ascontiguousarray(a, dtype=np.float32) for a in (cfg.X0, cfg.Y0, cfg.Z0)]
@@ -75,19 +79,22 @@ class RefSim:
         lib.ref_set_inlet(self.h, 1 if inlet == "hecht-harting" else 0)
         lib.ref_init_feq(self.h)
 
+    def _lib(self):
+        return _load(self._fast)
+
     def __del__(self):
         if getattr(self, "h", None):
-            _load().ref_destroy(self.h)
+            self._lib().ref_destroy(self.h)
             self.h = None
 
     def step_coupled(self, step: int):
-        lib = _load()
+        lib = self._lib()
         kin = self.cfg.motion_state(step)
         lib.ref_compute_force(self.h, *[float(k) for k in kin])
         lib.ref_step(self.h)
 
     def run(self, steps: int, first_step: int = 1, record: bool = False):
-        lib = _load()
+        lib = self._lib()
         hist = []
         out = np.zeros(2, np.float64)
         for n in range(first_step, first_step + steps):
@@ -101,24 +108,24 @@ class RefSim:
 
     def f(self) -> np.ndarray:
         a = np.empty(self.shape + (19,), np.float32)
-        _load().ref_get_f(self.h, _ptr(a))
+        self._lib().ref_get_f(self.h, _ptr(a))
         return a
 
     def set_f(self, f) -> None:
         a = np.ascontiguousarray(f, dtype=np.float32)
-        _load().ref_set_f(self.h, _ptr(a))
+        self._lib().ref_set_f(self.h, _ptr(a))
 
     def macro(self):
         out = [np.empty(self.shape, np.float32) for _ in range(4)]
-        _load().ref_get_macro(self.h, *[_ptr(a) for a in out])
+        self._lib().ref_get_macro(self.h, *[_ptr(a) for a in out])
         return out
 
     def forces(self):
         out = [np.empty(self.shape, np.float32) for _ in range(2)]
-        _load().ref_get_forces(self.h, *[_ptr(a) for a in out])
+        self._lib().ref_get_forces(self.h, *[_ptr(a) for a in out])
         return out
 
     def markers(self) -> dict:
         a = np.empty((9, self.n), np.float32)
-        _load().ref_get_markers(self.h, _ptr(a))
+        self._lib().ref_get_markers(self.h, _ptr(a))
         return dict(zip(("X", "Y", "Z", "VX", "VY", "U", "V", "FX", "FY"), a))
