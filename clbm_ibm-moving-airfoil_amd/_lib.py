@@ -23,7 +23,7 @@ BOX_SUBSLOTS = 64
 BOX_WORDS = 2 * 6 * BOX_SUBSLOTS + 16       # uint64 words of a force-box record
 FORCE_BOX_OFFSET = 2 * 6 * BOX_SUBSLOTS     # word offset of the decoded int32[6] box
 REGION_FLAG_OFFSET = 2 * 6 * BOX_SUBSLOTS + 3   # word offset of the region-violation flag
-ABI_VERSION = 9
+ABI_VERSION = 10
 CLOSURE_GRAD_NOALLOC_MAX_BLOCKS = 1024   # rows of the library's own partial-gradient array
 
 
@@ -125,6 +125,8 @@ _SIGNATURES = {
     "ibm_batch_build": (_I, [_I, _G, _LP, _P, _IP, C.POINTER(IbmBuffers), _P, C.c_size_t,
                              C.POINTER(IbmBatch)]),
     "ibm_batch_compute": (_I, [C.POINTER(IbmBatch), _I, _P]),
+    "ibm_batch_markers": (_I, [C.POINTER(IbmBatch), _I, _P]),
+    "clbm_batch_step_acc": (_I, [C.POINTER(ClbmBatch), C.POINTER(IbmBatch), _I, _P]),
     "closure_param_count": (_I, []),
     "closure_packed_bytes": (C.c_size_t, []),
     "closure_pack": (_I, [_P, _P, _P]),

@@ -3,11 +3,18 @@
 No reference counterpart: the reference runs one case per process
 (main_moving.py).  A sweep holds n instances, each a CLBMSolver and a
 DroneMovingIBM of its own grid, rates and kinematics.  All of them advance
-in lockstep, a few launches per step for a whole group:
+in lockstep, two launches per step for a whole group:
 
-* ``ibm_batch_compute``: the marker kernel and the clamp of every body (the
-  clamp's last workgroup bumps the device step counter);
-* ``clbm_batch_step``: the fused step of every lattice.
+* ``ibm_batch_markers``: the marker kernel of every body (its last workgroup
+  publishes each body's force box and bumps the device step counter);
+* ``clbm_batch_step_acc``: the fused step of every lattice, with the force
+  decoded and clamped straight from the body's fixed-point accumulators (the
+  clamp needs no launch; the bodies' fields fx, fy are rebuilt from the
+  markers when read).
+
+With the float-atomic spread mode (or ``fused_clamp=False``) the pair is
+``ibm_batch_compute`` (markers, then a clamp kernel into the fields, whose
+last workgroup bumps the counter) and ``clbm_batch_step``.
 
 The instances are grouped so that a group's populations fit in the GPU's
 last-level cache, and a run advances one group for all its steps before the
@@ -138,7 +145,7 @@ class BatchSweep:
                  block: int = 256, group: int = 8, first_step: int = 1,
                  cache_bytes: int = 192 << 20, deviation: str = "weights",
                  solo_min_cells: int = DroneMovingIBM.OVERLAP_MIN_CELLS, inlet=True,
-                 lanes: int = 1, lds: int = 0):
+                 lanes: int = 1, lds: int = 0, fused_clamp: bool = True):
         import torch
         self.cfgs = list(cfgs)
         self.device = torch.device(device if device is not None else "cuda")
@@ -157,6 +164,11 @@ class BatchSweep:
             raise ValueError("lanes must be >= 1")
         self.lanes = int(lanes)
         self._lanes = None
+        # two launches per step (markers; lattice with the clamp folded in)
+        # instead of three, with fixed-point spreading; the batched bodies'
+        # fields fx, fy are then rebuilt when read
+        self.fused_clamp = bool(fused_clamp) and \
+            self.ibms[0].ip.spread_mode == _lib.SPREAD_FIXED
         cells = [c.Nx * c.Ny * c.Nz for c in self.cfgs]
         # the groups in flight together share the cache budget
         self.groups = [_Group(g, solo=len(g) == 1 and cells[g[0]] >= solo_min_cells)
@@ -237,8 +249,12 @@ class BatchSweep:
         return _lib.load().clbm_state_after_step(state, C.byref(self.lbms[0].params))
 
     def _launch_step(self, gr: "_Group", state: int, stream) -> int:
-        _lib.call("ibm_batch_compute", C.byref(gr.ib), state, stream)   # advances the counter
-        _lib.call("clbm_batch_step", C.byref(gr.lb), state, stream)
+        if self.fused_clamp:
+            _lib.call("ibm_batch_markers", C.byref(gr.ib), state, stream)  # advances the counter
+            _lib.call("clbm_batch_step_acc", C.byref(gr.lb), C.byref(gr.ib), state, stream)
+        else:
+            _lib.call("ibm_batch_compute", C.byref(gr.ib), state, stream)  # advances the counter
+            _lib.call("clbm_batch_step", C.byref(gr.lb), state, stream)
         return self._next_state(state)
 
     def _solo_steps(self, gr: "_Group", k: int) -> None:
@@ -263,6 +279,8 @@ class BatchSweep:
         for n, i in enumerate(self.ibms):
             if n not in self._solo:
                 i.gen += k
+                if self.fused_clamp and k:
+                    i._fields_stale = True
 
     def step(self, k: int = 1) -> None:
         """k eager steps of the whole batch, group by group."""

@@ -21,6 +21,7 @@
 // so the grid force is the same bit pattern on every run.  The reference's
 // float atomics (airfoil_moving.py:213-214) are kept as IBM_SPREAD_ATOMIC_F32.
 #include "clbm_markers.h"
+#include "clbm_step.h"
 
 #include <algorithm>
 #include <climits>
@@ -262,6 +263,77 @@ __global__ __launch_bounds__(IBLK) void k_clip_batch(const ClipArgs* __restrict_
 }
 
 // ---------------------------------------------------------------------------
+// A batch step without the clamp launch (clbm_batch_step_acc).
+// ---------------------------------------------------------------------------
+// The marker kernel of such a batch (fixed-point spreading): as
+// k_compute_force_batch, then every workgroup counts itself in *done once its
+// atomics on the accumulators and box records are acknowledged (device-scope
+// atomics, seen by every XCD: no release fence, which would write back the
+// XCD's L2 once per workgroup).  The last one reads each body's box of this
+// generation with device-scope atomic loads, one wavefront per body, publishes
+// it (IBM_FORCE_BOX) and advances the step counter.  No workgroup waits for
+// another.
+template <int S, int UMODE, int G>
+__global__ __launch_bounds__(64 * G) void k_compute_force_batch_pub(
+    const IbmArgs* __restrict__ args, const ClipArgs* __restrict__ cargs, int nbody,
+    const uint16_t* __restrict__ inst, const int32_t* __restrict__ first, int32_t* step,
+    int32_t* done, int planes_on) {
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int i = inst[lb];
+  IbmArgs a = args[i];
+  a.pl.on &= planes_on;
+  const int st = *step;
+  const float* k = a.kin + 7 * (size_t)st;
+  const Kin kin{k[0], k[1], k[2], k[3], k[4], k[5], k[6]};
+  compute_group<S, UMODE, IBM_SPREAD_FIXED, G>(a, kin, a.gen0 + (uint32_t)st + 1u, lb - first[i],
+                                               blockIdx.x);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  __shared__ int last;
+  if (threadIdx.x == 0) last = atomicAdd(done, 1) == (int)gridDim.x - 1;
+  __syncthreads();
+  if (!last) return;
+  for (int j = threadIdx.x >> 6; j < nbody; j += G) {
+    const ClipArgs& c = cargs[j];
+    int lo[3], hi[3];
+    const bool any = box_get_wave<true>(c.box, c.gen0 + (uint32_t)st + 1u, lo, hi);
+    if ((threadIdx.x & 63) == 0) {
+      int32_t* out = IBM_FORCE_BOX(c.box);
+      for (int d = 0; d < 3; ++d) {
+        out[d] = any ? lo[d] : INT_MAX;
+        out[3 + d] = any ? hi[d] : INT_MIN;
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicExch(done, 0);
+    atomicAdd(step, 1);
+  }
+}
+
+// The lattice step of such a batch: as k_step_batch, with each lattice's
+// force decoded and clamped straight from its body's accumulators inside the
+// published box (AccForce, the values ibm_clip_forces would write), which it
+// clears for the next step.
+template <int S, int PAR, int BK>
+__global__ __launch_bounds__(BK) void k_step_batch_acc(const StepArgs* __restrict__ sargs,
+                                                       const ClipArgs* __restrict__ cargs,
+                                                       const uint16_t* __restrict__ inst,
+                                                       const int32_t* __restrict__ first,
+                                                       int planes_on) {
+  const int lb = sweep_block<PAR>(blockIdx.x, gridDim.x);
+  const int i = inst[lb];
+  StepArgs a = sargs[i];
+  a.pl.on &= planes_on;
+  const ClipArgs c = cargs[i];
+  const int32_t* fb = IBM_FORCE_BOX(c.box);
+  const AccForce<false> force{(long long*)c.accx, (long long*)c.accy, {fb[0], fb[1], fb[2]},
+                              {fb[3], fb[4], fb[5]}, c.f_max, (double)c.inv_scale};
+  step_cell<S, PAR, 0>(a, (lb - first[i]) * BK + threadIdx.x, force);
+}
+
+// ---------------------------------------------------------------------------
 // FP64 marker force sums, one block, fixed reduction tree (deterministic).
 // ---------------------------------------------------------------------------
 constexpr int SBLK = 1024;
@@ -449,6 +521,47 @@ static void launch_batch(const ibm_batch_t* b, int par, int planes, hipStream_t 
   hipLaunchKernelGGL(k_clip_batch<SMODE>, dim3(b->clip_blocks * b->n), dim3(IBLK), 0, st,
                      (const ClipArgs*)b->clip_args, b->clip_blocks, b->counter,
                      b->counter + BATCH_DONE_OFFSET);
+}
+
+template <int S, int G>
+static void launch_markers_pub(const ibm_batch_t* b, int par, int planes, hipStream_t st) {
+  const IbmArgs* a = (const IbmArgs*)b->args;
+  const ClipArgs* c = (const ClipArgs*)b->clip_args;
+  int32_t* done = b->counter + BATCH_DONE_OFFSET;
+  if (par)
+    hipLaunchKernelGGL((k_compute_force_batch_pub<S, 2, G>), dim3(b->blocks), dim3(64 * G), 0,
+                       st, a, c, b->n, b->inst, b->first, b->counter, done, planes);
+  else
+    hipLaunchKernelGGL((k_compute_force_batch_pub<S, 1, G>), dim3(b->blocks), dim3(64 * G), 0,
+                       st, a, c, b->n, b->inst, b->first, b->counter, done, planes);
+}
+
+template <int S>
+static void launch_markers_pub_g(const ibm_batch_t* b, int par, int planes, hipStream_t st) {
+  if (b->group == 4)
+    launch_markers_pub<S, 4>(b, par, planes, st);
+  else
+    launch_markers_pub<S, 8>(b, par, planes, st);
+}
+
+template <int S, int PAR, int BK>
+static void launch_step_acc(const clbm_batch_t* lb, const ibm_batch_t* ib, int planes,
+                            hipStream_t st) {
+  hipLaunchKernelGGL((k_step_batch_acc<S, PAR, BK>), dim3(lb->blocks), dim3(BK),
+                     (unsigned)lb->lds, st, (const StepArgs*)lb->args,
+                     (const ClipArgs*)ib->clip_args, lb->inst, lb->first, planes);
+}
+
+template <int S>
+static void launch_step_acc_s(const clbm_batch_t* lb, const ibm_batch_t* ib, int par, int planes,
+                              hipStream_t st) {
+  if (lb->block == 128) {
+    if (par) launch_step_acc<S, 1, 128>(lb, ib, planes, st);
+    else launch_step_acc<S, 0, 128>(lb, ib, planes, st);
+  } else {
+    if (par) launch_step_acc<S, 1, 256>(lb, ib, planes, st);
+    else launch_step_acc<S, 0, 256>(lb, ib, planes, st);
+  }
 }
 
 #define CLBM_STORAGE_SWITCH(storage, CALL)                   \
@@ -706,6 +819,42 @@ int ibm_batch_compute(const ibm_batch_t* b, int state, void* stream) {
     CLBM_STORAGE_SWITCH(b->storage, (launch_batch<SS, IBM_SPREAD_ATOMIC_F32>(b, par, planes, st)));
   }
   return check_launch("ibm_batch_compute");
+}
+
+int ibm_batch_markers(const ibm_batch_t* b, int state, void* stream) {
+  if (!b || !b->args || !b->clip_args || !b->inst || !b->first || !b->counter || b->blocks <= 0)
+    return fail(-1, "ibm_batch_markers: batch is not built");
+  if (state & ~(CLBM_STATE_PARITY | CLBM_STATE_PLANES))
+    return fail(-1, "ibm_batch_markers: state has unknown bits");
+  if (b->group != 4 && b->group != 8) return fail(-1, "ibm_batch_markers: bad group");
+  if (b->spread_mode != IBM_SPREAD_FIXED)
+    return fail(-1, "ibm_batch_markers: needs the fixed-point spread mode");
+  const int par = state & CLBM_STATE_PARITY;
+  const int planes = (state & CLBM_STATE_PLANES) ? 1 : 0;
+  hipStream_t st = (hipStream_t)stream;
+  CLBM_STORAGE_SWITCH(b->storage, (launch_markers_pub_g<SS>(b, par, planes, st)));
+  return check_launch("ibm_batch_markers");
+}
+
+int clbm_batch_step_acc(const clbm_batch_t* lb, const ibm_batch_t* ib, int state, void* stream) {
+  if (!lb || !lb->args || !lb->inst || !lb->first || lb->blocks <= 0)
+    return fail(-1, "clbm_batch_step_acc: lattice batch is not built");
+  if (!ib || !ib->clip_args || !ib->counter)
+    return fail(-1, "clbm_batch_step_acc: body batch is not built");
+  if (lb->n != ib->n || lb->storage != ib->storage)
+    return fail(-1, "clbm_batch_step_acc: the two batches differ in size or storage");
+  if (ib->spread_mode != IBM_SPREAD_FIXED)
+    return fail(-1, "clbm_batch_step_acc: needs the fixed-point spread mode");
+  if (state & ~(CLBM_STATE_PARITY | CLBM_STATE_PLANES))
+    return fail(-1, "clbm_batch_step_acc: state has unknown bits");
+  if (lb->block != 128 && lb->block != 256) return fail(-1, "clbm_batch_step_acc: bad block size");
+  if (lb->lds < 0 || lb->lds > 64 * 1024)
+    return fail(-1, "clbm_batch_step_acc: lds must be in [0, 65536]");
+  const int par = state & CLBM_STATE_PARITY;
+  const int planes = (state & CLBM_STATE_PLANES) ? 1 : 0;
+  hipStream_t st = (hipStream_t)stream;
+  CLBM_STORAGE_SWITCH(lb->storage, (launch_step_acc_s<SS>(lb, ib, par, planes, st)));
+  return check_launch("clbm_batch_step_acc");
 }
 
 int ibm_step_region(const clbm_grid_t* g, const clbm_params_t* p, const float* hx,

@@ -85,6 +85,34 @@ __device__ __forceinline__ bool box_get_block(const uint64_t* box, uint32_t gen,
   return ok;
 }
 
+// The same, combined by one wavefront alone (one lane per copy, no LDS, no
+// barrier): every lane of the calling wavefront must call it, each lane gets
+// the result.
+template <bool COHERENT = false>
+__device__ __forceinline__ bool box_get_wave(const uint64_t* box, uint32_t gen, int* lo, int* hi) {
+  static_assert(IBM_BOX_SUBSLOTS == 64, "one lane per box copy");
+  const int lane = threadIdx.x & 63;
+  uint64_t k[6] = {0, 0, 0, 0, 0, 0};
+  if (gen != 0) {
+    const uint64_t* slot = box + 6 * ((gen & 1u) * IBM_BOX_SUBSLOTS + lane);
+    for (int a = 0; a < 6; ++a)
+      k[a] = COHERENT ? __hip_atomic_load(slot + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                      : slot[a];
+  }
+  for (int o = 32; o > 0; o >>= 1)
+    for (int a = 0; a < 6; ++a) {
+      const uint64_t t = __shfl_xor(k[a], o);
+      k[a] = t > k[a] ? t : k[a];
+    }
+  bool ok = gen != 0;
+  for (int a = 0; a < 3; ++a) {
+    ok = ok && (uint32_t)(k[a] >> 32) == gen && (uint32_t)(k[3 + a] >> 32) == gen;
+    lo[a] = INT_MAX - (int)(uint32_t)k[a];
+    hi[a] = (int)(uint32_t)k[3 + a];
+  }
+  return ok;
+}
+
 // Inclusive index range of the 4 stencil cells along one axis.  x is
 // clamped; a periodic axis whose 4 cells wrap takes the whole axis.
 __device__ __forceinline__ void axis_range_clamped(int b, int n, int& lo, int& hi) {

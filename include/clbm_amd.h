@@ -51,7 +51,7 @@
 extern "C" {
 #endif
 
-#define CLBM_ABI_VERSION 9
+#define CLBM_ABI_VERSION 10
 
 /* population storage formats */
 #define CLBM_STORE_F32 0  /* float32 populations, float32 arithmetic (the drop-in) */
@@ -441,6 +441,19 @@ int ibm_batch_build(int n, const clbm_grid_t* grids, const clbm_params_t* params
  * velocity from each lattice's populations in state `state`; then counter
  * += 1 (the clamp kernel's last workgroup). */
 int ibm_batch_compute(const ibm_batch_t* b, int state, void* stream);
+/* A batch step in two launches instead of three (no reference counterpart;
+ * fixed-point spreading only):
+ *   ibm_batch_markers(&ib, state, stream);            markers of every body, then
+ *                                                     box published, counter += 1
+ *   clbm_batch_step_acc(&lb, &ib, state, stream);     every lattice, its force
+ *                                                     decoded and clamped from the
+ *                                                     accumulators
+ * The populations, marker arrays and boxes equal those of ibm_batch_compute +
+ * clbm_batch_step bit for bit; the fields fx, fy are not written (the
+ * accumulators are cleared by the lattice step).  lb and ib must be built
+ * over the same instances in the same order. */
+int ibm_batch_markers(const ibm_batch_t* b, int state, void* stream);
+int clbm_batch_step_acc(const clbm_batch_t* lb, const ibm_batch_t* ib, int state, void* stream);
 
 /* ---- learned closure (no reference counterpart: the reference fixes the
  * third- and fourth-order rates at 1, clbm_moving.py:188-196) -------------

@@ -17,7 +17,8 @@ one here or in test_gpu_fullsize.py).
   x periodic.
 * The batched sweep on its configured draw: 64 seeded instances over the four
   configured grids, graph-replayed; one instance of each grid size equals its
-  sequential run bit for bit, and every instance stays finite.
+  sequential run bit for bit (populations, marker forces; force fields within
+  1e-8), and every instance stays finite.
 """
 
 import os
@@ -212,5 +213,8 @@ def test_configured_64_instance_sweep(pkg, gpu):
     for (f, FX, fx), (lbm, ibm) in zip(got, seq):
         np.testing.assert_array_equal(f, lbm.f.to_numpy())
         np.testing.assert_array_equal(FX, ibm.FX_k.to_numpy())
-        np.testing.assert_array_equal(fx, ibm.fx.to_numpy())
+        # the batched bodies' fields are rebuilt from the markers when read
+        # (the clamp is folded into the batched lattice step); the solo
+        # lattices' come from the clamp kernel
+        np.testing.assert_allclose(fx, ibm.fx.to_numpy(), rtol=0, atol=1e-8)
     print(f"\n[64-instance sweep] {solo[1]} groups ({solo[0]} solo), grids {sizes}")
