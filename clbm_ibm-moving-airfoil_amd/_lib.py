@@ -156,6 +156,16 @@ def load():
         if not os.path.exists(LIB_PATH):
             raise ClbmError(f"{LIB_PATH} is missing: build it with "
                             f"`python {os.path.join(HERE, 'build.py')}` (hipcc, gfx950)")
+        # PyTorch brings its own HIP runtime.  Loaded first, it is the one the
+        # library binds to (the same shared-object name), and the package's
+        # device buffers and the library's launches share one runtime.  Loaded
+        # after the library, torch's libraries would bind to the system HIP
+        # runtime the library brought in, and the library's launches then found
+        # no device (measured on the GPU host).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = C.CDLL(LIB_PATH)
         for name, (res, args) in _SIGNATURES.items():
             fn = getattr(lib, name)

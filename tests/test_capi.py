@@ -111,3 +111,26 @@ def test_product_does_not_import_the_oracle():
             if fn.endswith(".py"):
                 src = open(os.path.join(dirpath, fn)).read()
                 assert "clbm_oracle" not in src and "clbm_ref" not in src, fn
+
+
+@pytest.mark.gpu
+def test_library_loaded_before_torch_touches_the_gpu_still_launches(gpu):
+    """A fresh process that loads the library before anything else (its host
+    entry points need no GPU) and only then uses torch's GPU: the library's
+    launches find the device (it binds to the HIP runtime torch brings)."""
+    import subprocess
+    import sys
+    code = (
+        "import importlib, sys\n"
+        f"sys.path.insert(0, {ROOT!r})\n"
+        "lib = importlib.import_module('clbm_ibm-moving-airfoil_amd._lib')\n"
+        "lib.load()\n"
+        "import torch\n"
+        "pkg = importlib.import_module('clbm_ibm-moving-airfoil_amd')\n"
+        "s = pkg.CLBMSolver(pkg.config.naca_case(40, 24, 4, 8.0, name='order'))\n"
+        "s.step()\n"
+        "rho_min, finite = s.health()\n"
+        "assert finite and abs(rho_min - 1.0) < 0.01, (rho_min, finite)\n"
+        "print('ok')\n")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and r.stdout.strip().endswith("ok"), r.stderr[-2000:]
