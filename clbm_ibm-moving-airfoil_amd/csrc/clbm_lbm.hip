@@ -10,6 +10,8 @@
 // someone reads rho/u.
 #include "clbm_step.h"
 
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -286,6 +288,11 @@ template <int S> struct InitRun {
   }
 };
 
+// Timing events for the next launch of the outside part (clbm_step_split):
+// the runtime stamps them with that kernel's own start and end, unlike events
+// recorded on the stream, whose start stamp is the end of the command before.
+static thread_local hipEvent_t t_outside_ev[2] = {nullptr, nullptr};
+
 // part: -1 the whole grid, 0 the cells outside the region (force-free), 1 the
 // cells inside it.
 template <int S> struct StepRun {
@@ -296,6 +303,12 @@ template <int S> struct StepRun {
     const int n = REG == 2 ? rg.vol : g.ncell;
     const unsigned lds = REG == 1 ? (unsigned)p->outside_lds : 0u;
     const StepArgs a{g, p->s_v, p->s_e, pl, f, fx, fy, fz, box, rg};
+    if (REG == 1 && (t_outside_ev[0] || t_outside_ev[1])) {
+      hipExtLaunchKernelGGL((k_step<S, PAR, FORCE, BOX, REG>), dim3(nblocks(n)), dim3(BLK), lds,
+                            st, t_outside_ev[0], t_outside_ev[1], 0u, a);
+      t_outside_ev[0] = t_outside_ev[1] = nullptr;
+      return;
+    }
     hipLaunchKernelGGL((k_step<S, PAR, FORCE, BOX, REG>), dim3(nblocks(n)), dim3(BLK), lds, st, a);
   }
   template <int PAR, int REG>
@@ -610,17 +623,12 @@ int clbm_step_split(const clbm_grid_t* g, const clbm_params_t* p, void* f, int s
                     const int32_t* force_box, void* main, void* side, void* join_ev,
                     void* start_ev, void* end_ev) {
   if (!join_ev) return fail(-1, "clbm_step_split: NULL join event");
-  hipStream_t m = (hipStream_t)main;
-  if (start_ev) {
-    const hipError_t r = hipEventRecord((hipEvent_t)start_ev, m);
-    if (r != hipSuccess) return fail((int)r, std::string("clbm_step_split: ") + hipGetErrorString(r));
-  }
-  if (int e = clbm_step_part(g, p, f, state, nullptr, nullptr, nullptr, nullptr, region, 0, main))
-    return e;
-  if (end_ev) {
-    const hipError_t r = hipEventRecord((hipEvent_t)end_ev, m);
-    if (r != hipSuccess) return fail((int)r, std::string("clbm_step_split: ") + hipGetErrorString(r));
-  }
+  // the events bracket the outside part's kernel itself (timing events)
+  t_outside_ev[0] = (hipEvent_t)start_ev;
+  t_outside_ev[1] = (hipEvent_t)end_ev;
+  const int e0 = clbm_step_part(g, p, f, state, nullptr, nullptr, nullptr, nullptr, region, 0, main);
+  t_outside_ev[0] = t_outside_ev[1] = nullptr;
+  if (e0) return e0;
   if (int e = clbm_stream_after(main, side, join_ev)) return e;
   return clbm_step_part(g, p, f, state, fx, fy, nullptr, force_box, region, 1, main);
 }

@@ -192,9 +192,11 @@ int ibm_step_region(const clbm_grid_t* g, const clbm_params_t* p, const float* h
 /* clbm_step_part 0, then the marker stream's join, then clbm_step_part 1, as
  * one call (the split step of a coupled loop whose marker kernels ran on
  * `side`): part 0 on `main`; `join_ev` recorded on `side` and waited for by
- * `main`; part 1 on `main` with the force.  start_ev / end_ev (nullable:
- * timing events) are recorded on `main` around part 0.  No reference
- * counterpart (the reference runs one queue). */
+ * `main`; part 1 on `main` with the force.  start_ev / end_ev (nullable,
+ * created with timing) receive the start and end of part 0's kernel itself
+ * (the launch's own time stamps, not a record on the stream, whose start
+ * would be the end of the command before).  No reference counterpart (the
+ * reference runs one queue). */
 int clbm_step_split(const clbm_grid_t* g, const clbm_params_t* p, void* f, int state,
                     const int32_t* region, const float* fx, const float* fy,
                     const int32_t* force_box, void* main, void* side, void* join_ev,
