@@ -334,3 +334,35 @@ def test_one_call_host_path_equals_the_calls_one_by_one(pkg, gpu, overlap):
     np.testing.assert_array_equal(a_ibm.force_history(), b_ibm.force_history())
     np.testing.assert_array_equal(a_ibm.fx.to_numpy(), b_ibm.fx.to_numpy())
     assert a_ibm.region_violations() == 0 and a_ibm.gen == b_ibm.gen == 9
+
+
+@pytest.mark.gpu
+def test_timed_outside_part_changes_nothing_and_times_the_kernel(pkg, gpu):
+    """Timing events handed to a split step (the outside part is then
+    launched with start/stop events of its own) leave the results bit for bit
+    those of an untimed run, and give positive durations no longer than the
+    step's stream time."""
+    import torch
+    cfg = pkg.config.naca_case(96, 48, 4, 12.0, le_x=60.0, name="timed")
+    runs = []
+    for timed in (False, True):
+        lbm = pkg.CLBMSolver(cfg)
+        ibm = pkg.DroneMovingIBM(cfg, overlap=True)
+        evs = []
+        for n in range(1, 8):
+            fx, fy, fz = ibm.compute_force(lbm.ux, lbm.uy, *cfg.motion_state(n))
+            if timed and n % 2:
+                e = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+                for x in e:        # created now, so the step can hand them to the library
+                    x.record()
+                lbm.kernel_events = e
+                evs.append(e)
+            lbm.step(fx, fy, fz)
+            ibm.record_force()
+            assert lbm.last_kernel_cells < lbm.ncell      # the split step ran
+        torch.cuda.synchronize()
+        runs.append((lbm, ibm, evs))
+    (a, ai, _), (b, bi, evs) = runs
+    np.testing.assert_array_equal(a.f.to_numpy(), b.f.to_numpy())
+    np.testing.assert_array_equal(ai.force_history(), bi.force_history())
+    assert len(evs) == 4 and all(0 < e0.elapsed_time(e1) < 50.0 for e0, e1 in evs)
