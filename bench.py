@@ -355,8 +355,9 @@ def measure(pkg, cfg, storage, steps, warmup, dev, tune, deviation="weights", co
 
 
 def measure_coupled(pkg, cfg, lbm, ibm, steps, warmup, knobs):
-    """measure() through coupled.step_coupled; HIP events bracket its two
-    launches on every 4th timed step."""
+    """measure() through coupled.step_coupled; HIP events on the stream
+    bracket its two launches on every EVENTS_EVERY-th timed step (as
+    measure() times the split step's outside part)."""
     import torch
     cm = importlib.import_module(PKG + ".coupled")
     il, spread, spins = knobs
@@ -370,7 +371,7 @@ def measure_coupled(pkg, cfg, lbm, ibm, steps, warmup, knobs):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(steps):
-        timed = i % 4 == 0
+        timed = i % max(EVENTS_EVERY, 1) == 0
         if timed:
             e = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
             e[0].record(st)
