@@ -48,22 +48,24 @@ __host__ __device__ constexpr int perm32(int slot) {
   return (slot & 7) < 4 ? 4 * (slot >> 3) + (slot & 7) : 16 + 4 * (slot >> 3) + (slot & 7) - 4;
 }
 
-// tanh(y) = 1 - 2 / (exp(2y) + 1), with the hardware base-2 exponential,
-// exp(2y) = 2^(2 y log2 e), and the hardware reciprocal: five instructions.
-// It saturates to +-1 (2^y overflows to infinity or underflows to 0) and
-// is within a few 1e-7 of tanh, far below the bf16 rounding the hidden
-// activations take next.
-__device__ __forceinline__ float tanh_fast(float y) {
-  const float e = __builtin_amdgcn_exp2f(2.8853900817779268f * y);
-  return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
+// GELU, tanh form: x/2 (1 + tanh(y)), y = sqrt(2/pi) (x + 0.044715 x^3).
+// Since (1 + tanh(y)) / 2 = 1 / (1 + exp(-2y)), it is x s with
+// s = 1 / (1 + 2^(x (c1 + c3 x^2))), c1 = -2 log2(e) sqrt(2/pi), c3 =
+// 0.044715 c1: seven instructions with the hardware base-2 exponential and
+// reciprocal, within a few 1e-7 relative of the tanh form (far below the bf16
+// rounding the hidden activations take next).  Large |x| saturate: 2^+big is
+// infinity (s = 0), 2^-big is 0 (s = 1).
+constexpr float GELU_C1 = -2.302208198144325f;   // -2 log2(e) sqrt(2/pi)
+constexpr float GELU_C3 = -0.1029432395800235f;  // 0.044715 GELU_C1
+__device__ __forceinline__ float gelu_sig(float x) {
+  const float e = __builtin_amdgcn_exp2f(x * __builtin_fmaf(GELU_C3, x * x, GELU_C1));
+  return __builtin_amdgcn_rcpf(e + 1.0f);
 }
-__device__ __forceinline__ float gelu(float x) {
-  return 0.5f * x * (1.0f + tanh_fast(0.7978845608028654f * (x + 0.044715f * x * x * x)));
-}
+__device__ __forceinline__ float gelu(float x) { return x * gelu_sig(x); }
+// d/dx x s = s + x s (1 - s) 2 y'(x), y' = sqrt(2/pi) (1 + 3 0.044715 x^2)
 __device__ __forceinline__ float gelu_grad(float x) {
-  const float t = tanh_fast(0.7978845608028654f * (x + 0.044715f * x * x * x));
-  return 0.5f * (1.0f + t) +
-         0.5f * x * (1.0f - t * t) * 0.7978845608028654f * (1.0f + 3.0f * 0.044715f * x * x);
+  const float s = gelu_sig(x);
+  return s + 2.0f * x * s * (1.0f - s) * 0.7978845608028654f * (1.0f + 3.0f * 0.044715f * x * x);
 }
 
 __device__ __forceinline__ f4v mfma(bf16x8 a, bf16x8 b, f4v c) {
