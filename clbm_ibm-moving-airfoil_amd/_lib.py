@@ -23,7 +23,7 @@ BOX_SUBSLOTS = 64
 BOX_WORDS = 2 * 6 * BOX_SUBSLOTS + 16       # uint64 words of a force-box record
 FORCE_BOX_OFFSET = 2 * 6 * BOX_SUBSLOTS     # word offset of the decoded int32[6] box
 REGION_FLAG_OFFSET = 2 * 6 * BOX_SUBSLOTS + 3   # word offset of the region-violation flag
-ABI_VERSION = 10
+ABI_VERSION = 11
 CLOSURE_GRAD_NOALLOC_MAX_BLOCKS = 1024   # rows of the library's own partial-gradient array
 
 
@@ -63,14 +63,14 @@ class IbmBuffers(C.Structure):
     _fields_ = [("n", C.c_int32), ("order", C.c_void_p)] + [
         (k, C.c_void_p) for k in ("X0", "Y0", "Z0", "X", "Y", "Z", "VX", "VY", "U", "V", "FX",
                                   "FY", "acc_x", "acc_y", "box", "fx", "fy", "kin")] + [
-        ("gen0", C.c_uint32)]
+        ("gen0", C.c_uint32), ("regions", C.c_void_p)]
 
 
 class IbmBatch(C.Structure):
     _fields_ = [("n", C.c_int32), ("storage", C.c_int32), ("spread_mode", C.c_int32),
                 ("group", C.c_int32), ("blocks", C.c_int32), ("clip_blocks", C.c_int32),
                 ("args", C.c_void_p), ("clip_args", C.c_void_p), ("inst", C.c_void_p),
-                ("first", C.c_void_p), ("counter", C.c_void_p)]
+                ("first", C.c_void_p), ("counter", C.c_void_p), ("regions", C.c_int32)]
 
 
 _P = C.c_void_p

@@ -404,7 +404,7 @@ class DroneMovingIBM:
 
     def force_box(self):
         """The (x0, y0, z0, x1, y1, z1) box of the cells the last compute_force reached."""
-        self._join()
+        self._refresh_fields()     # a step that left the fields unwritten left the box too
         return self.box[_lib.FORCE_BOX_OFFSET:_lib.FORCE_BOX_OFFSET + 3].cpu().numpy().view(
             np.int32).copy()
 

@@ -5,12 +5,15 @@ No reference counterpart: the reference runs one case per process
 DroneMovingIBM of its own grid, rates and kinematics.  All of them advance
 in lockstep, two launches per step for a whole group:
 
-* ``ibm_batch_markers``: the marker kernel of every body (its last workgroup
-  publishes each body's force box and bumps the device step counter);
+* ``ibm_batch_markers``: the marker kernel of every body, its stencils
+  checked against the region that the body's table predicts for the step
+  (``ibm_step_region``, as for the split step), and the next value of the
+  device step counter;
 * ``clbm_batch_step_acc``: the fused step of every lattice, with the force
-  decoded and clamped straight from the body's fixed-point accumulators (the
-  clamp needs no launch; the bodies' fields fx, fy are rebuilt from the
-  markers when read).
+  decoded and clamped straight from the body's fixed-point accumulators
+  inside that region (the clamp needs no launch; the bodies' fields fx, fy
+  and force box are rebuilt from the markers when read).  A run raises
+  ClbmError when some body's stencils left their region.
 
 With the float-atomic spread mode (or ``fused_clamp=False``) the pair is
 ``ibm_batch_compute`` (markers, then a clamp kernel into the fields, whose
@@ -89,6 +92,25 @@ def kinematics_table(cfg, first_step: int, steps: int) -> np.ndarray:
         t[j] = (np.float32(xc), np.float32(yc), np.float32(math.cos(a)), np.float32(math.sin(a)),
                 np.float32(vx), np.float32(vy), np.float32(om))
     return t
+
+
+def region_table(lbm, ibm, cfg, first_step: int, steps: int) -> np.ndarray:
+    """(steps, 6) int32: for the steps first_step .. first_step + steps - 1 of
+    cfg's motion, the region (x0, y0, z0, x1, y1, z1) that ibm_step_region
+    predicts for the body's stencils (the whole lattice when it predicts
+    none: the body off the grid or over half of it)."""
+    out = np.empty((steps, 6), dtype=np.int32)
+    r = (C.c_int32 * 6)()
+    fn = _lib.fn("ibm_step_region")
+    whole = (0, 0, 0, lbm.Nx - 1, lbm.Ny - 1, lbm.Nz - 1)
+    for s in range(steps):
+        xc, yc, th = cfg.motion_state(first_step + s)[:3]
+        rc = fn(C.byref(lbm.grid), C.byref(lbm.params), *ibm._hull_args, float(xc), float(yc),
+                float(th), r)
+        if rc < 0:
+            _lib.check(rc, "ibm_step_region")
+        out[s] = tuple(r) if rc == 1 else whole
+    return out
 
 
 def cache_groups(cfgs, storage: str, cache_bytes: int):
@@ -178,6 +200,13 @@ class BatchSweep:
             if gr.solo:
                 self._solo.add(gr.idx[0])
                 self.ibms[gr.idx[0]].overlap = True
+        # per batched body and step of the kinematics table, the region its
+        # stencils stay in (predicted as for the split step); the fused
+        # lattice step reads the force inside it
+        self._regions = [None if n in self._solo or not self.fused_clamp else
+                         torch.from_numpy(region_table(self.lbms[n], self.ibms[n], self.cfgs[n],
+                                                       first_step, self.capacity)).to(self.device)
+                         for n in range(len(self.cfgs))]
         self.build(block, group, lds)
 
     # -- tables ---------------------------------------------------------------
@@ -230,6 +259,7 @@ class BatchSweep:
             b.fx, b.fy = ibm.fx._t.data_ptr(), ibm.fy._t.data_ptr()
             b.kin = self._kin[gr.idx[k]].data_ptr()
             b.gen0 = ibm.gen - self.steps_done    # the generation of counter value c is gen0 + 1 + c
+            b.regions = self._regions[gr.idx[k]].data_ptr() if self.fused_clamp else None
         first = self.ibms[0].ip
         ip = _lib.IbmParams(float(first.f_max), int(first.wrap_weight_quirk),
                             int(first.spread_mode), 0, self.group, 0)
@@ -240,8 +270,12 @@ class BatchSweep:
         gr.ib = _lib.IbmBatch()
         _lib.call("ibm_batch_build", n, grids, params, f, C.byref(ip), bufs,
                   C.c_void_p(gr.iwork.data_ptr()), ib, C.byref(gr.ib))
-        # the counter starts at 0: move it to the steps already taken
-        gr.iwork.view(torch.int32)[(gr.ib.counter - gr.iwork.data_ptr()) // 4] = self.steps_done
+        # the counter starts at 0: move it to the steps already taken (with
+        # region tables a step of AA parity p reads counter[p] and writes
+        # counter[1 - p]; both start at the steps taken, whatever the parity)
+        off = (gr.ib.counter - gr.iwork.data_ptr()) // 4
+        cnt = gr.iwork.view(torch.int32)
+        cnt[off:off + (2 if self.fused_clamp else 1)] = self.steps_done
         gr.graph = None
 
     # -- stepping -------------------------------------------------------------
@@ -297,6 +331,7 @@ class BatchSweep:
                 s = self._launch_step(gr, s, st)
         self.state = self._state_after(k)
         self._advance_objects(k)
+        self.check_regions()
 
     def run(self, k: int, graph: bool = True, times=None) -> None:
         """k steps, group by group; with ``graph``, a group's steps after the
@@ -339,8 +374,27 @@ class BatchSweep:
                     main.wait_stream(st)
         self.state = self._state_after(k)
         self._advance_objects(k)
-        for i in self._solo:   # the split steps' stencils stayed in their regions
-            self.ibms[i].check_region()
+        self.check_regions()
+
+    def check_regions(self) -> None:
+        """Raise ClbmError when a body's stencils left the region that its
+        step read the force from (the split step of a solo group, or the
+        region table of a batched body); one synchronisation."""
+        import torch
+        checked = [n for n in range(len(self.ibms)) if n in self._solo or self.fused_clamp]
+        if not checked:
+            return
+        for n in checked:
+            self.ibms[n]._join()
+        flags = torch.stack([self.ibms[n].box[_lib.REGION_FLAG_OFFSET] for n in checked]).cpu()
+        for n, g in zip(checked, flags.tolist()):
+            if not g:
+                continue
+            if n in self._solo:
+                self.ibms[n].check_region()
+            raise _lib.ClbmError(
+                f"instance {n}: the stencils of generation {g} left the region of its "
+                f"step's table; its lattice step missed part of the force")
 
     def _units(self, gr: "_Group", k: int, stream, times):
         """Issue a group's k steps on ``stream``, yielding after each unit of

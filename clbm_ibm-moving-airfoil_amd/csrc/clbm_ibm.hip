@@ -178,6 +178,7 @@ struct ClipArgs {
   float *fx, *fy;
   uint64_t* box;
   uint32_t gen0;  // batched runs, as IbmArgs::gen0
+  const int32_t* regions;  // batched runs, as IbmArgs::regions
 };
 
 // Block `bid` of `nblk` working on one body.
@@ -263,73 +264,60 @@ __global__ __launch_bounds__(IBLK) void k_clip_batch(const ClipArgs* __restrict_
 }
 
 // ---------------------------------------------------------------------------
-// A batch step without the clamp launch (clbm_batch_step_acc).
+// A batch step without the clamp launch (ibm_batch_markers +
+// clbm_batch_step_acc).  Every body carries a table of its predicted region
+// per step (from the kinematics, as the split step predicts it); the lattice
+// step takes the force inside that region straight from the accumulators, so
+// no launch has to publish the box of the cells the stencils reached.
+//
+// The step counter is a pair indexed by the step's AA parity: a step of
+// parity p reads cnt[p] and its marker kernel's block 0 writes cnt[1 - p] =
+// cnt[p] + 1 for the next step (which has the other parity), so no workgroup
+// waits for another or counts the others.
 // ---------------------------------------------------------------------------
-// The marker kernel of such a batch (fixed-point spreading): as
-// k_compute_force_batch, then every workgroup counts itself in *done once its
-// atomics on the accumulators and box records are acknowledged (device-scope
-// atomics, seen by every XCD: no release fence, which would write back the
-// XCD's L2 once per workgroup).  The last one reads each body's box of this
-// generation with device-scope atomic loads, one wavefront per body, publishes
-// it (IBM_FORCE_BOX) and advances the step counter.  No workgroup waits for
-// another.
-template <int S, int UMODE, int G>
-__global__ __launch_bounds__(64 * G) void k_compute_force_batch_pub(
-    const IbmArgs* __restrict__ args, const ClipArgs* __restrict__ cargs, int nbody,
-    const uint16_t* __restrict__ inst, const int32_t* __restrict__ first, int32_t* step,
-    int32_t* done, int planes_on) {
+template <int S, int PAR, int G>
+__global__ __launch_bounds__(64 * G) void k_compute_force_batch_reg(
+    const IbmArgs* __restrict__ args, const uint16_t* __restrict__ inst,
+    const int32_t* __restrict__ first, int32_t* cnt, int planes_on) {
   const int lb = xcd_block(blockIdx.x, gridDim.x);
   const int i = inst[lb];
   IbmArgs a = args[i];
   a.pl.on &= planes_on;
-  const int st = *step;
+  const int st = cnt[PAR];
   const float* k = a.kin + 7 * (size_t)st;
   const Kin kin{k[0], k[1], k[2], k[3], k[4], k[5], k[6]};
-  compute_group<S, UMODE, IBM_SPREAD_FIXED, G>(a, kin, a.gen0 + (uint32_t)st + 1u, lb - first[i],
-                                               blockIdx.x);
-  __builtin_amdgcn_s_waitcnt(0);
-  __syncthreads();
-  __shared__ int last;
-  if (threadIdx.x == 0) last = atomicAdd(done, 1) == (int)gridDim.x - 1;
-  __syncthreads();
-  if (!last) return;
-  for (int j = threadIdx.x >> 6; j < nbody; j += G) {
-    const ClipArgs& c = cargs[j];
-    int lo[3], hi[3];
-    const bool any = box_get_wave<true>(c.box, c.gen0 + (uint32_t)st + 1u, lo, hi);
-    if ((threadIdx.x & 63) == 0) {
-      int32_t* out = IBM_FORCE_BOX(c.box);
-      for (int d = 0; d < 3; ++d) {
-        out[d] = any ? lo[d] : INT_MAX;
-        out[3 + d] = any ? hi[d] : INT_MIN;
-      }
-    }
+  // the stencils are checked against the region the lattice step will read
+  const int32_t* r = a.regions + 6 * (size_t)st;
+  a.rc.on = 1;
+  for (int d = 0; d < 3; ++d) {
+    a.rc.lo[d] = r[d];
+    a.rc.hi[d] = r[3 + d];
   }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    atomicExch(done, 0);
-    atomicAdd(step, 1);
-  }
+  compute_group<S, 1 + PAR, IBM_SPREAD_FIXED, G>(a, kin, a.gen0 + (uint32_t)st + 1u,
+                                                 lb - first[i], blockIdx.x);
+  if (blockIdx.x == 0 && threadIdx.x == 0) cnt[1 - PAR] = st + 1;
 }
 
 // The lattice step of such a batch: as k_step_batch, with each lattice's
 // force decoded and clamped straight from its body's accumulators inside the
-// published box (AccForce, the values ibm_clip_forces would write), which it
-// clears for the next step.
+// body's predicted region of this step (AccForce: the values ibm_clip_forces
+// would write; zero where no stencil reached), which it clears for the next
+// step.
 template <int S, int PAR, int BK>
 __global__ __launch_bounds__(BK) void k_step_batch_acc(const StepArgs* __restrict__ sargs,
                                                        const ClipArgs* __restrict__ cargs,
                                                        const uint16_t* __restrict__ inst,
                                                        const int32_t* __restrict__ first,
+                                                       const int32_t* __restrict__ cnt,
                                                        int planes_on) {
   const int lb = sweep_block<PAR>(blockIdx.x, gridDim.x);
   const int i = inst[lb];
   StepArgs a = sargs[i];
   a.pl.on &= planes_on;
   const ClipArgs c = cargs[i];
-  const int32_t* fb = IBM_FORCE_BOX(c.box);
-  const AccForce<false> force{(long long*)c.accx, (long long*)c.accy, {fb[0], fb[1], fb[2]},
-                              {fb[3], fb[4], fb[5]}, c.f_max, (double)c.inv_scale};
+  const int32_t* r = c.regions + 6 * (size_t)cnt[PAR];
+  const AccForce<false> force{(long long*)c.accx, (long long*)c.accy, {r[0], r[1], r[2]},
+                              {r[3], r[4], r[5]}, c.f_max, (double)c.inv_scale};
   step_cell<S, PAR, 0>(a, (lb - first[i]) * BK + threadIdx.x, force);
 }
 
@@ -524,24 +512,22 @@ static void launch_batch(const ibm_batch_t* b, int par, int planes, hipStream_t 
 }
 
 template <int S, int G>
-static void launch_markers_pub(const ibm_batch_t* b, int par, int planes, hipStream_t st) {
+static void launch_markers_reg(const ibm_batch_t* b, int par, int planes, hipStream_t st) {
   const IbmArgs* a = (const IbmArgs*)b->args;
-  const ClipArgs* c = (const ClipArgs*)b->clip_args;
-  int32_t* done = b->counter + BATCH_DONE_OFFSET;
   if (par)
-    hipLaunchKernelGGL((k_compute_force_batch_pub<S, 2, G>), dim3(b->blocks), dim3(64 * G), 0,
-                       st, a, c, b->n, b->inst, b->first, b->counter, done, planes);
+    hipLaunchKernelGGL((k_compute_force_batch_reg<S, 1, G>), dim3(b->blocks), dim3(64 * G), 0, st,
+                       a, b->inst, b->first, b->counter, planes);
   else
-    hipLaunchKernelGGL((k_compute_force_batch_pub<S, 1, G>), dim3(b->blocks), dim3(64 * G), 0,
-                       st, a, c, b->n, b->inst, b->first, b->counter, done, planes);
+    hipLaunchKernelGGL((k_compute_force_batch_reg<S, 0, G>), dim3(b->blocks), dim3(64 * G), 0, st,
+                       a, b->inst, b->first, b->counter, planes);
 }
 
 template <int S>
-static void launch_markers_pub_g(const ibm_batch_t* b, int par, int planes, hipStream_t st) {
+static void launch_markers_reg_g(const ibm_batch_t* b, int par, int planes, hipStream_t st) {
   if (b->group == 4)
-    launch_markers_pub<S, 4>(b, par, planes, st);
+    launch_markers_reg<S, 4>(b, par, planes, st);
   else
-    launch_markers_pub<S, 8>(b, par, planes, st);
+    launch_markers_reg<S, 8>(b, par, planes, st);
 }
 
 template <int S, int PAR, int BK>
@@ -549,7 +535,7 @@ static void launch_step_acc(const clbm_batch_t* lb, const ibm_batch_t* ib, int p
                             hipStream_t st) {
   hipLaunchKernelGGL((k_step_batch_acc<S, PAR, BK>), dim3(lb->blocks), dim3(BK),
                      (unsigned)lb->lds, st, (const StepArgs*)lb->args,
-                     (const ClipArgs*)ib->clip_args, lb->inst, lb->first, planes);
+                     (const ClipArgs*)ib->clip_args, lb->inst, lb->first, ib->counter, planes);
 }
 
 template <int S>
@@ -788,8 +774,11 @@ int ibm_batch_build(int n, const clbm_grid_t* grids, const clbm_params_t* params
     a.box = m.box;
     a.kin = m.kin;
     a.gen0 = m.gen0;
+    a.regions = m.regions;
+    if ((m.regions != nullptr) != (bufs[0].regions != nullptr))
+      return fail(-1, "ibm_batch_build: either every body or none carries a region table");
     args[i] = a;
-    clip[i] = ClipArgs{a.g, ip->f_max, inv, m.acc_x, m.acc_y, m.fx, m.fy, m.box, m.gen0};
+    clip[i] = ClipArgs{a.g, ip->f_max, inv, m.acc_x, m.acc_y, m.fx, m.fy, m.box, m.gen0, m.regions};
     first[i] = blk;
     const int nb = (m.n + group - 1) / group;
     for (int j = 0; j < nb; ++j) inst[blk + j] = (uint16_t)i;
@@ -800,7 +789,7 @@ int ibm_batch_build(int n, const clbm_grid_t* grids, const clbm_params_t* params
   unsigned char* w = (unsigned char*)work;
   *out = ibm_batch_t{n, grids[0].storage, ip->spread_mode, group, L.blocks, CLIP_BPI, w + L.args,
                      w + L.clip, (uint16_t*)(w + L.inst), (int32_t*)(w + L.first),
-                     (int32_t*)(w + L.counter)};
+                     (int32_t*)(w + L.counter), bufs[0].regions != nullptr ? 1 : 0};
   return 0;
 }
 
@@ -829,10 +818,11 @@ int ibm_batch_markers(const ibm_batch_t* b, int state, void* stream) {
   if (b->group != 4 && b->group != 8) return fail(-1, "ibm_batch_markers: bad group");
   if (b->spread_mode != IBM_SPREAD_FIXED)
     return fail(-1, "ibm_batch_markers: needs the fixed-point spread mode");
+  if (!b->regions) return fail(-1, "ibm_batch_markers: the bodies carry no region tables");
   const int par = state & CLBM_STATE_PARITY;
   const int planes = (state & CLBM_STATE_PLANES) ? 1 : 0;
   hipStream_t st = (hipStream_t)stream;
-  CLBM_STORAGE_SWITCH(b->storage, (launch_markers_pub_g<SS>(b, par, planes, st)));
+  CLBM_STORAGE_SWITCH(b->storage, (launch_markers_reg_g<SS>(b, par, planes, st)));
   return check_launch("ibm_batch_markers");
 }
 
@@ -845,6 +835,7 @@ int clbm_batch_step_acc(const clbm_batch_t* lb, const ibm_batch_t* ib, int state
     return fail(-1, "clbm_batch_step_acc: the two batches differ in size or storage");
   if (ib->spread_mode != IBM_SPREAD_FIXED)
     return fail(-1, "clbm_batch_step_acc: needs the fixed-point spread mode");
+  if (!ib->regions) return fail(-1, "clbm_batch_step_acc: the bodies carry no region tables");
   if (state & ~(CLBM_STATE_PARITY | CLBM_STATE_PLANES))
     return fail(-1, "clbm_batch_step_acc: state has unknown bits");
   if (lb->block != 128 && lb->block != 256) return fail(-1, "clbm_batch_step_acc: bad block size");

@@ -85,34 +85,6 @@ __device__ __forceinline__ bool box_get_block(const uint64_t* box, uint32_t gen,
   return ok;
 }
 
-// The same, combined by one wavefront alone (one lane per copy, no LDS, no
-// barrier): every lane of the calling wavefront must call it, each lane gets
-// the result.
-template <bool COHERENT = false>
-__device__ __forceinline__ bool box_get_wave(const uint64_t* box, uint32_t gen, int* lo, int* hi) {
-  static_assert(IBM_BOX_SUBSLOTS == 64, "one lane per box copy");
-  const int lane = threadIdx.x & 63;
-  uint64_t k[6] = {0, 0, 0, 0, 0, 0};
-  if (gen != 0) {
-    const uint64_t* slot = box + 6 * ((gen & 1u) * IBM_BOX_SUBSLOTS + lane);
-    for (int a = 0; a < 6; ++a)
-      k[a] = COHERENT ? __hip_atomic_load(slot + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                      : slot[a];
-  }
-  for (int o = 32; o > 0; o >>= 1)
-    for (int a = 0; a < 6; ++a) {
-      const uint64_t t = __shfl_xor(k[a], o);
-      k[a] = t > k[a] ? t : k[a];
-    }
-  bool ok = gen != 0;
-  for (int a = 0; a < 3; ++a) {
-    ok = ok && (uint32_t)(k[a] >> 32) == gen && (uint32_t)(k[3 + a] >> 32) == gen;
-    lo[a] = INT_MAX - (int)(uint32_t)k[a];
-    hi[a] = (int)(uint32_t)k[3 + a];
-  }
-  return ok;
-}
-
 // Inclusive index range of the 4 stencil cells along one axis.  x is
 // clamped; a periodic axis whose 4 cells wrap takes the whole axis.
 __device__ __forceinline__ void axis_range_clamped(int b, int n, int& lo, int& hi) {
@@ -276,6 +248,7 @@ struct IbmArgs {
   RegionCheck rc;
   const float* kin;  // batched runs: per step xc, yc, cos, sin, vx, vy, omega
   uint32_t gen0;     // batched runs: generation of step 0 of the step counter, minus one
+  const int32_t* regions;  // batched runs: per step the predicted region (6 int32), or NULL
 };
 
 // Marker group `grp` (G markers, one wavefront each) of generation gen;
@@ -339,6 +312,22 @@ __device__ __forceinline__ void compute_group(const IbmArgs& a, const Kin& kin, 
     }
   }
   __syncthreads();
+  // the box of the cells this generation's stencils reach
+  if (box && threadIdx.x == 0) {
+    int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+    for (int i = 0; i < G; ++i)
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = min(lo[a], srng[i][a]);
+        hi[a] = max(hi[a], srng[i][3 + a]);
+      }
+    box_grow(box, gen, lo, hi, sub);
+    if (rc.on && lo[0] != INT_MAX) {
+      bool out = false;
+      for (int a = 0; a < 3; ++a) out |= (lo[a] < rc.lo[a]) | (hi[a] > rc.hi[a]);
+      if (out)
+        atomicMax((unsigned long long*)IBM_REGION_FLAG(box), (unsigned long long)gen);
+    }
+  }
   // the tile: bounding box of the workgroup's unwrapped stencil coordinates
   int o[3] = {INT_MAX, INT_MAX, INT_MAX}, e[3] = {INT_MIN, INT_MIN, INT_MIN};
   for (int i = 0; i < G; ++i) {
@@ -432,22 +421,6 @@ __device__ __forceinline__ void compute_group(const IbmArgs& a, const Kin& kin, 
       const float a = ((const float*)sacc)[j], b = ((const float*)sacc)[TC + j];
       if (a != 0.0f) atomicAdd((float*)((unsigned long long*)accx + c), a);
       if (b != 0.0f) atomicAdd((float*)((unsigned long long*)accy + c), b);
-    }
-  }
-  // the box of the cells this generation's stencils reach
-  if (box && threadIdx.x == 0) {
-    int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
-    for (int i = 0; i < G; ++i)
-      for (int a = 0; a < 3; ++a) {
-        lo[a] = min(lo[a], srng[i][a]);
-        hi[a] = max(hi[a], srng[i][3 + a]);
-      }
-    box_grow(box, gen, lo, hi, sub);
-    if (rc.on && lo[0] != INT_MAX) {
-      bool out = false;
-      for (int a = 0; a < 3; ++a) out |= (lo[a] < rc.lo[a]) | (hi[a] > rc.hi[a]);
-      if (out)
-        atomicMax((unsigned long long*)IBM_REGION_FLAG(box), (unsigned long long)gen);
     }
   }
 }

@@ -51,7 +51,7 @@
 extern "C" {
 #endif
 
-#define CLBM_ABI_VERSION 10
+#define CLBM_ABI_VERSION 11
 
 /* population storage formats */
 #define CLBM_STORE_F32 0  /* float32 populations, float32 arithmetic (the drop-in) */
@@ -420,6 +420,10 @@ typedef struct {
   const float* kin; /* device: for each step s of the counter, 7 floats
                        xc, yc, cos(theta), sin(theta), vx, vy, omega */
   uint32_t gen0;    /* the body's generation is gen0 + 1 + counter */
+  const int32_t* regions; /* device, nullable: for each step of the counter, 6 int32 x0, y0, z0,
+                             x1, y1, z1 of a region that holds the body's stencils (e.g. from
+                             ibm_step_region); needed by ibm_batch_markers / clbm_batch_step_acc,
+                             and given for every body of a batch or for none */
 } ibm_buffers_t;
 
 typedef struct {
@@ -429,7 +433,9 @@ typedef struct {
   uint16_t* inst;
   int32_t* first;
   int32_t* counter;   /* device: steps taken by the batch (followed by the clamp
-                         kernel's own count in the same 256 bytes) */
+                         kernel's own count in the same 256 bytes); with region
+                         tables, counter[p] holds it before a step of AA parity p */
+  int32_t regions;    /* the bodies carry region tables */
 } ibm_batch_t;
 
 /* Device bytes of the body tables (the step counter included). */
@@ -444,16 +450,20 @@ int ibm_batch_build(int n, const clbm_grid_t* grids, const clbm_params_t* params
  * += 1 (the clamp kernel's last workgroup). */
 int ibm_batch_compute(const ibm_batch_t* b, int state, void* stream);
 /* A batch step in two launches instead of three (no reference counterpart;
- * fixed-point spreading only):
- *   ibm_batch_markers(&ib, state, stream);            markers of every body, then
- *                                                     box published, counter += 1
- *   clbm_batch_step_acc(&lb, &ib, state, stream);     every lattice, its force
- *                                                     decoded and clamped from the
- *                                                     accumulators
- * The populations, marker arrays and boxes equal those of ibm_batch_compute +
- * clbm_batch_step bit for bit; the fields fx, fy are not written (the
- * accumulators are cleared by the lattice step).  lb and ib must be built
- * over the same instances in the same order. */
+ * fixed-point spreading; bodies with region tables):
+ *   ibm_batch_markers(&ib, state, stream);            markers of every body (stencils
+ *                                                     checked against the step's region,
+ *                                                     IBM_REGION_FLAG), next counter set
+ *   clbm_batch_step_acc(&lb, &ib, state, stream);     every lattice, its force decoded
+ *                                                     and clamped from the accumulators
+ *                                                     inside the region
+ * The populations and marker arrays equal those of ibm_batch_compute +
+ * clbm_batch_step bit for bit while the stencils stay in the regions; the
+ * fields fx, fy and the published force box are not written (the
+ * accumulators are cleared by the lattice step).  The step counter is
+ * counter[p] before a step of AA parity p (the caller sets counter[p] for
+ * the first step).  lb and ib must be built over the same instances in the
+ * same order. */
 int ibm_batch_markers(const ibm_batch_t* b, int state, void* stream);
 int clbm_batch_step_acc(const clbm_batch_t* lb, const ibm_batch_t* ib, int state, void* stream);
 

@@ -92,7 +92,8 @@ def _assert_same(batch_objs, seq_objs, exact_fields=False):
 def test_batch_equals_each_instance_alone(pkg, gpu, block, group, lds, fused):
     """Two launches per step (the clamp folded into the lattice step) or
     three (markers, clamp into the fields, lattice): populations, marker
-    forces and published force boxes equal the per-instance runs bit for bit."""
+    forces and force boxes equal the per-instance runs bit for bit (the
+    two-launch step leaves the fields and box to be rebuilt when read)."""
     bm = load_sub("batch")
     cfgs = _small(pkg)
     sweep = bm.BatchSweep(cfgs, steps=16, block=block, group=group, lds=lds, fused_clamp=fused)
@@ -100,9 +101,34 @@ def test_batch_equals_each_instance_alone(pkg, gpu, block, group, lds, fused):
     sweep.step(7)
     seq, _ = bm.sequential(cfgs, 7)
     for (bl, bi), (sl, si) in zip(zip(sweep.lbms, sweep.ibms), seq):
-        np.testing.assert_array_equal(bi.force_box(), si.force_box())
         assert bi._fields_stale == fused
+        np.testing.assert_array_equal(bi.force_box(), si.force_box())   # rebuilds the fields
     _assert_same(list(zip(sweep.lbms, sweep.ibms)), seq, exact_fields=not fused)
+
+
+@pytest.mark.gpu
+def test_batched_bodies_stay_in_their_region_tables(pkg, gpu):
+    """The fused lattice step reads each batched body's force inside the
+    region its table predicts for that step: the predicted regions hold the
+    stencils on a lattice larger than twice the body (no violation), and a
+    table shrunk to one cell makes the run raise instead of dropping force."""
+    bm = load_sub("batch")
+    lib = load_sub("_lib")
+    cfgs = [pkg.config.naca_case(96, 48, 6, 9.6, Re=200, name="wide"),
+            pkg.config.naca_case(80, 40, 6, 8.0, Re=300, k=0.9, name="wide2")]
+    sweep = bm.BatchSweep(cfgs, steps=12, solo_min_cells=1 << 30)
+    assert all(not gr.solo for gr in sweep.groups)
+    # at least one body's table is narrower than its lattice
+    whole = [(0, 0, 0, c.Nx - 1, c.Ny - 1, c.Nz - 1) for c in cfgs]
+    tabs = [r.cpu().numpy() for r in sweep._regions]
+    assert any(tuple(t[0]) != w for t, w in zip(tabs, whole))
+    sweep.step(5)
+    assert all(i.region_violations() == 0 for i in sweep.ibms)
+    seq, _ = bm.sequential(cfgs, 5)
+    _assert_same(list(zip(sweep.lbms, sweep.ibms)), seq)
+    sweep._regions[1][:] = 0                 # a region of one cell, (0, 0, 0)
+    with pytest.raises(lib.ClbmError, match="instance 1"):
+        sweep.step(2)
 
 
 @pytest.mark.gpu
@@ -242,12 +268,12 @@ def test_batch_step_rejects_an_lds_reservation_out_of_range(pkg, lds):
 
 @pytest.mark.parametrize("case,match", [
     ("unbuilt", "not built"), ("sizes", "differ in size"), ("atomic", "fixed-point"),
-    ("state", "unknown bits")])
+    ("state", "unknown bits"), ("regions", "no region tables")])
 def test_fused_batch_step_rejects_bad_arguments(pkg, case, match):
     """Checked on the host before any launch (the tables are never read)."""
     lib = load_sub("_lib")
     lb = lib.ClbmBatch(2, lib.STORE_F32, 256, 4, 16, 16, 16, 0)
-    ib = lib.IbmBatch(2, lib.STORE_F32, lib.SPREAD_FIXED, 8, 4, 32, 16, 16, 16, 16, 16)
+    ib = lib.IbmBatch(2, lib.STORE_F32, lib.SPREAD_FIXED, 8, 4, 32, 16, 16, 16, 16, 16, 1)
     state = 0
     if case == "unbuilt":
         lb = lib.ClbmBatch()
@@ -255,10 +281,12 @@ def test_fused_batch_step_rejects_bad_arguments(pkg, case, match):
         ib.n = 3
     elif case == "atomic":
         ib.spread_mode = lib.SPREAD_ATOMIC_F32
+    elif case == "regions":
+        ib.regions = 0
     else:
         state = 8
     with pytest.raises(lib.ClbmError, match=match):
         lib.call("clbm_batch_step_acc", C.byref(lb), C.byref(ib), state, None)
-    if case in ("atomic", "state"):
+    if case in ("atomic", "state", "regions"):
         with pytest.raises(lib.ClbmError, match=match):
             lib.call("ibm_batch_markers", C.byref(ib), state, None)
