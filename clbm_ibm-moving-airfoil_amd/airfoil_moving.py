@@ -200,7 +200,7 @@ class DroneMovingIBM:
         if solver is not None and self.fast_host:
             return self._compute_force_plan(solver, xc, yc, theta, vx, vy, omega)
         region = self._region(solver, xc, yc, theta) if solver is not None else None
-        main_h = torch.cuda.current_stream(self.device).cuda_stream
+        main_h = _lib.stream_of(self.device)
         if region is not None:
             # run on this object's stream once everything queued so far is done
             if self._side is None:
@@ -230,7 +230,6 @@ class DroneMovingIBM:
         """compute_force with the velocity from ``solver``'s populations as one
         library call (ibm_compute_force_split) over a launch plan built once:
         the step region, the fork of the marker stream and both launches."""
-        import torch
         plan = self._plan
         ip = self.ip
         if (plan is None or self._plan_solver() is not solver or
@@ -240,7 +239,7 @@ class DroneMovingIBM:
             plan = self._build_plan(solver)
         if self._pending:
             self._join()
-        main_h = torch.cuda.current_stream(self.device).cuda_stream
+        main_h = _lib.stream_of(self.device)
         self.gen += 1
         region = self._region_buf
         rc = self._split_fn(C.byref(plan), solver.state_word, self.gen, float(xc), float(yc),
@@ -348,8 +347,7 @@ class DroneMovingIBM:
     def _join(self) -> None:
         """Order the current stream after this object's pending kernels."""
         if self._pending:
-            import torch
-            main_h = torch.cuda.current_stream(self.device).cuda_stream
+            main_h = _lib.stream_of(self.device)
             _lib.check(_lib.fn("clbm_stream_after")(main_h, self._side_h, self._ev[1]),
                        "clbm_stream_after")
             self._pending = False
@@ -485,8 +483,7 @@ class DroneMovingIBM:
         return cd, cl
 
     def _stream(self) -> C.c_void_p:
-        import torch
-        return C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+        return C.c_void_p(_lib.stream_of(self.device))
 
 
 class FishIBM(DroneMovingIBM):

@@ -51,8 +51,7 @@ def _m_raw_inv_f32() -> np.ndarray:
 
 
 def _stream_handle(device) -> C.c_void_p:
-    import torch
-    return C.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+    return C.c_void_p(_lib.stream_of(device))
 
 
 class PopulationField:
@@ -183,8 +182,7 @@ class CLBMSolver:
         split = self._split_source(fx, fy, fz) if self.closure is None else None
         ev = self.kernel_events
         self.kernel_events = None
-        main = torch.cuda.current_stream(self.device)
-        st = C.c_void_p(main.cuda_stream)
+        st = C.c_void_p(_lib.stream_of(self.device))
         state = self._state_ptr
         if split is not None and (ev is None or (ev[0].cuda_event and ev[1].cuda_event)):
             # part 0, the join of the marker stream and part 1 in one call
@@ -205,11 +203,11 @@ class CLBMSolver:
             self._split_ibms.add(ibm)
             part = _lib.fn("clbm_step_part")
             if ev is not None:
-                ev[0].record(main)
+                ev[0].record(torch.cuda.current_stream(self.device))
             _lib.check(part(self._gref, self._pref, state, self.state_word, None, None, None,
                             None, region, 0, st), "clbm_step_part")
             if ev is not None:
-                ev[1].record(main)
+                ev[1].record(torch.cuda.current_stream(self.device))
             ibm._join()
             self._readers.clear()
             fxp, fyp, boxp = ibm._force_ptrs
@@ -230,7 +228,7 @@ class CLBMSolver:
                 fzp = 0     # the immersed-boundary fz is identically zero (airfoil_moving.py:225)
             fxp, fyp = as_device_ptr(fx, shape), as_device_ptr(fy, shape)
             if ev is not None:
-                ev[0].record(main)
+                ev[0].record(torch.cuda.current_stream(self.device))
             if self.closure is None:
                 _lib.call("clbm_step", C.byref(self.grid), C.byref(self.params), state,
                           self.state_word, C.c_void_p(fxp), C.c_void_p(fyp), C.c_void_p(fzp),
@@ -242,7 +240,7 @@ class CLBMSolver:
                           C.c_void_p(box), C.c_void_p(net.packed.data_ptr()),
                           C.c_void_p(net.params.data_ptr()), net.lo, net.hi, st)
             if ev is not None:
-                ev[1].record(main)
+                ev[1].record(torch.cuda.current_stream(self.device))
             self.last_kernel_cells = self.ncell
         self.state_word = _lib.fn("clbm_state_after_step")(self.state_word, self._pref)
         self.steps += 1
