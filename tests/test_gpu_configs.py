@@ -19,6 +19,12 @@ one here or in test_gpu_fullsize.py).
   configured grids, graph-replayed; one instance of each grid size equals its
   sequential run bit for bit (populations, marker forces; force fields within
   1e-8), and every instance stays finite.
+* The learned-closure configuration's coarse run as configured (600 x 200 x
+  50, chord 80, Re 5,000, tau 0.5024, the first held-out kinematics): the
+  plain coupled method trips the divergence guard of main_moving.py:143-145
+  within 3,000 steps, with or without the closure kernel (a network whose
+  corrections are zero), while the same grid and kinematics at Re 1,000 run
+  3,000 steps healthy.
 """
 
 import os
@@ -218,3 +224,31 @@ def test_configured_64_instance_sweep(pkg, gpu):
         # lattices' come from the clamp kernel
         np.testing.assert_allclose(fx, ibm.fx.to_numpy(), rtol=0, atol=1e-8)
     print(f"\n[64-instance sweep] {solo[1]} groups ({solo[0]} solo), grids {sizes}")
+
+
+# ---------------------------------------------------------------------------
+# the learned-closure configuration at Re 5,000
+# ---------------------------------------------------------------------------
+def test_c4_coarse_run_at_re5000_trips_the_divergence_guard(pkg, gpu):
+    """At tau 0.5024 the coarse run of the closure configuration cannot be
+    held: the divergence guard (non-finite populations or rho_min < 0.3,
+    checked every 100 steps) trips for the plain method, and for the fused
+    closure step with zero corrections (s = 1 for the third- and fourth-order
+    moments, their full relaxation: the strongest damping a rate in [0, 2]
+    gives, so no correction in that range can hold the run).  The same grid
+    and kinematics at Re 1,000 (tau 0.512, the regime the closure study uses)
+    stay healthy over the same 3,000 steps."""
+    cl = load_sub("closure")
+    kw = dict(coarse=(600, 200, 50), chord=80.0, steps=3000, check=100, seed=100, refine=(1,),
+              log=lambda s: None)
+    plain = cl.stability_probe(Re_list=(5000.0, 1000.0), **kw)
+    net = cl.ClosureNet(cl.init_params(0, out_scale=0.0), device=gpu)
+    with_net = cl.stability_probe(Re_list=(5000.0,), net=net, **kw)
+    for row in plain + with_net:
+        print(f"\n[C4 probe] Re {row['Re']:.0f} tau {row['tau']:.4f} closure {row['closure']}: "
+              f"diverged at {row['diverged_at']}, rho_min {row['rho_min']:.3f}")
+    re5000, re1000 = plain
+    assert abs(re5000["tau"] - 0.5024) < 1e-9 and re5000["grid"] == [600, 200, 50]
+    assert re5000["diverged_at"] is not None and re5000["diverged_at"] <= 3000
+    assert with_net[0]["closure"] and with_net[0]["diverged_at"] is not None
+    assert re1000["diverged_at"] is None and re1000["rho_min"] > 0.9
