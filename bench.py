@@ -206,7 +206,11 @@ def spread_sort_probe(pkg, cfg, reps: int = 200) -> int:
     clamp kernel); a stable radix sort of the keys alone (torch.sort, rocPRIM);
     and the sort with the rest of a sort-based spread (gather of two int64
     contributions per key, prefix sums, the sum of each cell's run, one store
-    per cell).  Prints one JSON record."""
+    per cell).  Then the library's own sort-by-cell spread
+    (``spread_mode="sorted"``): its whole compute_force, and its spread pass
+    alone against the separate fixed-point spread kernel (one atomic per
+    stencil point, no LDS pre-sum) on the same markers.  Prints one JSON
+    record."""
     import torch
     lbm = pkg.CLBMSolver(cfg, device="cuda:0")
     ibm = pkg.DroneMovingIBM(cfg, device="cuda:0", overlap=False)
@@ -254,10 +258,32 @@ def spread_sort_probe(pkg, cfg, reps: int = 200) -> int:
 
     t_sort = timed(sort_only)
     t_spread = timed(sort_spread)
+    # the library's own sort-by-cell spread (spread_mode="sorted": markers,
+    # interpolation, the grouped spread and the clamp as separate launches),
+    # and the two spreads alone on the same markers and forces
+    lib = importlib.import_module(PKG + "._lib")
+    ibs = pkg.DroneMovingIBM(cfg, device="cuda:0", overlap=False, spread_mode="sorted")
+    t_sorted_cf = timed(lambda: ibs.compute_force(lbm.ux, lbm.uy, *kin))
+    P = C.c_void_p
+    g32 = lib.ClbmGrid(cfg.Nx, cfg.Ny, cfg.Nz, lib.STORE_F32)
+    marks = [P(t._t.data_ptr()) for t in (ibs.X_marker, ibs.Y_marker, ibs.Z_marker, ibs.FX_k,
+                                          ibs.FY_k)]
+    scratch = torch.zeros(2, lbm.ncell, dtype=torch.int64, device="cuda:0")
+    acc = [P(scratch[0].data_ptr()), P(scratch[1].data_ptr())]
+    work = ibs._sort_work
+    t_sorted_spread = timed(lambda: lib.call(
+        "ibm_spread_sorted", C.byref(g32), C.byref(ibs.ip), ibs.N, *marks, *acc,
+        P(work.data_ptr()), work.numel(), P(st.cuda_stream)))
+    ipf = lib.IbmParams(float(ibs.ip.f_max), int(ibs.ip.wrap_weight_quirk), lib.SPREAD_FIXED, 0)
+    t_fixed_spread = timed(lambda: lib.call(
+        "ibm_spread", C.byref(g32), C.byref(ipf), ibs.N, *marks, *acc, P(st.cuda_stream)))
     rec = {"probe": "spreading by sort vs the fused marker kernel", "workload": cfg.name,
            "markers": ibm.N, "keys": int(cells.numel()), "reps": reps,
            "us_compute_force": round(t_ibm, 2), "us_sort_keys": round(t_sort, 2),
-           "us_sort_based_spread": round(t_spread, 2)}
+           "us_sort_based_spread": round(t_spread, 2),
+           "us_compute_force_sorted": round(t_sorted_cf, 2),
+           "us_spread_sorted_alone": round(t_sorted_spread, 2),
+           "us_spread_fixed_alone": round(t_fixed_spread, 2)}
     print(json.dumps(rec))
     return 0
 

@@ -17,13 +17,13 @@ STORE_F32, STORE_F16, STORE_BF16, STORE_F64, STORE_F16C = 0, 1, 2, 3, 4
 STORAGE = {"f32": STORE_F32, "f16": STORE_F16, "bf16": STORE_BF16, "f64": STORE_F64,
            "f16c": STORE_F16C}
 ELEM_BYTES = {STORE_F32: 4, STORE_F16: 2, STORE_BF16: 2, STORE_F64: 8, STORE_F16C: 2}
-SPREAD_FIXED, SPREAD_ATOMIC_F32 = 0, 1
-SPREAD_MODES = {"fixed": SPREAD_FIXED, "atomic": SPREAD_ATOMIC_F32}
+SPREAD_FIXED, SPREAD_ATOMIC_F32, SPREAD_SORTED = 0, 1, 2
+SPREAD_MODES = {"fixed": SPREAD_FIXED, "atomic": SPREAD_ATOMIC_F32, "sorted": SPREAD_SORTED}
 BOX_SUBSLOTS = 64
 BOX_WORDS = 2 * 6 * BOX_SUBSLOTS + 16       # uint64 words of a force-box record
 FORCE_BOX_OFFSET = 2 * 6 * BOX_SUBSLOTS     # word offset of the decoded int32[6] box
 REGION_FLAG_OFFSET = 2 * 6 * BOX_SUBSLOTS + 3   # word offset of the region-violation flag
-ABI_VERSION = 11
+ABI_VERSION = 12
 CLOSURE_GRAD_NOALLOC_MAX_BLOCKS = 1024   # rows of the library's own partial-gradient array
 
 
@@ -112,6 +112,8 @@ _SIGNATURES = {
     "ibm_interp_force": (_I, [_G, _IP, _LP, _P, _I, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                               _P, _P]),
     "ibm_spread": (_I, [_G, _IP, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "ibm_spread_sorted_bytes": (C.c_size_t, [_G, _I]),
+    "ibm_spread_sorted": (_I, [_G, _IP, _I, _P, _P, _P, _P, _P, _P, _P, _P, C.c_size_t, _P]),
     "ibm_clip_forces": (_I, [_G, _IP, _P, _P, _P, _P, _P, C.c_uint32, _P]),
     "ibm_force_sum_f64": (_I, [_I, _P, _P, _P, _P, _P]),
     "clbm_ibm_step": (_I, [_G, _LP, _P, _I, _IP, _I, _P, _P, _P, _P, _F, _F, _F, _F, _F, _F,

@@ -83,8 +83,11 @@ class DroneMovingIBM:
     """Direct-forcing IBM with the Peskin 4-point delta for a rigid body in heave + pitch.
 
     Options beyond the reference: ``spread_mode`` "fixed" (order-independent
-    fixed-point sums, the default) or "atomic" (float atomics like
-    ti.atomic_add); ``wrap_weight_quirk`` False to weight wrapped stencil
+    fixed-point sums, the default), "atomic" (float atomics like
+    ti.atomic_add) or "sorted" (the stencil points grouped by cell and each
+    cell's contributions added in marker order, in float32: deterministic
+    without atomics on the fields; separate marker, interpolation, spread and
+    clamp launches on the caller's stream); ``wrap_weight_quirk`` False to weight wrapped stencil
     points by their unwrapped offset; ``record_stencil`` to keep floor(pos)-1
     per marker (for index checks); ``order`` "natural" (the template's own
     order, neighbours along each section) or "morton" (a Z-order sort of the
@@ -152,6 +155,13 @@ class DroneMovingIBM:
                                  _lib.SPREAD_MODES[spread_mode], 0, int(group), 0)
         self._stencil_base = (torch.zeros((self.N, 3), dtype=torch.int32, device=dev)
                               if record_stencil else None)
+        # work buffer of the sorted spread (zero before its first call; each call
+        # leaves it so)
+        self._sort_work = None
+        if spread_mode == "sorted":
+            nb = _lib.fn("ibm_spread_sorted_bytes")(
+                C.byref(_lib.ClbmGrid(self.Nx, self.Ny, self.Nz, _lib.STORE_F32)), self.N)
+            self._sort_work = torch.zeros(max(int(nb), 1), dtype=torch.uint8, device=dev)
         # region prediction: hull of the template in the body frame, z range of its stencils
         if overlap == "auto":
             overlap = self.Nx * self.Ny * self.Nz >= self.OVERLAP_MIN_CELLS
@@ -196,6 +206,8 @@ class DroneMovingIBM:
             self._fields_stale = False
             self.fx._t.zero_()
             self.fy._t.zero_()
+        if self._sort_work is not None:
+            return self._compute_force_sorted(ux, uy, xc, yc, theta, vx, vy, omega)
         pre, post, clip, solver, state = self._call_args(ux, uy)
         if solver is not None and self.fast_host:
             return self._compute_force_plan(solver, xc, yc, theta, vx, vy, omega)
@@ -224,6 +236,38 @@ class DroneMovingIBM:
         if region is not None:
             self._pending = True
             solver._add_reader(self, region)
+        return self.fx, self.fy, self.fz
+
+    def _compute_force_sorted(self, ux, uy, xc, yc, theta, vx, vy, omega):
+        """compute_force with the sorted spread: ibm_update_markers,
+        ibm_interp_force, ibm_spread_sorted and ibm_clip_forces on the
+        caller's stream (K8, K10 + K11, K12, K9 + K13)."""
+        self._join()
+        if self._fields_stale:
+            self._fields_stale = False
+            self.fx._t.zero_()
+            self.fy._t.zero_()
+        grid, lp, f, state, uxp, uyp = self._velocity_source(ux, uy)
+        P = C.c_void_p
+        m = lambda a: P(a._t.data_ptr())
+        st = self._stream()
+        self.gen += 1
+        gref, ipref = C.byref(grid), C.byref(self.ip)
+        boxp = P(self.box.data_ptr())
+        acc0, acc1 = P(self._acc[0].data_ptr()), P(self._acc[1].data_ptr())
+        _lib.call("ibm_update_markers", gref, self.N, m(self.X0), m(self.Y0), m(self.Z0),
+                  float(xc), float(yc), float(theta), float(vx), float(vy), float(omega),
+                  m(self.X_marker), m(self.Y_marker), m(self.Z_marker), m(self.VX_body),
+                  m(self.VY_body), boxp, self.gen, st)
+        _lib.call("ibm_interp_force", gref, ipref, lp, P(f), state, P(uxp), P(uyp), self.N,
+                  m(self.X_marker), m(self.Y_marker), m(self.Z_marker), m(self.VX_body),
+                  m(self.VY_body), m(self.U_k), m(self.V_k), m(self.FX_k), m(self.FY_k),
+                  P(self._stencil_base.data_ptr() if self._stencil_base is not None else 0), st)
+        _lib.call("ibm_spread_sorted", gref, ipref, self.N, m(self.X_marker), m(self.Y_marker),
+                  m(self.Z_marker), m(self.FX_k), m(self.FY_k), acc0, acc1,
+                  P(self._sort_work.data_ptr()), self._sort_work.numel(), st)
+        _lib.call("ibm_clip_forces", gref, ipref, acc0, acc1, m(self.fx), m(self.fy), boxp,
+                  self.gen, st)
         return self.fx, self.fy, self.fz
 
     def _compute_force_plan(self, solver, xc, yc, theta, vx, vy, omega):

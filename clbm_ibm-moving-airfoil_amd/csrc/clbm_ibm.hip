@@ -561,6 +561,151 @@ static void launch_step_acc_s(const clbm_batch_t* lb, const ibm_batch_t* ib, int
   }
 
 // Velocity source: 0 = ux/uy fields, 1/2 = populations at parity 0/1.
+// ---------------------------------------------------------------------------
+// K12 by sorting (IBM_SPREAD_SORTED).  The 64 n (cell, F_k w) pairs of the
+// markers' stencils are grouped by cell (a counting sort keyed by the cell
+// index), and one thread per cell adds up its group: a segmented reduction
+// instead of atomics on the accumulators.  The sums are the 64-bit
+// fixed-point ones of IBM_SPREAD_FIXED (each F_k w rounded to 2^-shift, then
+// added exactly), so they do not depend on the order of a group and equal
+// the atomic fixed-point spread bit for bit.  Four passes:
+//  * per pair: its cell, its contribution in fixed point and its rank in the
+//    cell's group (the count the pair's increment finds); the pair of rank 0
+//    appends the cell to the list of groups;
+//  * per group: its offset in the pair list (a prefix sum of the counts
+//    across the lanes of a wavefront, one atomic per wavefront on the
+//    cursor); the cell's count goes back to zero for the next call;
+//  * per pair: its contribution moved to its slot, the group's offset plus
+//    its rank (no atomics);
+//  * per group: the sum of its now contiguous contributions into the cell's
+//    accumulators.
+// The wavefronts append and reserve with one aggregated atomic each.
+// ---------------------------------------------------------------------------
+struct SortWork {
+  int32_t *count, *off;               // per cell: group size, group offset
+  int32_t *key, *rank;                // per pair: its cell, its rank in the cell's group
+  long long *vx, *vy;                 // per pair: F_k w in fixed point, in pair order
+  long long *gx, *gy;                 // the same, grouped by cell
+  int32_t *ukey, *uoff, *ulen;        // per group: cell, offset, length
+  int32_t* ctr;                       // [0] groups, [1] pairs reserved
+};
+
+static size_t sort_layout(long long ncell, long long pairs, unsigned char* w, SortWork* out) {
+  size_t o = 0;
+  auto take = [&](long long n) {
+    const size_t at = o;
+    o = align256(o + (size_t)n * 4);
+    return w ? (int32_t*)(w + at) : nullptr;
+  };
+  SortWork W;
+  W.count = take(ncell);
+  W.off = take(ncell);
+  W.key = take(pairs);
+  W.rank = take(pairs);
+  W.vx = (long long*)take(2 * pairs);
+  W.vy = (long long*)take(2 * pairs);
+  W.gx = (long long*)take(2 * pairs);
+  W.gy = (long long*)take(2 * pairs);
+  W.ukey = take(pairs);
+  W.uoff = take(pairs);
+  W.ulen = take(pairs);
+  W.ctr = take(2);
+  if (out) *out = W;
+  return o;
+}
+
+// Exclusive prefix sum of v over the lanes of the wavefront, and its total.
+__device__ __forceinline__ int wave_excl_scan(int v, int lane, int& total) {
+  int incl = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int t = __shfl_up(incl, d);
+    if (lane >= d) incl += t;
+  }
+  total = __shfl(incl, 63);
+  return incl - v;
+}
+
+__global__ __launch_bounds__(64) void k_sort_clear(int32_t* ctr) {
+  if (threadIdx.x < 2) ctr[threadIdx.x] = 0;
+}
+
+__global__ __launch_bounds__(IBLK) void k_sort_pairs(Geo g, int quirk, int n,
+                                                     const float* __restrict__ X,
+                                                     const float* __restrict__ Y,
+                                                     const float* __restrict__ Z,
+                                                     const float* __restrict__ FX,
+                                                     const float* __restrict__ FY, float fscale,
+                                                     SortWork W) {
+  const long long i = (long long)blockIdx.x * IBLK + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  bool first = false;
+  int c = 0;
+  if (i < 64ll * n) {  // every lane takes part in the wavefront's append below
+    const int m = (int)(i >> 6), st = (int)(i & 63);
+    const Lane L = lane_stencil(g, quirk, X[m], Y[m], Z[m], st);
+    c = L.cell;
+    W.key[i] = c;
+    // the rounding of the fixed-point spread: (F_k w) * 2^shift to nearest
+    W.vx[i] = __float2ll_rn((FX[m] * L.w) * fscale);
+    W.vy[i] = __float2ll_rn((FY[m] * L.w) * fscale);
+    const int r = atomicAdd(W.count + c, 1);
+    W.rank[i] = r;
+    first = r == 0;
+  }
+  const unsigned long long who = __ballot(first);
+  int base = 0;
+  if (lane == 0 && who) base = atomicAdd(W.ctr, __popcll(who));
+  base = __shfl(base, 0);
+  if (first) W.ukey[base + __popcll(who & ((1ull << lane) - 1ull))] = c;
+}
+
+__global__ __launch_bounds__(IBLK) void k_sort_claim(long long pairs, SortWork W) {
+  const long long sgm = (long long)blockIdx.x * IBLK + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const long long ngroups = W.ctr[0];
+  if (sgm - lane >= ngroups) return;  // the whole wavefront lies past the groups
+  int c = 0, k = 0;
+  if (sgm < ngroups) {
+    c = W.ukey[sgm];
+    k = W.count[c];
+    W.count[c] = 0;  // zero again for the next call
+  }
+  int total;
+  const int excl = wave_excl_scan(k, lane, total);
+  int base = 0;
+  if (lane == 0 && total > 0) base = atomicAdd(W.ctr + 1, total);
+  base = __shfl(base, 0);
+  if (sgm < ngroups) {
+    W.off[c] = base + excl;
+    W.uoff[sgm] = base + excl;
+    W.ulen[sgm] = k;
+  }
+}
+
+__global__ __launch_bounds__(IBLK) void k_sort_place(long long pairs, SortWork W) {
+  const long long i = (long long)blockIdx.x * IBLK + threadIdx.x;
+  if (i >= pairs) return;
+  const int slot = W.off[W.key[i]] + W.rank[i];
+  W.gx[slot] = W.vx[i];
+  W.gy[slot] = W.vy[i];
+}
+
+__global__ __launch_bounds__(IBLK) void k_sort_sum(long long pairs, SortWork W, void* accx,
+                                                   void* accy) {
+  const long long sgm = (long long)blockIdx.x * IBLK + threadIdx.x;
+  if (sgm >= pairs || sgm >= W.ctr[0]) return;
+  const int c = W.ukey[sgm], o = W.uoff[sgm], k = W.ulen[sgm];
+  unsigned long long sx = 0ull, sy = 0ull;
+  for (int a = 0; a < k; ++a) {
+    sx += (unsigned long long)W.gx[o + a];
+    sy += (unsigned long long)W.gy[o + a];
+  }
+  // this thread is the cell's only writer
+  ((unsigned long long*)accx)[c] += sx;
+  ((unsigned long long*)accy)[c] += sy;
+}
+
 static int velocity_source(const clbm_grid_t* g, const clbm_params_t* p, const void* f, int state,
                            const float* ux, const float* uy, Planes& pl, int& umode) {
   pl.on = 0;
@@ -598,6 +743,9 @@ int ibm_compute_force(const clbm_grid_t* g, const ibm_params_t* ip, const clbm_p
   if (!acc_x || !acc_y) return fail(-1, "ibm_compute_force: NULL accumulator");
   if (box && gen == 0) return fail(-1, "ibm_compute_force: generation must be >= 1");
   if (ip->check_region && !box) return fail(-1, "ibm_compute_force: check_region needs the box record");
+  if (ip->spread_mode == IBM_SPREAD_SORTED)
+    return fail(-1, "ibm_compute_force: the sorted spread runs as ibm_update_markers + "
+                    "ibm_interp_force + ibm_spread_sorted");
   Planes pl;
   int umode;
   if (int e = velocity_source(g, p, f, state, ux, uy, pl, umode)) return e;
@@ -680,6 +828,8 @@ int ibm_spread(const clbm_grid_t* g, const ibm_params_t* ip, int n, const float*
   if (n < 0) return fail(-1, "ibm_spread: n < 0");
   if (n == 0) return 0;
   if (!X || !Y || !Z || !FX || !FY || !acc_x || !acc_y) return fail(-1, "ibm_spread: NULL buffer");
+  if (ip->spread_mode == IBM_SPREAD_SORTED)
+    return fail(-1, "ibm_spread: the sorted spread needs a work buffer (ibm_spread_sorted)");
   const Geo geo = make_geo(*g);
   const Planes pl{0, 0, 0.0, 0.0};
   hipStream_t st = (hipStream_t)stream;
@@ -694,6 +844,39 @@ int ibm_spread(const clbm_grid_t* g, const ibm_params_t* ip, int n, const float*
   return check_launch("ibm_spread");
 }
 
+size_t ibm_spread_sorted_bytes(const clbm_grid_t* g, int n) {
+  if (check_grid(g, false) || n < 0) return 0;
+  return sort_layout((long long)g->nx * g->ny * g->nz, 64ll * n, nullptr, nullptr);
+}
+
+int ibm_spread_sorted(const clbm_grid_t* g, const ibm_params_t* ip, int n, const float* X,
+                      const float* Y, const float* Z, const float* FX, const float* FY,
+                      void* acc_x, void* acc_y, void* work, size_t work_bytes, void* stream) {
+  if (int e = check_grid(g, false)) return e;
+  if (int e = check_ibm(ip)) return e;
+  if (ip->spread_mode != IBM_SPREAD_SORTED)
+    return fail(-1, "ibm_spread_sorted: ip->spread_mode must be IBM_SPREAD_SORTED");
+  if (n < 0) return fail(-1, "ibm_spread_sorted: n < 0");
+  if (n == 0) return 0;
+  if (!X || !Y || !Z || !FX || !FY || !acc_x || !acc_y || !work)
+    return fail(-1, "ibm_spread_sorted: NULL buffer");
+  const long long ncell = (long long)g->nx * g->ny * g->nz, pairs = 64ll * n;
+  if (pairs > INT_MAX) return fail(-1, "ibm_spread_sorted: more than 2^31 stencil points");
+  SortWork W;
+  if (work_bytes < sort_layout(ncell, pairs, (unsigned char*)work, &W))
+    return fail(-1, "ibm_spread_sorted: work buffer smaller than ibm_spread_sorted_bytes");
+  const Geo geo = make_geo(*g);
+  hipStream_t st = (hipStream_t)stream;
+  const unsigned nb = (unsigned)((pairs + IBLK - 1) / IBLK);
+  hipLaunchKernelGGL(k_sort_clear, dim3(1), dim3(64), 0, st, W.ctr);
+  hipLaunchKernelGGL(k_sort_pairs, dim3(nb), dim3(IBLK), 0, st, geo, ip->wrap_weight_quirk, n, X,
+                     Y, Z, FX, FY, (float)std::ldexp(1.0, fixed_shift(ip)), W);
+  hipLaunchKernelGGL(k_sort_claim, dim3(nb), dim3(IBLK), 0, st, pairs, W);
+  hipLaunchKernelGGL(k_sort_place, dim3(nb), dim3(IBLK), 0, st, pairs, W);
+  hipLaunchKernelGGL(k_sort_sum, dim3(nb), dim3(IBLK), 0, st, pairs, W, acc_x, acc_y);
+  return check_launch("ibm_spread_sorted");
+}
+
 int ibm_clip_forces(const clbm_grid_t* g, const ibm_params_t* ip, void* acc_x, void* acc_y,
                     float* fx, float* fy, uint64_t* box, uint32_t gen, void* stream) {
   if (int e = check_grid(g, false)) return e;
@@ -703,7 +886,7 @@ int ibm_clip_forces(const clbm_grid_t* g, const ibm_params_t* ip, void* acc_x, v
   const Geo geo = make_geo(*g);
   const ClipArgs a{geo, ip->f_max, (float)std::ldexp(1.0, -fixed_shift(ip)), acc_x, acc_y, fx, fy,
                    box, 0u};
-  if (ip->spread_mode == IBM_SPREAD_FIXED)
+  if (ip->spread_mode == IBM_SPREAD_FIXED || ip->spread_mode == IBM_SPREAD_SORTED)
     hipLaunchKernelGGL(k_clip<IBM_SPREAD_FIXED>, dim3(box_blocks(geo)), dim3(IBLK), 0,
                        (hipStream_t)stream, a, gen);
   else
@@ -723,6 +906,8 @@ int ibm_batch_build(int n, const clbm_grid_t* grids, const clbm_params_t* params
                     size_t work_bytes, ibm_batch_t* out) {
   if (int e = check_ibm(ip)) return e;
   if (ip->check_region) return fail(-1, "ibm_batch_build: batches take no region check");
+  if (ip->spread_mode == IBM_SPREAD_SORTED)
+    return fail(-1, "ibm_batch_build: batches spread with fixed-point or float atomics");
   const int group = ip->group == 0 ? 8 : ip->group;
   IbmBatchLayout L;
   if (int e = ibm_batch_layout(n, bufs, group, L)) return e;

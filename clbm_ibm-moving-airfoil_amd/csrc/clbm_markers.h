@@ -430,7 +430,8 @@ __device__ __forceinline__ void compute_group(const IbmArgs& a, const Kin& kin, 
 // ---------------------------------------------------------------------------
 static inline int check_ibm(const ibm_params_t* ip) {
   if (!ip) return fail(-1, "ibm params are NULL");
-  if (ip->spread_mode != IBM_SPREAD_FIXED && ip->spread_mode != IBM_SPREAD_ATOMIC_F32)
+  if (ip->spread_mode != IBM_SPREAD_FIXED && ip->spread_mode != IBM_SPREAD_ATOMIC_F32 &&
+      ip->spread_mode != IBM_SPREAD_SORTED)
     return fail(-1, "unknown spread mode");
   if (!(ip->f_max > 0.0f) || !std::isfinite(ip->f_max)) return fail(-1, "f_max must be > 0");
   if (ip->fixed_shift < 0 || ip->fixed_shift > 60) return fail(-1, "fixed_shift out of [0, 60]");

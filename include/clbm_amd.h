@@ -51,7 +51,7 @@
 extern "C" {
 #endif
 
-#define CLBM_ABI_VERSION 11
+#define CLBM_ABI_VERSION 12
 
 /* population storage formats */
 #define CLBM_STORE_F32 0  /* float32 populations, float32 arithmetic (the drop-in) */
@@ -64,6 +64,8 @@ extern "C" {
 /* force spreading modes */
 #define IBM_SPREAD_FIXED 0 /* order-independent 64-bit fixed-point sums (deterministic) */
 #define IBM_SPREAD_ATOMIC_F32 1 /* float32 atomics, as ti.atomic_add (order-dependent) */
+#define IBM_SPREAD_SORTED 2 /* the stencil points grouped by cell, one fixed-point sum per cell
+                               (the sums of IBM_SPREAD_FIXED; ibm_spread_sorted only) */
 
 /* inlet rules at x = 0 (clbm_params_t.inlet) */
 #define CLBM_INLET_NONE 0           /* x periodic, no outlet copy */
@@ -325,6 +327,25 @@ int ibm_interp_force(const clbm_grid_t* g, const ibm_params_t* ip, const clbm_pa
 int ibm_spread(const clbm_grid_t* g, const ibm_params_t* ip, int n, const float* X, const float* Y,
                const float* Z, const float* FX, const float* FY, void* acc_x, void* acc_y,
                void* stream);
+
+/* Replaces DroneMovingIBM._spread (airfoil_moving.py:188-214) by sorting
+ * instead of atomics on the fields: the 64 n (cell, F_k w) pairs of the
+ * stencils are grouped by cell (a counting sort on the cell index) and one
+ * thread per cell adds its group into acc_x/acc_y, in the 64-bit fixed-point
+ * format of IBM_SPREAD_FIXED (ibm_clip_forces converts them as for that
+ * mode): the same sums as ibm_spread in that mode, bit for bit.
+ * ip->spread_mode must be IBM_SPREAD_SORTED.  `work` (device memory of
+ * ibm_spread_sorted_bytes(g, n) bytes: 8 per cell and 52 per stencil
+ * point) must be zero before the first call; every call leaves the parts
+ * that need it at zero.  The fused entry points (ibm_compute_force, the
+ * batches, clbm_ibm_step) reject this mode: it runs as ibm_update_markers +
+ * ibm_interp_force + ibm_spread_sorted + ibm_clip_forces.  Calls that overlap
+ * on one work buffer race.  No reference counterpart as a mode (the
+ * reference adds with ti.atomic_add). */
+size_t ibm_spread_sorted_bytes(const clbm_grid_t* g, int n);
+int ibm_spread_sorted(const clbm_grid_t* g, const ibm_params_t* ip, int n, const float* X,
+                      const float* Y, const float* Z, const float* FX, const float* FY,
+                      void* acc_x, void* acc_y, void* work, size_t work_bytes, void* stream);
 
 /* Replaces DroneMovingIBM._zero_forces + _clip_forces (airfoil_moving.py:
  * 132-137,219-225): over the union of generation gen's box and generation

@@ -221,3 +221,124 @@ def test_ibm_rejects_bad_arguments(pkg, gpu):
     ip = lib.IbmParams(0.05, 1, 0, 0)
     with pytest.raises(lib.ClbmError, match="generation"):
         lib.call("ibm_clip_forces", C.byref(g), C.byref(ip), *([C.c_void_p(1)] * 5), 0, None)
+
+
+@pytest.mark.parametrize("quirk", [True, False])
+@pytest.mark.parametrize("kin", KIN[:2])
+def test_sorted_spread_matches_oracle_and_repeats_bit_for_bit(pkg, oracle, gpu, quirk, kin):
+    """The sort-by-cell spread (each cell's stencil contributions grouped and
+    added in marker order) against the FP64 oracle, to the tolerances of the
+    fixed-point path; two calls give the same fields bit for bit, and the
+    fixed-point path agrees within a few float32 roundings of the sums."""
+    cfg = _cfg(pkg)
+    ux, uy = _fields((cfg.Nx, cfg.Ny, cfg.Nz), 7)
+    ibm = pkg.DroneMovingIBM(cfg, spread_mode="sorted", wrap_weight_quirk=quirk,
+                             record_stencil=True)
+    fx, fy, fz = ibm.compute_force(_dev(ux), _dev(uy), *kin)
+    a = (fx.to_numpy(), fy.to_numpy())
+    ref = oracle.compute_force(ux.astype(np.float64), uy.astype(np.float64), cfg.X0, cfg.Y0, cfg.Z0,
+                               *kin, wrap_weight_quirk=quirk, weight_dtype=np.float64)
+    np.testing.assert_allclose(ibm.FX_k.to_numpy(), ref["FX"], rtol=0, atol=1e-6)
+    np.testing.assert_allclose(ibm.FY_k.to_numpy(), ref["FY"], rtol=0, atol=1e-6)
+    np.testing.assert_allclose(a[0], ref["fx"], rtol=0, atol=1e-7)
+    np.testing.assert_allclose(a[1], ref["fy"], rtol=0, atol=1e-7)
+    assert not fz.to_numpy().any()
+    X, Y, Z, *_ = oracle.update_markers(cfg.X0, cfg.Y0, cfg.Z0, *kin)
+    base, *_ = oracle.stencil(X, Y, Z, cfg.Nx, cfg.Ny, cfg.Nz)
+    np.testing.assert_array_equal(ibm.stencil_base.cpu().numpy(), base)
+    fx2, fy2, _ = ibm.compute_force(_dev(ux), _dev(uy), *kin)
+    np.testing.assert_array_equal(fx2.to_numpy(), a[0])
+    np.testing.assert_array_equal(fy2.to_numpy(), a[1])
+    fixed = pkg.DroneMovingIBM(cfg, wrap_weight_quirk=quirk)
+    ffx, ffy, _ = fixed.compute_force(_dev(ux), _dev(uy), *kin)
+    np.testing.assert_allclose(a[0], ffx.to_numpy(), rtol=0, atol=5e-8)
+    np.testing.assert_allclose(a[1], ffy.to_numpy(), rtol=0, atol=5e-8)
+
+
+def test_sorted_spread_equals_the_fixed_point_spread(pkg, gpu):
+    """ibm_spread_sorted (stencil points grouped by cell, one fixed-point sum
+    per cell) fills the accumulators with the same 64-bit sums as ibm_spread
+    in the fixed-point mode (one atomic per stencil point), bit for bit, on
+    the markers and forces of a coupled step of the 600 x 200 x 50 case and
+    on a dense body (the reference's airplane builder with its filled tip
+    caps, ellipsoid fuselage and tails at 0.7 lu spacing, on a 128 x 64 x 48
+    lattice: 5,521 markers), twice in a row on one work buffer."""
+    import torch
+    lib = load_sub("_lib")
+    P = C.c_void_p
+    dense = pkg.config.airplane_case("dense", 128, 64, 48, 20.0, 15.0, 40.0, (20.0, 3.0, 3.0),
+                                     25.0, 8.0, 8.0, 6.0, 6.0, 2.0, 200.0, 10, "")
+    for cfg in (pkg.config.case_c1(), dense):
+        lbm = pkg.CLBMSolver(cfg)
+        ibm = pkg.DroneMovingIBM(cfg, spread_mode="sorted")
+        for n in range(1, 3):
+            fx, fy, fz = ibm.compute_force(lbm.ux, lbm.uy, *cfg.motion_state(n))
+            lbm.step(fx, fy, fz)
+        g = lib.ClbmGrid(cfg.Nx, cfg.Ny, cfg.Nz, lib.STORE_F32)
+        marks = [P(t._t.data_ptr()) for t in (ibm.X_marker, ibm.Y_marker, ibm.Z_marker,
+                                              ibm.FX_k, ibm.FY_k)]
+        ipf = lib.IbmParams(float(ibm.ip.f_max), int(ibm.ip.wrap_weight_quirk), lib.SPREAD_FIXED, 0)
+        for _ in range(2):
+            a = torch.zeros(2, lbm.ncell, dtype=torch.int64, device=gpu)
+            b = torch.zeros_like(a)
+            lib.call("ibm_spread", C.byref(g), C.byref(ipf), ibm.N, *marks, P(a[0].data_ptr()),
+                     P(a[1].data_ptr()), None)
+            w = ibm._sort_work
+            lib.call("ibm_spread_sorted", C.byref(g), C.byref(ibm.ip), ibm.N, *marks,
+                     P(b[0].data_ptr()), P(b[1].data_ptr()), P(w.data_ptr()), w.numel(), None)
+            assert torch.equal(a, b)
+            assert int((a != 0).sum()) > 0
+        # the work buffer's per-cell counters are back at zero
+        ncell = lbm.ncell
+        cnt = w[:4 * ncell].view(torch.int32)
+        assert int(cnt.abs().sum()) == 0
+
+
+def test_sorted_spread_in_a_coupled_run(pkg, gpu):
+    """A coupled run with the sorted spread (the step reads its clamped
+    fields) follows the fixed-point run to float32 rounding, repeats bit for
+    bit, and a moving body leaves no stale force behind (the clamp clears the
+    previous generation's box in this mode too)."""
+    cfg = pkg.config.naca_case(96, 48, 8, 12.0, name="sorted")
+    runs = []
+    for mode in ("sorted", "sorted", "fixed"):
+        lbm = pkg.CLBMSolver(cfg)
+        ibm = pkg.DroneMovingIBM(cfg, spread_mode=mode)
+        for n in range(1, 9):
+            fx, fy, fz = ibm.compute_force(lbm.ux, lbm.uy, *cfg.motion_state(n))
+            lbm.step(fx, fy, fz)
+            ibm.record_force()
+        runs.append((lbm.f.to_numpy(), ibm.force_history(), ibm.fx.to_numpy(), ibm))
+    (fa, ha, xa, ia), (fb, hb, xb, _), (fc, hc, xc, ic) = runs
+    np.testing.assert_array_equal(fa, fb)
+    np.testing.assert_array_equal(ha, hb)
+    np.testing.assert_array_equal(xa, xb)
+    np.testing.assert_allclose(fa, fc, rtol=0, atol=1e-6)
+    np.testing.assert_allclose(ha, hc, rtol=0, atol=1e-5 * np.abs(hc).max())
+    np.testing.assert_allclose(xa, xc, rtol=0, atol=1e-7)
+    box = ia.force_box()
+    inside = np.zeros(xa.shape, bool)
+    inside[box[0]:box[3] + 1, box[1]:box[4] + 1, box[2]:box[5] + 1] = True
+    assert not xa[~inside].any() and np.abs(xa[inside]).max() > 0
+
+
+def test_sorted_spread_is_rejected_by_the_fused_entry_points(pkg, gpu):
+    lib = load_sub("_lib")
+    g = lib.ClbmGrid(8, 8, 8, 0)
+    ip = lib.IbmParams(0.05, 1, lib.SPREAD_SORTED, 0)
+    dummy = [C.c_void_p(1)] * 7
+    with pytest.raises(lib.ClbmError, match="work buffer"):
+        lib.call("ibm_spread", C.byref(g), C.byref(ip), 1, *dummy, None)
+    with pytest.raises(lib.ClbmError, match="ibm_spread_sorted"):
+        lib.call("ibm_compute_force", C.byref(g), C.byref(ip), None, C.c_void_p(1), 0,
+                 C.c_void_p(1), C.c_void_p(1), 1, None, *([C.c_void_p(1)] * 3), 0.0, 0.0, 0.0,
+                 0.0, 0.0, 0.0, *([C.c_void_p(1)] * 9), None, C.c_void_p(1), C.c_void_p(1),
+                 C.c_void_p(1), 1, None)
+    small = lib.fn("ibm_spread_sorted_bytes")(C.byref(g), 1) - 1
+    with pytest.raises(lib.ClbmError, match="smaller"):
+        lib.call("ibm_spread_sorted", C.byref(g), C.byref(ip), 1, *dummy, C.c_void_p(1), small,
+                 None)
+    ip = lib.IbmParams(0.05, 1, lib.SPREAD_FIXED, 0)
+    with pytest.raises(lib.ClbmError, match="IBM_SPREAD_SORTED"):
+        lib.call("ibm_spread_sorted", C.byref(g), C.byref(ip), 1, *dummy, C.c_void_p(1),
+                 small + 1, None)
