@@ -191,8 +191,10 @@ def test_step_part_rejects_unnormalised_region(pkg, gpu):
                  C.c_void_p(a.state.data_ptr()), a.state_word, None, None, None, None, bad, 0, None)
 
 
-def _coupled(pkg, cfg, steps, overlap, spread="fixed", inlet=True, fast_host=True):
-    lbm = pkg.CLBMSolver(cfg, inlet=inlet)
+def _coupled(pkg, cfg, steps, overlap, spread="fixed", inlet=True, fast_host=True,
+             storage="f32"):
+    lbm = pkg.CLBMSolver(cfg, inlet=inlet, storage=storage,
+                         deviation="inflow" if storage in ("f16", "bf16", "f16c") else "weights")
     ibm = pkg.DroneMovingIBM(cfg, overlap=overlap, spread_mode=spread)
     ibm.fast_host = fast_host
     split = 0
@@ -205,8 +207,9 @@ def _coupled(pkg, cfg, steps, overlap, spread="fixed", inlet=True, fast_host=Tru
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("storage", ["f32", "f16c", "f64"])
 @pytest.mark.parametrize("where", ["centre", "outlet", "y-seam"])
-def test_overlapped_run_equals_single_stream_run(pkg, gpu, where):
+def test_overlapped_run_equals_single_stream_run(pkg, gpu, where, storage):
     nx, ny, nz = 64, 32, 4
     kw = dict(le_x=20.0)
     if where == "outlet":
@@ -214,8 +217,8 @@ def test_overlapped_run_equals_single_stream_run(pkg, gpu, where):
     cfg = pkg.config.naca_case(nx, ny, nz, 12.0, name=where, **kw)
     if where == "y-seam":
         cfg = cfg.with_(y0=1.0)        # the body straddles the periodic y boundary
-    a_lbm, a_ibm, split = _coupled(pkg, cfg, 12, True)
-    b_lbm, b_ibm, nosplit = _coupled(pkg, cfg, 12, False)
+    a_lbm, a_ibm, split = _coupled(pkg, cfg, 12, True, storage=storage)
+    b_lbm, b_ibm, nosplit = _coupled(pkg, cfg, 12, False, storage=storage)
     assert nosplit == 0 and split == 12
     for name in ("rho", "ux", "uy", "uz"):
         np.testing.assert_array_equal(getattr(a_lbm, name).to_numpy(),
