@@ -10,6 +10,8 @@ one here or in test_gpu_fullsize.py).
   over a run; the tolerances below are stated in the test and the FP64 error
   of the float32 run (against float64 storage and arithmetic of the same
   kernels) is printed beside them.
+* The 600 x 200 x 50 case (the benchmark's workload), 400 coupled steps
+  against the C restatement and float64 storage, to the same tolerances.
 * 16-bit deviation storage at 600 x 200 x 50 (f16c, f16, bf16 with inflow
   deviations), 50 coupled steps against float64 storage (within 4 roundings
   of the largest deviation in the format's mantissa), and f16c at
@@ -100,6 +102,47 @@ def test_c0_two_thousand_coupled_steps_match_the_c_restatement(pkg, gpu, ref_c, 
         assert step_c < STEP_TOL, rows[-1]
         assert rel_64 < MEAN_RTOL, rows[-1]
     print(f"\n[c0 {END_STEPS} steps, inlet {inlet}] " + "\n  ".join(rows))
+
+
+# ---------------------------------------------------------------------------
+# 400 coupled steps of the 600 x 200 x 50 case (the benchmark's workload)
+# ---------------------------------------------------------------------------
+C1_STEPS = 400
+C1_WINDOW = 200
+
+
+def test_c1_four_hundred_coupled_steps_match_the_c_restatement(pkg, gpu, ref_c):
+    """The benchmark's workload (600 x 200 x 50, chord 80, tau 0.56, 8,200
+    markers) for 400 coupled steps (the GPU's split step with the marker
+    kernels beside the lattice update) against the C restatement (float32,
+    the reference's kernel order) and against float64 storage of the same
+    kernels: window means of C_D / C_L over the last 200 steps and every
+    step's coefficients within the tolerances of the 2,000-step test above."""
+    cfg = pkg.config.case_c1()
+    assert (cfg.Nx, cfg.Ny, cfg.Nz) == (600, 200, 50) and abs(cfg.tau - 0.56) < 1e-12
+    sim = ref_c.RefSim(cfg, threads=min(16, len(os.sched_getaffinity(0))))
+    hist_c = sim.run(C1_STEPS, record=True)
+    del sim
+    ibm, h32, rho32, fin32 = _gpu_run(pkg, cfg, C1_STEPS, "f32", True, gpu)
+    _, h64, rho64, fin64 = _gpu_run(pkg, cfg, C1_STEPS, "f64", True, gpu)
+    assert fin32 and fin64 and rho32 > 0.9
+    assert ibm.region_violations() == 0
+    w = slice(C1_STEPS - C1_WINDOW, C1_STEPS)
+    rows = []
+    for i, name in ((0, "C_D"), (1, "C_L")):
+        c = {k: ibm.coefficients(h)[i] for k, h in (("c", hist_c), ("32", h32), ("64", h64))}
+        m = {k: float(np.mean(v[w])) for k, v in c.items()}
+        scale = float(np.abs(c["64"]).max())
+        rel_c = abs(m["32"] - m["c"]) / abs(m["c"])
+        rel_64 = abs(m["32"] - m["64"]) / abs(m["64"])
+        step_c = float(np.abs(c["32"] - c["c"]).max()) / scale
+        rows.append(f"{name}: mean gpu32 {m['32']:.6f} c32 {m['c']:.6f} gpu64 {m['64']:.6f} | "
+                    f"mean rel gpu32-c32 {rel_c:.2e}, gpu32-gpu64 {rel_64:.2e} | "
+                    f"max step |gpu32-c32|/scale {step_c:.2e}")
+        assert rel_c < MEAN_RTOL, rows[-1]
+        assert step_c < STEP_TOL, rows[-1]
+        assert rel_64 < MEAN_RTOL, rows[-1]
+    print(f"\n[c1 {C1_STEPS} steps] " + "\n  ".join(rows))
 
 
 # ---------------------------------------------------------------------------
