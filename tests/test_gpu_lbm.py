@@ -51,10 +51,14 @@ def test_init_feq(pkg, oracle, gpu, storage, atol):
 
 
 @pytest.mark.parametrize("storage,atol", [("f32", 2e-6), ("f64", 1e-12)])
-@pytest.mark.parametrize("shape", [(9, 7, 5), (16, 12, 8), (4, 3, 3)])
+@pytest.mark.parametrize("shape", [(9, 7, 5), (16, 12, 8), (4, 3, 3), (5, 1, 4), (6, 4, 1),
+                                   (3, 1, 1), (67, 5, 3)])
 def test_steps_with_force_match_oracle(pkg, oracle, gpu, storage, atol, shape):
     """Random post-stream state, random dense force, inlet + outlet: 1..6 steps
-    (both AA parities) against clbm_oracle.lbm_step in FP64."""
+    (both AA parities) against clbm_oracle.lbm_step in FP64.  Shapes include
+    periodic axes one cell thick (every y or z neighbour is the cell itself),
+    the narrowest lattice the inlet/outlet planes allow (Nx = 3: inlet,
+    one interior column, outlet) and rows that straddle 256-cell workgroups."""
     cfg = _case(pkg, *shape)
     s = pkg.CLBMSolver(cfg, storage=storage)
     f = _rand_state(oracle, shape, 1)
