@@ -342,3 +342,26 @@ def test_sorted_spread_is_rejected_by_the_fused_entry_points(pkg, gpu):
     with pytest.raises(lib.ClbmError, match="IBM_SPREAD_SORTED"):
         lib.call("ibm_spread_sorted", C.byref(g), C.byref(ip), 1, *dummy, C.c_void_p(1),
                  small + 1, None)
+
+
+@pytest.mark.parametrize("spread", ["fixed", "sorted"])
+def test_body_without_markers_leaves_the_flow_force_free(pkg, gpu, spread):
+    """An immersed boundary with no markers: compute_force returns zero
+    fields, the coupled loop equals the force-free step bit for bit, and the
+    FP64 force sums are zero."""
+    cfg = pkg.config.CaseConfig(name="empty", Nx=24, Ny=16, Nz=4, U_inf=0.05, Re=100, chord=4,
+                                s_v=1.7, ux_in=0.05, uy_in=0.0,
+                                X0=np.zeros(0, np.float32), Y0=np.zeros(0, np.float32),
+                                Z0=np.zeros(0, np.float32), x0=8.0, y0=8.0)
+    a = pkg.CLBMSolver(cfg)
+    b = pkg.CLBMSolver(cfg)
+    ibm = pkg.DroneMovingIBM(cfg, spread_mode=spread)
+    assert ibm.N == 0
+    for n in range(1, 5):
+        fx, fy, fz = ibm.compute_force(a.ux, a.uy, *cfg.motion_state(n))
+        a.step(fx, fy, fz)
+        ibm.record_force()
+        b.step()
+    assert not ibm.fx.to_numpy().any() and not ibm.fy.to_numpy().any()
+    np.testing.assert_array_equal(a.f.to_numpy(), b.f.to_numpy())
+    assert not ibm.force_history().any() and ibm.force_history().shape == (4, 2)
