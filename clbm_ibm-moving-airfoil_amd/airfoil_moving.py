@@ -243,10 +243,6 @@ class DroneMovingIBM:
         ibm_interp_force, ibm_spread_sorted and ibm_clip_forces on the
         caller's stream (K8, K10 + K11, K12, K9 + K13)."""
         self._join()
-        if self._fields_stale:
-            self._fields_stale = False
-            self.fx._t.zero_()
-            self.fy._t.zero_()
         grid, lp, f, state, uxp, uyp = self._velocity_source(ux, uy)
         P = C.c_void_p
         m = lambda a: P(a._t.data_ptr())
