@@ -29,7 +29,6 @@ from __future__ import annotations
 import ctypes as C
 import json
 import math
-import os
 import sys
 import time
 

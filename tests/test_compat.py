@@ -4,7 +4,6 @@ DroneMovingIBM`` (main_moving.py:55-56), the fish pair (fish/main_fish.py:51-52)
 and ``ti.init`` (main_moving.py:28-37) resolve to the HIP-backed objects,
 built from whatever module the driver imported as its config."""
 
-import importlib
 import os
 import sys
 

@@ -10,7 +10,6 @@ plain step (2e-6); a closure step against the FP64 oracle collision with the
 rates of the bf16 network (1e-6).
 """
 
-import ctypes as C
 
 import numpy as np
 import pytest
