@@ -126,3 +126,35 @@ def test_full_size_overlapped_step_equals_one_stream(pkg, gpu):
     assert runs[0][2] == 6 and runs[1][2] == 0
     np.testing.assert_array_equal(runs[0][0], runs[1][0])
     np.testing.assert_array_equal(runs[0][1], runs[1][1])
+
+
+@pytest.mark.parametrize("case", ["reference_airplane", "reference_highre"])
+def test_sorted_spread_equals_the_fixed_point_spread_at_full_size(pkg, gpu, case):
+    """The sort-by-cell spread on the reference's two airplanes (74,201 and
+    166,765 markers with filled tip caps and fuselage: 4.7 M and 10.7 M
+    stencil points, densest where the fills overlap) fills the accumulators with the
+    fixed-point spread's sums, bit for bit, and leaves its counters at zero."""
+    import ctypes as C
+    import torch
+    from conftest import load_sub
+    lib = load_sub("_lib")
+    cfg = getattr(pkg.config, case)()
+    lbm = pkg.CLBMSolver(cfg, device=gpu)
+    ibm = pkg.DroneMovingIBM(cfg, device=gpu, spread_mode="sorted")
+    for n in range(1, 3):
+        fx, fy, fz = ibm.compute_force(lbm.ux, lbm.uy, *cfg.motion_state(n))
+        lbm.step(fx, fy, fz)
+    P = C.c_void_p
+    g = lib.ClbmGrid(cfg.Nx, cfg.Ny, cfg.Nz, lib.STORE_F32)
+    marks = [P(t._t.data_ptr()) for t in (ibm.X_marker, ibm.Y_marker, ibm.Z_marker, ibm.FX_k,
+                                          ibm.FY_k)]
+    ipf = lib.IbmParams(float(ibm.ip.f_max), int(ibm.ip.wrap_weight_quirk), lib.SPREAD_FIXED, 0)
+    a = torch.zeros(2, lbm.ncell, dtype=torch.int64, device=gpu)
+    b = torch.zeros_like(a)
+    lib.call("ibm_spread", C.byref(g), C.byref(ipf), ibm.N, *marks, P(a[0].data_ptr()),
+             P(a[1].data_ptr()), None)
+    w = ibm._sort_work
+    lib.call("ibm_spread_sorted", C.byref(g), C.byref(ibm.ip), ibm.N, *marks, P(b[0].data_ptr()),
+             P(b[1].data_ptr()), P(w.data_ptr()), w.numel(), None)
+    assert torch.equal(a, b) and int((a != 0).sum()) > 0
+    assert int(w[:4 * lbm.ncell].view(torch.int32).abs().sum()) == 0
