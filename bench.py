@@ -336,11 +336,14 @@ def measure(pkg, cfg, storage, steps, warmup, dev, tune, deviation="weights", co
     # The dominant kernel is the fused step kernel over the lattice away from
     # the body (the whole lattice when the step is not split).  CLBMSolver
     # records a pair of HIP events around it on the stream it launches on, on
-    # every EVERY-th timed step, alternating parities.  With EVERY = 0 the timed
-    # steps record none, and the kernel is timed over KERNEL_STEPS further
-    # steps right after them (events on every other step).
+    # every EVERY-th timed step, alternating parities; a run of fewer than
+    # 2 EVERY steps times its two middle steps instead (one of each parity, away
+    # from the start-up of the timed region).  With EVERY = 0 the timed steps
+    # record none, and the kernel is timed over KERNEL_STEPS further steps right
+    # after them (events on every other step).
     EVERY = EVENTS_EVERY
-    n_ev = steps // max(EVERY, 1) + 1 if EVERY > 0 else KERNEL_STEPS // 2 + 1
+    middle = (steps // 2 - 1, steps // 2) if 0 < EVERY and 2 <= steps < 2 * EVERY else None
+    n_ev = steps // max(EVERY, 1) + 2 if EVERY > 0 else KERNEL_STEPS // 2 + 1
     pool = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
             for _ in range(n_ev)]
     for e0, e1 in pool:        # created now (a first record), so the step records them itself
@@ -353,7 +356,10 @@ def measure(pkg, cfg, storage, steps, warmup, dev, tune, deviation="weights", co
         for i in range(n):
             xc, yc, th, vx, vy, om = cfg.motion_state(step)
             fx, fy, fz = ibm.compute_force(lbm.ux, lbm.uy, xc, yc, th, vx, vy, om)
-            timed = every > 0 and i % every == (i // every) % 2
+            if middle is not None and every == EVERY:
+                timed = i in middle
+            else:
+                timed = every > 0 and i % every == (i // every) % 2
             if timed:
                 ev.append(pool[len(ev)])
                 lbm.kernel_events = ev[-1]
