@@ -19,7 +19,9 @@
 // Spreading is order-independent by default: each product F_k w becomes a
 // 64-bit fixed-point integer and all sums (LDS and global) are integer sums,
 // so the grid force is the same bit pattern on every run.  The reference's
-// float atomics (airfoil_moving.py:213-214) are kept as IBM_SPREAD_ATOMIC_F32.
+// float atomics (airfoil_moving.py:213-214) are kept as IBM_SPREAD_ATOMIC_F32,
+// and IBM_SPREAD_SORTED spreads by grouping the stencil points by cell and
+// summing each group once (ibm_spread_sorted, the same fixed-point sums).
 #include "clbm_markers.h"
 #include "clbm_step.h"
 
