@@ -23,7 +23,7 @@ BOX_SUBSLOTS = 64
 BOX_WORDS = 2 * 6 * BOX_SUBSLOTS + 16       # uint64 words of a force-box record
 FORCE_BOX_OFFSET = 2 * 6 * BOX_SUBSLOTS     # word offset of the decoded int32[6] box
 REGION_FLAG_OFFSET = 2 * 6 * BOX_SUBSLOTS + 3   # word offset of the region-violation flag
-ABI_VERSION = 12
+ABI_VERSION = 13
 CLOSURE_GRAD_NOALLOC_MAX_BLOCKS = 1024   # rows of the library's own partial-gradient array
 
 
@@ -96,6 +96,8 @@ _SIGNATURES = {
     "clbm_event_create": (_I, [C.POINTER(C.c_void_p)]),
     "clbm_event_destroy": (_I, [_P]),
     "clbm_stream_after": (_I, [_P, _P, _P]),
+    "clbm_streams_concurrent": (_I, [_P, _P, C.POINTER(C.c_int)]),
+    "clbm_side_stream": (_I, [_P, C.POINTER(C.c_void_p), C.POINTER(C.c_int)]),
     "clbm_macro": (_I, [_G, _LP, _P, _I, _P, _P, _P, _P, _P]),
     "clbm_export_state": (_I, [_G, _LP, _P, _I, _P, _P]),
     "clbm_export_state_f64": (_I, [_G, _LP, _P, _I, _P, _P]),
@@ -194,6 +196,26 @@ def stream_of(device) -> int:
         return torch.cuda.current_stream(device).cuda_stream
     idx = device.index
     return raw(torch.cuda.current_device() if idx is None else idx)
+
+
+_side = {}
+
+
+def side_stream(device):
+    """(handle, concurrent): the library's marker stream on ``device``
+    (clbm_side_stream; one per device and process, shared by every body),
+    as a ctypes pointer, and whether it passed the check that its kernels
+    run beside those of torch's current stream rather than behind them on
+    one hardware queue."""
+    import torch
+    idx = torch.cuda.current_device() if device.index is None else device.index
+    got = _side.get(idx)
+    if got is None:
+        h, ok = C.c_void_p(), C.c_int()
+        with torch.cuda.device(idx):
+            call("clbm_side_stream", C.c_void_p(stream_of(device)), C.byref(h), C.byref(ok))
+        got = _side[idx] = (h, bool(ok.value))
+    return got
 
 
 def fn(name: str):

@@ -174,10 +174,12 @@ class DroneMovingIBM:
         self._zr = stencil_z_range(cfg.Z0, self.Nz) if self.N else (0, self.Nz - 1)
         self._hull_args = (C.c_void_p(self._hx.ctypes.data), C.c_void_p(self._hy.ctypes.data),
                            len(self._hx), int(self._zr[0]), int(self._zr[1]))
-        # the concurrent stream and the two events that order it against the
-        # caller's stream (fork before the marker kernels, join after them)
-        self._side = None
+        # the marker stream (the library's, shared by all bodies on the device),
+        # whether it runs beside the caller's stream, and the two events that
+        # order it against that stream (fork before the marker kernels, join
+        # after them)
         self._side_h = None
+        self.side_concurrent = None
         self._ev = []
         # ctypes arguments of the marker and clamp calls that stay fixed for a
         # velocity source: (weak reference to the solver, before-state,
@@ -199,7 +201,6 @@ class DroneMovingIBM:
         (airfoil_moving.py:230-255).  ``ux``/``uy`` are the solver's velocity
         fields (read straight from its populations) or any [Nz,Ny,Nx] float32
         device Field/tensor.  Returns (fx, fy, fz), this object's own fields."""
-        import torch
         if self._fields_stale:
             # the last coupled step left fx, fy unwritten: clear them so that the
             # clamp of this call (which clears the last two boxes) leaves them exact
@@ -214,11 +215,9 @@ class DroneMovingIBM:
         region = self._region(solver, xc, yc, theta) if solver is not None else None
         main_h = _lib.stream_of(self.device)
         if region is not None:
-            # run on this object's stream once everything queued so far is done
-            if self._side is None:
-                self._side = torch.cuda.Stream(device=self.device)
-                self._side_h = C.c_void_p(self._side.cuda_stream)
-                self._ev = [self._new_event(), self._new_event()]
+            # run on the marker stream once everything queued so far is done
+            if self._side_h is None:
+                self._open_side()
             _lib.check(_lib.fn("clbm_stream_after")(self._side_h, main_h, self._ev[0]),
                        "clbm_stream_after")
             st = self._side_h
@@ -294,13 +293,10 @@ class DroneMovingIBM:
     def _build_plan(self, solver):
         """The ibm_split_plan_t of this body on ``solver``'s lattice (and the
         marker stream with its two events when overlapping)."""
-        import torch
         if (solver.Nx, solver.Ny, solver.Nz) != (self.Nx, self.Ny, self.Nz):
             raise ValueError("solver and IBM lattices differ")
-        if self.overlap and self.N and self._side is None:
-            self._side = torch.cuda.Stream(device=self.device)
-            self._side_h = C.c_void_p(self._side.cuda_stream)
-            self._ev = [self._new_event(), self._new_event()]
+        if self.overlap and self.N and self._side_h is None:
+            self._open_side()
         pl = _lib.IbmSplitPlan()
         pl.grid = solver.grid
         pl.params = solver.params
@@ -359,6 +355,10 @@ class DroneMovingIBM:
         return pre, post, clip, solver, state
 
     # -- the concurrent stream ------------------------------------------------
+    def _open_side(self) -> None:
+        self._side_h, self.side_concurrent = _lib.side_stream(self.device)
+        self._ev = [self._new_event(), self._new_event()]
+
     def _new_event(self) -> C.c_void_p:
         ev = C.c_void_p()
         _lib.call("clbm_event_create", C.byref(ev))

@@ -4,14 +4,16 @@
  * Plain pointers and sizes only.  Every pointer argument named as a device
  * buffer is device memory the caller owns (in the Python layer: torch-ROCm
  * tensors, passed as tensor.data_ptr()); `stream` is a hipStream_t or NULL
- * for the null stream.  The library allocates no device memory; it holds one
- * fixed array per GPU, the 7.1 MiB partial-gradient rows of
- * closure_grad_noalloc, which come with its code object.
+ * for the null stream.  The library allocates no device memory; it holds two
+ * fixed arrays per GPU, which come with its code object: the 7.1 MiB
+ * partial-gradient rows of closure_grad_noalloc and the two-word flag of
+ * clbm_side_stream's queue check.
  *
  * Return codes: 0 = ok, < 0 = argument / configuration error, > 0 = a
  * hipError_t from a launch.  The message of the last failure on the calling
  * thread is returned by clbm_last_error().  All launches are asynchronous on
- * `stream`; no entry point synchronises.
+ * `stream`; no entry point synchronises, except clbm_side_stream on its
+ * first call per device.
  *
  * Memory layout (the library's own, not the reference's):
  *   populations  structure of arrays, f[q][z][y][x], x fastest, 19 planes of
@@ -51,7 +53,7 @@
 extern "C" {
 #endif
 
-#define CLBM_ABI_VERSION 12
+#define CLBM_ABI_VERSION 13
 
 /* population storage formats */
 #define CLBM_STORE_F32 0  /* float32 populations, float32 arithmetic (the drop-in) */
@@ -213,6 +215,22 @@ int clbm_step_split(const clbm_grid_t* g, const clbm_params_t* p, void* f, int s
 int clbm_event_create(void** ev);
 int clbm_event_destroy(void* ev);
 int clbm_stream_after(void* waiter, void* signaler, void* ev);
+
+/* Streams and hardware queues (no reference counterpart; Taichi runs one
+ * queue).  HIP spreads streams over a few hardware queues (GPU_MAX_HW_QUEUES)
+ * and two streams on one queue run their kernels one after the other, never
+ * side by side.  clbm_streams_concurrent tells which: a one-wave kernel on
+ * `main` waits (at most 50 ms) for a flag that a one-wave kernel queued on
+ * `other` afterwards sets; *concurrent = 1 when it saw the flag.  It
+ * synchronises both streams.
+ * clbm_side_stream gives the library's marker stream on the current device: a
+ * non-blocking stream created on the first call per device and kept until the
+ * process ends, shared by every caller.  The first call checks it against
+ * `main` as above and creates another candidate while the check fails (four
+ * at most; that call synchronises `main`).  *concurrent (nullable) is 1 when
+ * the stream passed, 0 when none did (it then still orders correctly). */
+int clbm_streams_concurrent(void* main, void* other, int* concurrent);
+int clbm_side_stream(void* main, void** side, int* concurrent);
 
 /* Replaces CLBMSolver._macro_post (clbm_moving.py:305-321): force-free rho
  * and u of the current state into [z][y][x] float32 fields (each nullable). */

@@ -4,8 +4,10 @@ CPU: the host prediction holds every stencil cell of the oracle's float32
 marker placement over many kinematics (including markers on cell faces and
 near the y and x boundaries); the C-ABI normalisation (host-only) keeps the
 outlet pair x = Nx-2, Nx-1 together.  GPU: the two parts of a split step
-equal one full step bit for bit, and a coupled run with the marker kernels on
-their own stream equals the same run on one stream bit for bit.
+equal one full step bit for bit, a coupled run with the marker kernels on
+their own stream equals the same run on one stream bit for bit, and that
+stream (the library's one per device) runs beside the solver's stream rather
+than behind it on one hardware queue.
 """
 
 import ctypes as C
@@ -272,6 +274,31 @@ def test_overlap_splits_the_benchmark_shape(pkg, gpu):
     lbm, ibm, split = _coupled(pkg, cfg, 6, True)
     assert split == 6
     assert ibm.region_violations() == 0
+
+
+@pytest.mark.gpu
+def test_bodies_share_the_library_marker_stream(pkg, gpu):
+    """Every body overlaps on the library's one marker stream per device,
+    which passed the check that its kernels run beside (not behind) those of
+    the solver's stream; the entry point hands out the same stream again."""
+    lib = load_sub("_lib")
+    cfg = pkg.config.naca_case(96, 48, 4, 12.0, name="small")
+    _, a, _ = _coupled(pkg, cfg, 2, True)
+    _, b, _ = _coupled(pkg, cfg, 2, True, fast_host=False)
+    assert a._side_h.value and a._side_h.value == b._side_h.value
+    assert a.side_concurrent is True and b.side_concurrent is True
+    h, ok = C.c_void_p(), C.c_int(-1)
+    so = lib.load()
+    assert so.clbm_side_stream(None, C.byref(h), C.byref(ok)) == 0
+    assert h.value == a._side_h.value and ok.value == 1
+    assert so.clbm_side_stream(None, None, None) < 0
+    # the check itself: the marker stream runs beside the null stream, a
+    # stream never beside itself (the waiting wave runs out its 50 ms)
+    ok.value = -1
+    assert so.clbm_streams_concurrent(None, h, C.byref(ok)) == 0 and ok.value == 1
+    assert so.clbm_streams_concurrent(h, h, C.byref(ok)) == 0 and ok.value == 0
+    assert so.clbm_streams_concurrent(None, None, C.byref(ok)) == 0 and ok.value == 0
+    assert so.clbm_streams_concurrent(None, h, None) < 0
 
 
 @pytest.mark.gpu
