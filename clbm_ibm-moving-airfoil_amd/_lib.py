@@ -23,7 +23,7 @@ BOX_SUBSLOTS = 64
 BOX_WORDS = 2 * 6 * BOX_SUBSLOTS + 16       # uint64 words of a force-box record
 FORCE_BOX_OFFSET = 2 * 6 * BOX_SUBSLOTS     # word offset of the decoded int32[6] box
 REGION_FLAG_OFFSET = 2 * 6 * BOX_SUBSLOTS + 3   # word offset of the region-violation flag
-ABI_VERSION = 13
+ABI_VERSION = 14
 CLOSURE_GRAD_NOALLOC_MAX_BLOCKS = 1024   # rows of the library's own partial-gradient array
 
 
@@ -88,9 +88,9 @@ _SIGNATURES = {
     "clbm_step": (_I, [_G, _LP, _P, _I, _P, _P, _P, _P, _P]),
     "clbm_step_part": (_I, [_G, _LP, _P, _I, _P, _P, _P, _P, _P, _I, _P]),
     "clbm_step_region": (_I, [_G, _LP, _P, _P]),
-    "clbm_step_split": (_I, [_G, _LP, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "clbm_step_split": (_I, [_G, _LP, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "ibm_compute_force_split": (_I, [C.POINTER(IbmSplitPlan), _I, C.c_uint32, _F, _F, _F, _F, _F,
-                                     _F, _P, _P]),
+                                     _F, _P, _I, _P]),
     "ibm_step_region": (_I, [_G, _LP, _P, _P, _I, _I, _I, C.c_double, C.c_double, C.c_double,
                              _P]),
     "clbm_event_create": (_I, [C.POINTER(C.c_void_p)]),

@@ -181,6 +181,10 @@ class DroneMovingIBM:
         self._side_h = None
         self.side_concurrent = None
         self._ev = []
+        # True once anything may have touched this object's buffers on the
+        # caller's stream since the solver bound the fork event to its last
+        # step (compute_force then records the event again)
+        self._fork_void = True
         # ctypes arguments of the marker and clamp calls that stay fixed for a
         # velocity source: (weak reference to the solver, before-state,
         # after-state, clamp prefix)
@@ -205,6 +209,7 @@ class DroneMovingIBM:
             # the last coupled step left fx, fy unwritten: clear them so that the
             # clamp of this call (which clears the last two boxes) leaves them exact
             self._fields_stale = False
+            self._fork_void = True
             self.fx._t.zero_()
             self.fy._t.zero_()
         if self._sort_work is not None:
@@ -279,10 +284,17 @@ class DroneMovingIBM:
         if self._pending:
             self._join()
         main_h = _lib.stream_of(self.device)
+        # the fork event already completes with the solver's last step when
+        # nothing else went onto this stream since (no new record needed)
+        fork = solver._fork
+        bound = int(fork is not None and fork[0] is self and fork[1] == main_h
+                    and not self._fork_void)
+        solver._fork = None
         self.gen += 1
         region = self._region_buf
         rc = self._split_fn(C.byref(plan), solver.state_word, self.gen, float(xc), float(yc),
-                            float(theta), float(vx), float(vy), float(omega), main_h, region)
+                            float(theta), float(vx), float(vy), float(omega), main_h, bound,
+                            region)
         if rc < 0:
             _lib.check(rc, "ibm_compute_force_split")
         if rc == 1:
@@ -385,7 +397,9 @@ class DroneMovingIBM:
         return out if rc == 1 else None
 
     def _join(self) -> None:
-        """Order the current stream after this object's pending kernels."""
+        """Order the current stream after this object's pending kernels
+        (before the caller's stream touches its buffers)."""
+        self._fork_void = True
         if self._pending:
             main_h = _lib.stream_of(self.device)
             _lib.check(_lib.fn("clbm_stream_after")(main_h, self._side_h, self._ev[1]),

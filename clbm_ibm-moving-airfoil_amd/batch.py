@@ -308,6 +308,7 @@ class BatchSweep:
         for n, l in enumerate(self.lbms):
             if n not in self._solo:
                 l.state_word = self.state
+                l._fork = None
                 l.steps += k
                 l._macro_valid = False
         for n, i in enumerate(self.ibms):

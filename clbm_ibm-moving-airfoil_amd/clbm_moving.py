@@ -156,6 +156,11 @@ class CLBMSolver:
         # immersed-boundary objects reading the populations on their own
         # stream: {id: (ibm, region int32[6], event after their last read)}
         self._readers = {}
+        # (ibm, stream): the immersed-boundary object whose fork event completes
+        # with this solver's last kernel, the inside part of a split step queued
+        # on that stream; its next compute_force then needs no record of the
+        # event (None after anything else that writes the populations)
+        self._fork = None
         # the immersed-boundary objects whose forces a split step has used:
         # health() checks that their stencils stayed in the predicted regions
         self._split_ibms = weakref.WeakSet()
@@ -182,18 +187,24 @@ class CLBMSolver:
         split = self._split_source(fx, fy, fz) if self.closure is None else None
         ev = self.kernel_events
         self.kernel_events = None
-        st = C.c_void_p(_lib.stream_of(self.device))
+        self._fork = None
+        st_h = _lib.stream_of(self.device)
+        st = C.c_void_p(st_h)
         state = self._state_ptr
         if split is not None and (ev is None or (ev[0].cuda_event and ev[1].cuda_event)):
             # part 0, the join of the marker stream and part 1 in one call
             ibm, region = split
             self._split_ibms.add(ibm)
             fxp, fyp, boxp = ibm._force_ptrs
+            # the body's fork event completes with the inside part (the next
+            # marker kernels read the populations it writes)
             _lib.check(_lib.fn("clbm_step_split")(
                 self._gref, self._pref, state, self.state_word, region, fxp, fyp, boxp, st,
                 ibm._side_h, ibm._ev[1], ev[0].cuda_event if ev is not None else None,
-                ev[1].cuda_event if ev is not None else None), "clbm_step_split")
+                ev[1].cuda_event if ev is not None else None, ibm._ev[0]), "clbm_step_split")
             ibm._pending = False
+            self._fork = (ibm, st_h)
+            ibm._fork_void = False
             self._readers.clear()
             r = list(region)
             self.last_kernel_cells = self.ncell - ((r[3] - r[0] + 1) * (r[4] - r[1] + 1)
@@ -311,6 +322,7 @@ class CLBMSolver:
     def _wait_readers(self) -> None:
         """Order the current stream after every pending concurrent reader
         (before anything that writes the populations)."""
+        self._fork = None
         if not self._readers:
             return
         for ibm, _ in self._readers.values():

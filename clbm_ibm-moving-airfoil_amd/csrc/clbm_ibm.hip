@@ -1078,7 +1078,7 @@ int ibm_step_region(const clbm_grid_t* g, const clbm_params_t* p, const float* h
 
 int ibm_compute_force_split(ibm_split_plan_t* plan, int state, uint32_t gen, float xc, float yc,
                             float theta, float vx, float vy, float omega, void* main,
-                            int32_t* region_out) {
+                            int fork_bound, int32_t* region_out) {
   if (!plan || !region_out) return fail(-1, "ibm_compute_force_split: NULL argument");
   ibm_split_plan_t& P = *plan;
   int split = ibm_step_region(&P.grid, &P.params, P.hx, P.hy, P.nh, P.z0, P.z1, (double)xc,
@@ -1087,7 +1087,14 @@ int ibm_compute_force_split(ibm_split_plan_t* plan, int state, uint32_t gen, flo
   if (split && (!P.side || !P.fork_ev)) split = 0;
   void* st = main;
   if (split) {
-    if (int e = clbm_stream_after(P.side, main, P.fork_ev)) return e;
+    if (fork_bound) {
+      // fork_ev completes with the last step's inside part (clbm_step_split's done_ev)
+      const hipError_t r = hipStreamWaitEvent((hipStream_t)P.side, (hipEvent_t)P.fork_ev, 0);
+      if (r != hipSuccess)
+        return fail((int)r, std::string("ibm_compute_force_split: ") + hipGetErrorString(r));
+    } else if (int e = clbm_stream_after(P.side, main, P.fork_ev)) {
+      return e;
+    }
     st = P.side;
     P.ip.check_region = 1;
     for (int i = 0; i < 6; ++i) P.ip.region[i] = region_out[i];

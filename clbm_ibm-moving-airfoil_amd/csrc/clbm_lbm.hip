@@ -294,6 +294,9 @@ template <int S> struct InitRun {
 // the runtime stamps them with that kernel's own start and end, unlike events
 // recorded on the stream, whose start stamp is the end of the command before.
 static thread_local hipEvent_t t_outside_ev[2] = {nullptr, nullptr};
+// An event for the next launch of the inside part to complete with the
+// kernel itself (clbm_step_split's done_ev): no marker follows it on the stream.
+static thread_local hipEvent_t t_inside_done = nullptr;
 
 // part: -1 the whole grid, 0 the cells outside the region (force-free), 1 the
 // cells inside it.
@@ -309,6 +312,12 @@ template <int S> struct StepRun {
       hipExtLaunchKernelGGL((k_step<S, PAR, FORCE, BOX, REG>), dim3(nblocks(n)), dim3(BLK), lds,
                             st, t_outside_ev[0], t_outside_ev[1], 0u, a);
       t_outside_ev[0] = t_outside_ev[1] = nullptr;
+      return;
+    }
+    if (REG == 2 && t_inside_done) {
+      hipExtLaunchKernelGGL((k_step<S, PAR, FORCE, BOX, REG>), dim3(nblocks(n)), dim3(BLK), lds,
+                            st, nullptr, t_inside_done, 0u, a);
+      t_inside_done = nullptr;
       return;
     }
     hipLaunchKernelGGL((k_step<S, PAR, FORCE, BOX, REG>), dim3(nblocks(n)), dim3(BLK), lds, st, a);
@@ -728,7 +737,7 @@ int clbm_step_part(const clbm_grid_t* g, const clbm_params_t* p, void* f, int st
 int clbm_step_split(const clbm_grid_t* g, const clbm_params_t* p, void* f, int state,
                     const int32_t* region, const float* fx, const float* fy,
                     const int32_t* force_box, void* main, void* side, void* join_ev,
-                    void* start_ev, void* end_ev) {
+                    void* start_ev, void* end_ev, void* done_ev) {
   if (!join_ev) return fail(-1, "clbm_step_split: NULL join event");
   // the events bracket the outside part's kernel itself (timing events)
   t_outside_ev[0] = (hipEvent_t)start_ev;
@@ -737,7 +746,10 @@ int clbm_step_split(const clbm_grid_t* g, const clbm_params_t* p, void* f, int s
   t_outside_ev[0] = t_outside_ev[1] = nullptr;
   if (e0) return e0;
   if (int e = clbm_stream_after(main, side, join_ev)) return e;
-  return clbm_step_part(g, p, f, state, fx, fy, nullptr, force_box, region, 1, main);
+  t_inside_done = (hipEvent_t)done_ev;
+  const int e1 = clbm_step_part(g, p, f, state, fx, fy, nullptr, force_box, region, 1, main);
+  t_inside_done = nullptr;
+  return e1;
 }
 
 int clbm_macro(const clbm_grid_t* g, const clbm_params_t* p, const void* f, int state, float* rho,

@@ -52,10 +52,11 @@ class Field:
         a = np.asarray(arr, dtype=np.float32)
         if a.shape != self.shape:
             raise ValueError(f"shape {a.shape} != field shape {self.shape}")
-        self._t.copy_(torch.from_numpy(np.ascontiguousarray(a.transpose(2, 1, 0))))
+        # through ``tensor``: the owner orders its own pending work first
+        self.tensor.copy_(torch.from_numpy(np.ascontiguousarray(a.transpose(2, 1, 0))))
 
     def fill(self, value: float) -> None:
-        self._t.fill_(float(value))
+        self.tensor.fill_(float(value))
 
     def __repr__(self) -> str:
         return f"Field(shape={self.shape})"

@@ -53,7 +53,7 @@
 extern "C" {
 #endif
 
-#define CLBM_ABI_VERSION 13
+#define CLBM_ABI_VERSION 14
 
 /* population storage formats */
 #define CLBM_STORE_F32 0  /* float32 populations, float32 arithmetic (the drop-in) */
@@ -199,12 +199,17 @@ int ibm_step_region(const clbm_grid_t* g, const clbm_params_t* p, const float* h
  * `main`; part 1 on `main` with the force.  start_ev / end_ev (nullable,
  * created with timing) receive the start and end of part 0's kernel itself
  * (the launch's own time stamps, not a record on the stream, whose start
- * would be the end of the command before).  No reference counterpart (the
+ * would be the end of the command before).  done_ev (nullable, any event)
+ * completes with part 1's kernel itself, with no marker queued on `main`
+ * after it: a caller whose next marker kernels read this step's populations
+ * passes the plan's fork_ev here and then calls ibm_compute_force_split with
+ * fork_bound = 1 (a marker between two launches on `main` holds the next
+ * one back by several microseconds).  No reference counterpart (the
  * reference runs one queue). */
 int clbm_step_split(const clbm_grid_t* g, const clbm_params_t* p, void* f, int state,
                     const int32_t* region, const float* fx, const float* fy,
                     const int32_t* force_box, void* main, void* side, void* join_ev,
-                    void* start_ev, void* end_ev);
+                    void* start_ev, void* end_ev, void* done_ev);
 
 /* Host-only stream ordering for running the marker kernels beside the
  * lattice step (no reference counterpart; Taichi runs one queue):
@@ -311,11 +316,16 @@ typedef struct {
  * it pays, `side` waits for `main` (fork_ev) and ibm_compute_force (region
  * checked) + ibm_clip_forces run on `side`, the region goes to region_out
  * and the call returns 1; otherwise both run on `main` and it returns 0.
- * Velocity from the populations in state `state`.  < 0 on errors.  No
+ * Velocity from the populations in state `state`.  fork_bound = 1: fork_ev
+ * already completes with the last work on `main` that the marker kernels
+ * must follow (bound to the last step's part 1 by clbm_step_split's done_ev,
+ * and nothing queued on `main` since that writes the populations or touches
+ * the plan's buffers), so `side` waits for it without a new record on
+ * `main`; 0: fork_ev is recorded on `main` first.  < 0 on errors.  No
  * reference counterpart as one call. */
 int ibm_compute_force_split(ibm_split_plan_t* plan, int state, uint32_t gen, float xc, float yc,
                             float theta, float vx, float vy, float omega, void* main,
-                            int32_t* region_out);
+                            int fork_bound, int32_t* region_out);
 
 /* Replaces DroneMovingIBM._update_markers (airfoil_moving.py:100-127) on its
  * own (same arithmetic as in ibm_compute_force).  box (nullable) collects the

@@ -366,6 +366,78 @@ def test_one_call_host_path_equals_the_calls_one_by_one(pkg, gpu, overlap):
     assert a_ibm.region_violations() == 0 and a_ibm.gen == b_ibm.gen == 9
 
 
+def _loop_seeing_forks(pkg, cfg, steps, overlap, storage, between=None):
+    """A coupled loop as the benchmark runs it (no force record between
+    steps); ``between(n, lbm, ibm)`` runs after step n.  Returns the solver,
+    the body and, per compute_force after the first, the fork_bound argument
+    it passed to ibm_compute_force_split."""
+    lbm = pkg.CLBMSolver(cfg, storage=storage,
+                         deviation="inflow" if storage in ("f16", "bf16", "f16c") else "weights")
+    ibm = pkg.DroneMovingIBM(cfg, overlap=overlap)
+    bound = []
+    for n in range(1, steps + 1):
+        fx, fy, fz = ibm.compute_force(lbm.ux, lbm.uy, *cfg.motion_state(n))
+        if n == 1:
+            orig = ibm._split_fn
+
+            def spy(*a, orig=orig):
+                bound.append(a[10])
+                return orig(*a)
+            ibm._split_fn = spy
+        lbm.step(fx, fy, fz)
+        if between is not None:
+            between(n, lbm, ibm)
+    return lbm, ibm, bound
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("storage", ["f32", "f16c", "f64"])
+def test_bound_fork_run_equals_single_stream_run(pkg, gpu, storage):
+    """With nothing queued between a step and the next compute_force, the
+    body's fork event completes with the step's inside part and the marker
+    stream waits for it with no record on the solver's stream: every call
+    after the first takes that path, and the run equals the one-stream run
+    bit for bit."""
+    cfg = pkg.config.naca_case(64, 32, 4, 12.0, le_x=20.0, name="bound-fork")
+    a_lbm, a_ibm, bound = _loop_seeing_forks(pkg, cfg, 12, True, storage)
+    b_lbm, b_ibm, _ = _loop_seeing_forks(pkg, cfg, 12, False, storage)
+    assert bound == [1] * 11
+    np.testing.assert_array_equal(a_lbm.f.to_numpy(), b_lbm.f.to_numpy())
+    np.testing.assert_array_equal(a_ibm.FX_k.to_numpy(), b_ibm.FX_k.to_numpy())
+    np.testing.assert_array_equal(a_ibm.fx.to_numpy(), b_ibm.fx.to_numpy())
+    assert a_ibm.region_violations() == 0
+
+
+@pytest.mark.gpu
+def test_work_between_steps_records_the_fork_again(pkg, gpu):
+    """Reading or writing the body's fields, writing the populations, or a
+    step without the body between a step and the next compute_force puts
+    work on the solver's stream that the marker kernels must follow: that
+    call records the fork event again (fork_bound 0), and the run equals the
+    one-stream run bit for bit."""
+    cfg = pkg.config.naca_case(64, 32, 4, 12.0, le_x=20.0, name="fork-again")
+
+    def between(n, lbm, ibm):
+        if n == 2:
+            ibm.fx.to_numpy()                      # a read of the body's field
+        elif n == 4:
+            lbm.f.from_numpy(lbm.f.to_numpy())     # a write of the populations
+        elif n == 6:
+            ibm.fy.fill(0.0)                       # a write of the body's field
+        elif n == 8:
+            ibm.FX_k.to_numpy()                    # a read of a marker array
+        elif n == 9:
+            lbm.step()                             # a force-free step
+
+    a_lbm, a_ibm, bound = _loop_seeing_forks(pkg, cfg, 12, True, "f32", between)
+    b_lbm, b_ibm, _ = _loop_seeing_forks(pkg, cfg, 12, False, "f32", between)
+    # bound[i] belongs to compute_force i + 2, right after step i + 1
+    assert bound == [1, 0, 1, 0, 1, 0, 1, 0, 0, 1, 1]
+    np.testing.assert_array_equal(a_lbm.f.to_numpy(), b_lbm.f.to_numpy())
+    np.testing.assert_array_equal(a_ibm.FX_k.to_numpy(), b_ibm.FX_k.to_numpy())
+    assert a_ibm.region_violations() == 0
+
+
 @pytest.mark.gpu
 def test_timed_outside_part_changes_nothing_and_times_the_kernel(pkg, gpu):
     """Timing events handed to a split step (the outside part is then
