@@ -64,6 +64,12 @@ def parse(argv=None):
                     help="keep the marker kernels on the step's stream")
     ap.add_argument("--no-16bit", dest="with_16bit", action="store_false",
                     help="skip the FP16/BF16-storage runs of the same workload")
+    ap.add_argument("--two-launch", dest="fused", action="store_const", const=False,
+                    default=True, help="split steps as two launches around a join of the "
+                    "marker stream (clbm_step_split) instead of one (clbm_step_fused)")
+    ap.add_argument("--fused-joined", dest="fused", action="store_const", const="joined",
+                    help="the one-launch split step after a join of the marker stream each "
+                    "step (for rocprofv3 --pmc, which runs kernels one at a time)")
     ap.add_argument("--profile-only", action="store_true",
                     help="run warmup + timed steps only (for rocprofv3)")
     ap.add_argument("--coupled", default=None, metavar="IL,SPREAD,SPINS",
@@ -313,17 +319,29 @@ def device_warmup(seconds: float) -> None:
 # 1.5 % (f32) to 3.5 % (16-bit) of its rate when done on every 4th step, and
 # 0.4-0.8 % on every 16th.
 EVENTS_EVERY = 64
+
+# the dominant kernel of a step, by how the step ran (CLBMSolver.last_step_kind)
+KERNEL_NAMES = {
+    "fused": "k_step_fused: the lattice outside the body region (fused collide+stream), beside "
+             "the marker kernels, then the region with the force once the marker stream has "
+             "published it, in one launch",
+    "split": "k_step outside the body region (fused collide+stream), concurrent with the "
+             "marker kernels",
+    "full": "k_step (fused collide+force+stream)",
+    "coupled": "k_coupled_step (markers, lattice and region in one launch) and its tail",
+}
 KERNEL_STEPS = 200
 
 
 def measure(pkg, cfg, storage, steps, warmup, dev, tune, deviation="weights", coupled=None,
-            device_warmup_s=0.0):
+            device_warmup_s=0.0, fused=True):
     """Warm-up, then ``steps`` timed coupled steps (compute_force + step) of a
     fresh solver/IBM pair with ``storage`` populations.  ``coupled``: a
     (interleave, inside_spread, wait_spins) triple to step through
     coupled.step_coupled instead (its two kernels timed as the step kernel)."""
     import torch
     lbm = pkg.CLBMSolver(cfg, storage=storage, device=dev, deviation=deviation, **tune["lbm"])
+    lbm.fused_split = fused
     ibm = pkg.DroneMovingIBM(cfg, device=dev, **tune["ibm"])
     device_warmup(device_warmup_s)
     if coupled is not None:
@@ -380,7 +398,7 @@ def measure(pkg, cfg, storage, steps, warmup, dev, tune, deviation="weights", co
     rho_min, finite = lbm.health()
     k_ms = float(np.mean([e0.elapsed_time(e1) for e0, e1 in ev]))
     out = dict(dt=dt, k_ms=k_ms, k_cells=float(np.mean(cells)), rho_min=rho_min, finite=finite,
-               host_ms=t_host / steps * 1e3)
+               host_ms=t_host / steps * 1e3, kind=lbm.last_step_kind)
     del lbm, ibm
     torch.cuda.empty_cache()
     return out
@@ -419,7 +437,7 @@ def measure_coupled(pkg, cfg, lbm, ibm, steps, warmup, knobs):
     rho_min, finite = lbm.health()
     k_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
     out = dict(dt=dt, k_ms=k_ms, k_cells=float(lbm.ncell), rho_min=rho_min, finite=finite,
-               host_ms=t_host / steps * 1e3)
+               host_ms=t_host / steps * 1e3, kind="coupled")
     del lbm, ibm
     torch.cuda.empty_cache()
     return out
@@ -454,7 +472,7 @@ def main(argv=None) -> int:
         tune["ibm"]["group"] = a.group
     knobs = tuple(int(v) for v in a.coupled.split(",")) if a.coupled else None
     r = measure(pkg, cfg, a.storage, a.steps, a.warmup, dev, tune, deviation=a.deviation,
-                coupled=knobs, device_warmup_s=a.device_warmup)
+                coupled=knobs, device_warmup_s=a.device_warmup, fused=a.fused)
     if knobs is not None:
         print(json.dumps({"coupled": list(knobs), "workload": cfg.name, "storage": a.storage,
                           "steps": a.steps,
@@ -475,7 +493,7 @@ def main(argv=None) -> int:
     if a.storage == "f32" and a.with_16bit:
         for st in ("f16c", "f16", "bf16"):
             o = measure(pkg, cfg, st, a.steps, a.warmup, dev, tune, deviation="inflow",
-                        device_warmup_s=a.device_warmup)
+                        device_warmup_s=a.device_warmup, fused=a.fused)
             m = a.steps * ncell / o["dt"] / 1e6
             others[st] = {"value": round(m, 1), "deviation": "inflow",
                           "ms_per_step": round(o["dt"] / a.steps * 1e3, 5),
@@ -517,9 +535,7 @@ def main(argv=None) -> int:
                    "grid": [cfg.Nx, cfg.Ny, cfg.Nz], "markers": cfg.N_MARKERS,
                    "storage": a.storage, "spread": "fixed-point", "parallelism": "single GPU"},
         "roofline": {"bound": "hbm",
-                     "kernel": ("k_step outside the body region (fused collide+stream), "
-                                "concurrent with the marker kernels"
-                                if k_cells < ncell else "k_step (fused collide+force+stream)"),
+                     "kernel": KERNEL_NAMES.get(r.get("kind"), "k_step (fused collide+force+stream)"),
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic_record(a.case, a.storage),

@@ -20,10 +20,12 @@ ELEM_BYTES = {STORE_F32: 4, STORE_F16: 2, STORE_BF16: 2, STORE_F64: 8, STORE_F16
 SPREAD_FIXED, SPREAD_ATOMIC_F32, SPREAD_SORTED = 0, 1, 2
 SPREAD_MODES = {"fixed": SPREAD_FIXED, "atomic": SPREAD_ATOMIC_F32, "sorted": SPREAD_SORTED}
 BOX_SUBSLOTS = 64
-BOX_WORDS = 2 * 6 * BOX_SUBSLOTS + 16       # uint64 words of a force-box record
+BOX_WORDS = 2 * 6 * BOX_SUBSLOTS + 32       # uint64 words of a force-box record
 FORCE_BOX_OFFSET = 2 * 6 * BOX_SUBSLOTS     # word offset of the decoded int32[6] box
 REGION_FLAG_OFFSET = 2 * 6 * BOX_SUBSLOTS + 3   # word offset of the region-violation flag
-ABI_VERSION = 14
+SPLIT_READY_OFFSET = 2 * 6 * BOX_SUBSLOTS + 16  # last generation published for clbm_step_fused
+SPLIT_EXPIRED_OFFSET = 2 * 6 * BOX_SUBSLOTS + 17   # generations whose region wait ran out
+ABI_VERSION = 15
 CLOSURE_GRAD_NOALLOC_MAX_BLOCKS = 1024   # rows of the library's own partial-gradient array
 
 
@@ -91,6 +93,8 @@ _SIGNATURES = {
     "clbm_step_split": (_I, [_G, _LP, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "ibm_compute_force_split": (_I, [C.POINTER(IbmSplitPlan), _I, C.c_uint32, _F, _F, _F, _F, _F,
                                      _F, _P, _I, _P]),
+    "clbm_step_fused": (_I, [_G, _LP, _P, _I, _P, _P, _P, _P, C.c_uint32, C.c_double, _P, _P, _P,
+                             _P]),
     "ibm_step_region": (_I, [_G, _LP, _P, _P, _I, _I, _I, C.c_double, C.c_double, C.c_double,
                              _P]),
     "clbm_event_create": (_I, [C.POINTER(C.c_void_p)]),

@@ -181,6 +181,9 @@ class DroneMovingIBM:
         self._side_h = None
         self.side_concurrent = None
         self._ev = []
+        # the generation whose clamped forces the last split compute_force
+        # published for a one-launch split step (clbm_step_fused)
+        self._published = 0
         # True once anything may have touched this object's buffers on the
         # caller's stream since the solver bound the fork event to its last
         # step (compute_force then records the event again)
@@ -299,6 +302,7 @@ class DroneMovingIBM:
             _lib.check(rc, "ibm_compute_force_split")
         if rc == 1:
             self._pending = True
+            self._published = self.gen
             solver._add_reader(self, region)
         return self.fx, self.fy, self.fz
 
@@ -335,6 +339,7 @@ class DroneMovingIBM:
         self._plan_solver = weakref.ref(solver)
         self._region_buf = (C.c_int32 * 6)()
         self._split_fn = _lib.fn("ibm_compute_force_split")
+        self._box_ptr = C.c_void_p(self.box.data_ptr())
         return pl
 
     def _call_args(self, ux, uy):
@@ -440,19 +445,33 @@ class DroneMovingIBM:
         self._join()
         return int(self.box[_lib.REGION_FLAG_OFFSET].item())
 
+    def wait_expirations(self) -> int:
+        """The last generation whose one-launch split step (clbm_step_fused)
+        ran out of its wait for the clamped forces (0 if none)."""
+        self._join()
+        return int(self.box[_lib.SPLIT_EXPIRED_OFFSET].item())
+
     def check_region(self) -> None:
         """Raise ClbmError when the stencils of some compute_force call left
         the region predicted for the split step (the marker kernel marks the
         generation in the box record; reading it synchronises).  A split step
         whose markers left their region ran them concurrently with the update
         of cells they read, so the results after that generation are not
-        trustworthy."""
-        g = self.region_violations()
+        trustworthy.  Likewise when a one-launch split step stepped the region
+        without waiting for its forces (the wait for them ran out)."""
+        self._join()
+        g, late = (int(v) for v in self.box[[_lib.REGION_FLAG_OFFSET,
+                                             _lib.SPLIT_EXPIRED_OFFSET]].tolist())
         if g:
             raise _lib.ClbmError(
                 f"the stencils of compute_force call {g} (generation {g}) left the region "
                 f"predicted for the split step; the marker kernels raced the lattice update "
                 f"away from the body (use overlap=False for this body)")
+        if late:
+            raise _lib.ClbmError(
+                f"the split step of compute_force call {late} (generation {late}) stepped the "
+                f"body's region before its forces were published (its wait ran out); the "
+                f"results from that step on are not trustworthy")
 
     def force_box(self):
         """The (x0, y0, z0, x1, y1, z1) box of the cells the last compute_force reached."""

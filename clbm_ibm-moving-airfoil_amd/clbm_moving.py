@@ -112,7 +112,21 @@ class CLBMSolver:
     leans on instruction issue), ``deviation`` what 16-bit
     storage subtracts before rounding: "weights" f_q - w_q, or "inflow"
     f_q - feq_q(1, inlet velocity), which is near zero wherever the flow is
-    near the free stream."""
+    near the free stream.
+
+    ``fused_split`` (attribute, default True): a split step runs as one
+    launch whose last workgroups wait on the device for the body's published
+    forces (clbm_step_fused, waiting at most FUSED_WAIT_MS) instead of two
+    launches around a join of the marker stream (clbm_step_split); same
+    results.  Used only when the marker stream passed the check that it runs
+    beside the solver's stream (DroneMovingIBM.side_concurrent): where kernels
+    run one at a time (a profiler collecting counters) no kernel can wait for
+    another, and the two launches are used.  "joined": the one launch after a
+    join of the marker stream, whatever the check said (for measuring that
+    kernel where kernels run one at a time)."""
+
+    # the longest a one-launch split step's region waits for its forces
+    FUSED_WAIT_MS = 2000.0
 
     def __init__(self, config=None, storage: str = "f32", device=None, inlet=True,
                  outside_lds=None, deviation: str = "weights"):
@@ -168,8 +182,13 @@ class CLBMSolver:
         # next step (the whole step, or its outside part); cells that kernel updated
         self.kernel_events = None
         self.last_kernel_cells = 0
+        # how the last step ran: "full" (one launch over the lattice), "split"
+        # (outside part, join, inside part), "fused" (both parts in one launch)
+        # or "coupled" (coupled.step_coupled)
+        self.last_step_kind = None
         # a learned closure (closure.ClosureNet) replaces the plain cascade when set
         self.closure = None
+        self.fused_split = True
         self._init_feq()
 
     # -- reference surface ------------------------------------------------
@@ -196,19 +215,36 @@ class CLBMSolver:
             ibm, region = split
             self._split_ibms.add(ibm)
             fxp, fyp, boxp = ibm._force_ptrs
-            # the body's fork event completes with the inside part (the next
-            # marker kernels read the populations it writes)
-            _lib.check(_lib.fn("clbm_step_split")(
-                self._gref, self._pref, state, self.state_word, region, fxp, fyp, boxp, st,
-                ibm._side_h, ibm._ev[1], ev[0].cuda_event if ev is not None else None,
-                ev[1].cuda_event if ev is not None else None, ibm._ev[0]), "clbm_step_split")
+            e0 = ev[0].cuda_event if ev is not None else None
+            e1 = ev[1].cuda_event if ev is not None else None
+            # the body's fork event completes with the step's last kernel (the
+            # next marker kernels read the populations it writes)
+            joined = self.fused_split == "joined"
+            if ibm._published == ibm.gen and (joined or (self.fused_split and ibm.side_concurrent)):
+                # one launch; its region waits on the device for the published forces
+                # (only where the marker stream was seen running beside this one: a
+                # profiler that serialises kernels fails that check at its creation),
+                # or finds them published at once after a join
+                if joined:
+                    ibm._join()
+                _lib.check(_lib.fn("clbm_step_fused")(
+                    self._gref, self._pref, state, self.state_word, region, fxp, fyp,
+                    ibm._box_ptr, ibm.gen, self.FUSED_WAIT_MS, st, e0, e1, ibm._ev[0]),
+                    "clbm_step_fused")
+                fused = True
+            else:
+                _lib.check(_lib.fn("clbm_step_split")(
+                    self._gref, self._pref, state, self.state_word, region, fxp, fyp, boxp, st,
+                    ibm._side_h, ibm._ev[1], e0, e1, ibm._ev[0]), "clbm_step_split")
+                fused = False
             ibm._pending = False
             self._fork = (ibm, st_h)
             ibm._fork_void = False
             self._readers.clear()
             r = list(region)
-            self.last_kernel_cells = self.ncell - ((r[3] - r[0] + 1) * (r[4] - r[1] + 1)
-                                                   * (r[5] - r[2] + 1))
+            self.last_kernel_cells = self.ncell if fused else self.ncell - (
+                (r[3] - r[0] + 1) * (r[4] - r[1] + 1) * (r[5] - r[2] + 1))
+            self.last_step_kind = "fused" if fused else "split"
         elif split is not None:
             ibm, region = split
             self._split_ibms.add(ibm)
@@ -227,6 +263,7 @@ class CLBMSolver:
             r = list(region)
             self.last_kernel_cells = self.ncell - ((r[3] - r[0] + 1) * (r[4] - r[1] + 1)
                                                    * (r[5] - r[2] + 1))
+            self.last_step_kind = "split"
         else:
             self._wait_readers()
             shape = (self.Nx, self.Ny, self.Nz)
@@ -253,6 +290,7 @@ class CLBMSolver:
             if ev is not None:
                 ev[1].record(torch.cuda.current_stream(self.device))
             self.last_kernel_cells = self.ncell
+            self.last_step_kind = "full"
         self.state_word = _lib.fn("clbm_state_after_step")(self.state_word, self._pref)
         self.steps += 1
         self._macro_valid = False

@@ -72,3 +72,4 @@ def step_coupled(lbm, ibm, xc, yc, theta, vx, vy, omega, interleave: int = 2,
     lbm.steps += 1
     lbm._macro_valid = False
     lbm.last_kernel_cells = lbm.ncell
+    lbm.last_step_kind = "coupled"

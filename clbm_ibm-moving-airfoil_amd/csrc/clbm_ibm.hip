@@ -245,6 +245,14 @@ __global__ __launch_bounds__(IBLK) void k_clip(ClipArgs a, uint32_t gen) {
   clip_body<SMODE>(a, gen, blockIdx.x, gridDim.x);
 }
 
+// Publishes generation gen for a lattice step that waits in its own launch
+// for the clamped forces (clbm_step_fused): queued after the clamp on its
+// stream, it runs once the clamp's kernel has ended and its writes (fx, fy,
+// the decoded box) are visible to the whole device.
+__global__ void k_publish(unsigned long long* ready, unsigned long long gen) {
+  if (threadIdx.x == 0) __hip_atomic_store(ready, gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // bpi blocks per body, body i = blockIdx.x / bpi.  Every block reads the
 // batch's step counter, then counts itself in *done; the last block to
 // finish (all others have read the counter by then) clears *done and
@@ -1108,6 +1116,11 @@ int ibm_compute_force_split(ibm_split_plan_t* plan, int state, uint32_t gen, flo
     return e;
   if (int e = ibm_clip_forces(&P.grid, &P.ip, P.acc_x, P.acc_y, P.fx, P.fy, P.box, gen, st))
     return e;
+  if (split) {
+    hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, (hipStream_t)st,
+                       (unsigned long long*)IBM_SPLIT_READY(P.box), (unsigned long long)gen);
+    if (int e = check_launch("ibm_compute_force_split")) return e;
+  }
   return split;
 }
 

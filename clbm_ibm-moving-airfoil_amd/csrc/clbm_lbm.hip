@@ -103,6 +103,50 @@ __global__ __launch_bounds__(BLK) void k_step(StepArgs a) {
                          FieldForce<FORCE, BOX>{a.fx, a.fy, a.fz, a.box});
 }
 
+// The split step in one launch (clbm_step_fused).  Blocks [0, nb_out) update
+// the cells outside the region force-free, in the sweep order of part 0; the
+// blocks after them update the region's cells with the force, as part 1,
+// once the clamp of generation `gen` has published its forces in *ready
+// (k_publish, after the clamp on the marker stream).  The region's blocks are
+// dispatched after every outside block, by when the marker kernels beside
+// the sweep have long finished, so the wait is normally one poll.  Thread 0
+// of each region block polls (with s_sleep) for at most `ticks` of the
+// constant-rate clock; when that runs out it marks the generation in
+// *expired and the block steps its cells with the forces as they are, which
+// the health check reports as an error.  ready and expired sit in a 128-byte
+// line of their own: the loads that poll them must not bring a stale copy of
+// the decoded box, written by the clamp meanwhile, into this XCD's L2.  No
+// other line the region reads is touched in this launch before the poll
+// succeeds, so plain loads after it read what the clamp wrote.
+struct FusedWait {
+  const unsigned long long* ready;
+  unsigned long long* expired;
+  unsigned long long gen;
+  long long ticks;
+};
+template <int S, int PAR>
+__global__ __launch_bounds__(BLK) void k_step_fused(StepArgs a, int nb_out, FusedWait w) {
+  const int b = blockIdx.x;
+  if (b < nb_out) {
+    step_cell<S, PAR, 1>(a, sweep_block<PAR>(b, nb_out) * BLK + threadIdx.x,
+                         FieldForce<false, false>{nullptr, nullptr, nullptr, nullptr});
+    return;
+  }
+  if (threadIdx.x == 0) {
+    const long long t0 = wall_clock64();
+    unsigned long long v = __hip_atomic_load(w.ready, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    while (v < w.gen && wall_clock64() - t0 < w.ticks) {
+      __builtin_amdgcn_s_sleep(8);
+      v = __hip_atomic_load(w.ready, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (v < w.gen) atomicMax(w.expired, w.gen);
+  }
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  step_cell<S, PAR, 2>(a, (b - nb_out) * BLK + threadIdx.x,
+                       FieldForce<true, true>{a.fx, a.fy, a.fz, a.box});
+}
+
 // The data movement of one AA step without the collision: every cell's 19
 // post-stream slots loaded and stored into the slots the step writes (the
 // populations permuted, their values kept), in the step's sweep order.  The
@@ -353,6 +397,30 @@ template <int S> struct StepRun {
     else
       launch_par<1>(g, p, pl, f, fx, fy, fz, box, rg, part, st);
     return check_launch(part < 0 ? "clbm_step" : "clbm_step_part");
+  }
+};
+
+template <int S> struct FusedRun {
+  static int run(const Geo& g, const clbm_params_t* p, const Planes& pl, void* f, int parity,
+                 const float* fx, const float* fy, const int32_t* box, const Region& rg,
+                 const FusedWait& w, hipEvent_t e0, hipEvent_t e1, hipStream_t st) {
+    const StepArgs a{g, p->s_v, p->s_e, pl, f, fx, fy, nullptr, box, rg};
+    const int nb_out = nblocks(g.ncell);
+    const dim3 grid(nb_out + nblocks(rg.vol));
+    const unsigned lds = (unsigned)p->outside_lds;
+    const bool timed = e0 || e1;
+    if (parity == 0) {
+      if (timed)
+        hipExtLaunchKernelGGL((k_step_fused<S, 0>), grid, dim3(BLK), lds, st, e0, e1, 0u, a, nb_out, w);
+      else
+        hipLaunchKernelGGL((k_step_fused<S, 0>), grid, dim3(BLK), lds, st, a, nb_out, w);
+    } else {
+      if (timed)
+        hipExtLaunchKernelGGL((k_step_fused<S, 1>), grid, dim3(BLK), lds, st, e0, e1, 0u, a, nb_out, w);
+      else
+        hipLaunchKernelGGL((k_step_fused<S, 1>), grid, dim3(BLK), lds, st, a, nb_out, w);
+    }
+    return check_launch("clbm_step_fused");
   }
 };
 
@@ -750,6 +818,48 @@ int clbm_step_split(const clbm_grid_t* g, const clbm_params_t* p, void* f, int s
   const int e1 = clbm_step_part(g, p, f, state, fx, fy, nullptr, force_box, region, 1, main);
   t_inside_done = nullptr;
   return e1;
+}
+
+int clbm_step_fused(const clbm_grid_t* g, const clbm_params_t* p, void* f, int state,
+                    const int32_t* region, const float* fx, const float* fy, uint64_t* box,
+                    uint32_t gen, double wait_ms, void* main, void* start_ev, void* end_ev,
+                    void* done_ev) {
+  if (!p) return fail(-1, "clbm_step_fused: params is NULL");
+  if (int e = check_grid(g, p->inlet != 0)) return e;
+  if (!f || !region || !fx || !fy || !box) return fail(-1, "clbm_step_fused: NULL buffer");
+  if (gen == 0) return fail(-1, "clbm_step_fused: gen must be >= 1");
+  if (!(wait_ms > 0.0) || wait_ms > 60000.0)
+    return fail(-1, "clbm_step_fused: wait_ms must be in (0, 60000]");
+  if (p->outside_lds < 0 || p->outside_lds > 64 * 1024)
+    return fail(-1, "clbm_step_fused: outside_lds must be in [0, 65536]");
+  Planes pl;
+  if (int e = check_state(g, p, state, pl)) return e;
+  if (int e = check_inlet(p, "clbm_step_fused")) return e;
+  int32_t r[6];
+  if (int e = clbm_step_region(g, p, region, r)) return e;
+  for (int a = 0; a < 6; ++a)
+    if (r[a] != region[a])
+      return fail(-1, "clbm_step_fused: region is not normalised (pass it through clbm_step_region)");
+  int dev = 0, khz = 0;
+  hipError_t err = hipGetDevice(&dev);
+  if (err == hipSuccess) err = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev);
+  if (err == hipSuccess && khz <= 0) err = hipErrorInvalidValue;
+  if (err != hipSuccess) return fail((int)err, std::string("clbm_step_fused: ") + hipGetErrorString(err));
+  const FusedWait w{(const unsigned long long*)IBM_SPLIT_READY(box),
+                    (unsigned long long*)IBM_SPLIT_EXPIRED(box), (unsigned long long)gen,
+                    (long long)(wait_ms * (double)khz)};
+  // the launch carries at most one end event: the timing one when asked for,
+  // then done_ev is recorded after it on the stream
+  hipEvent_t e1 = (hipEvent_t)(end_ev ? end_ev : done_ev);
+  if (int e = dispatch_storage<FusedRun>(g->storage, make_geo(*g), p, pl, f, state & CLBM_STATE_PARITY,
+                                         fx, fy, IBM_FORCE_BOX(box), make_region(r), w,
+                                         (hipEvent_t)start_ev, e1, (hipStream_t)main))
+    return e;
+  if (end_ev && done_ev) {
+    err = hipEventRecord((hipEvent_t)done_ev, (hipStream_t)main);
+    if (err != hipSuccess) return fail((int)err, std::string("clbm_step_fused: ") + hipGetErrorString(err));
+  }
+  return 0;
 }
 
 int clbm_macro(const clbm_grid_t* g, const clbm_params_t* p, const void* f, int state, float* rho,

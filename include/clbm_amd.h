@@ -53,7 +53,7 @@
 extern "C" {
 #endif
 
-#define CLBM_ABI_VERSION 14
+#define CLBM_ABI_VERSION 15
 
 /* population storage formats */
 #define CLBM_STORE_F32 0  /* float32 populations, float32 arithmetic (the drop-in) */
@@ -78,7 +78,7 @@ extern "C" {
 #define CLBM_STATE_PLANES 2
 
 #define IBM_BOX_SUBSLOTS 64
-#define IBM_BOX_WORDS (2 * 6 * IBM_BOX_SUBSLOTS + 16)
+#define IBM_BOX_WORDS (2 * 6 * IBM_BOX_SUBSLOTS + 32)
 /* the decoded int32[6] box inside the uint64 box record */
 #define IBM_FORCE_BOX(box) ((int32_t*)((box) + 2 * 6 * IBM_BOX_SUBSLOTS))
 /* the word of the box record where ibm_compute_force marks (as an atomic
@@ -89,6 +89,12 @@ extern "C" {
  * uint32 counts of finished marker workgroups */
 #define IBM_STEP_DECISION(box) ((box) + 2 * 6 * IBM_BOX_SUBSLOTS + 4)
 #define IBM_STEP_DONE(box) ((uint32_t*)((box) + 2 * 6 * IBM_BOX_SUBSLOTS + 8))
+/* words of the box record for clbm_step_fused, in a 128-byte line of their
+ * own (polled while the line of the decoded box is being written): the last
+ * generation whose clamped forces ibm_compute_force_split has published, and
+ * (atomic maximum) the generations whose region ran out of its wait */
+#define IBM_SPLIT_READY(box) ((box) + 2 * 6 * IBM_BOX_SUBSLOTS + 16)
+#define IBM_SPLIT_EXPIRED(box) ((box) + 2 * 6 * IBM_BOX_SUBSLOTS + 17)
 
 typedef struct {
   int32_t nx, ny, nz; /* lattice size, Nx >= 3; one population plane (nx*ny*nz elements of
@@ -311,10 +317,30 @@ typedef struct {
   void *side, *fork_ev;       /* the marker stream and its fork event */
 } ibm_split_plan_t;
 
+/* CLBMSolver.step (clbm_moving.py:326-343) for the split coupled loop as
+ * one launch on `main` with no join of the marker stream: the cells outside
+ * `region` force-free (as clbm_step_part 0), then, in the last workgroups of
+ * the same launch, the region's cells with the force (as part 1) once
+ * *IBM_SPLIT_READY(box) >= gen, which ibm_compute_force_split publishes on
+ * the marker stream after its clamp of generation gen.  Those workgroups
+ * wait at most wait_ms milliseconds; when the wait runs out they mark gen in
+ * *IBM_SPLIT_EXPIRED(box) and step the region with the forces as they are
+ * (the step's result is then not to be trusted: check that word, as
+ * CLBMSolver.health does).  The marker stream must run beside `main`
+ * (clbm_side_stream).  start_ev / end_ev (nullable, created with timing)
+ * bracket the launch itself; done_ev (nullable) completes with it, as in
+ * clbm_step_split.  Results equal clbm_step_split bit for bit.  No
+ * reference counterpart (the reference runs one queue). */
+int clbm_step_fused(const clbm_grid_t* g, const clbm_params_t* p, void* f, int state,
+                    const int32_t* region, const float* fx, const float* fy, uint64_t* box,
+                    uint32_t gen, double wait_ms, void* main, void* start_ev, void* end_ev,
+                    void* done_ev);
+
 /* Replaces DroneMovingIBM.compute_force (airfoil_moving.py:230-255) in the
  * split coupled loop, as one call: the step region (ibm_step_region); when
  * it pays, `side` waits for `main` (fork_ev) and ibm_compute_force (region
- * checked) + ibm_clip_forces run on `side`, the region goes to region_out
+ * checked) + ibm_clip_forces run on `side`, generation gen is published in
+ * IBM_SPLIT_READY(box) for clbm_step_fused, the region goes to region_out
  * and the call returns 1; otherwise both run on `main` and it returns 0.
  * Velocity from the populations in state `state`.  fork_bound = 1: fork_ev
  * already completes with the last work on `main` that the marker kernels

@@ -43,7 +43,8 @@ def test_bench_prints_one_json_line_of_the_contract():
     rl = d["roofline"]
     assert rl["bound"] == "hbm" and rl["unit"] == "GB/s" and rl["peak"] == 8000.0
     assert 0 < rl["frac"] < 1.0 and abs(rl["frac"] - rl["achieved"] / rl["peak"]) < 1e-3
-    assert rl["cells_per_launch"] < 600 * 200 * 50     # the split step's outside part
+    # the split step in one launch: every cell, the region after the markers' forces
+    assert rl["cells_per_launch"] == 600 * 200 * 50 and rl["kernel"].startswith("k_step_fused")
     assert rl["bytes_per_launch"] == pytest.approx(152 * rl["cells_per_launch"], rel=1e-6)
     assert d["health"]["finite"] and d["health"]["rho_min"] > 0.9
     assert set(d["storage_16bit"]) == {"f16c", "f16", "bf16"}

@@ -203,7 +203,7 @@ def _coupled(pkg, cfg, steps, overlap, spread="fixed", inlet=True, fast_host=Tru
     for n in range(1, steps + 1):
         fx, fy, fz = ibm.compute_force(lbm.ux, lbm.uy, *cfg.motion_state(n))
         lbm.step(fx, fy, fz)
-        split += lbm.last_kernel_cells < lbm.ncell
+        split += lbm.last_step_kind in ("split", "fused")
         ibm.record_force()
     return lbm, ibm, split
 
@@ -265,6 +265,103 @@ def test_overlapped_run_equals_single_stream_run_at_size(pkg, gpu):
     np.testing.assert_array_equal(a_lbm.f.to_numpy(), b_lbm.f.to_numpy())
     np.testing.assert_array_equal(a_ibm.force_history(), b_ibm.force_history())
     assert a_ibm.region_violations() == 0
+
+
+def _fused_run(pkg, cfg, steps, fused, storage="f32"):
+    lbm = pkg.CLBMSolver(cfg, storage=storage,
+                         deviation="inflow" if storage in ("f16", "bf16", "f16c") else "weights")
+    lbm.fused_split = fused
+    ibm = pkg.DroneMovingIBM(cfg, overlap=True)
+    one_launch = []
+    for n in range(1, steps + 1):
+        fx, fy, fz = ibm.compute_force(lbm.ux, lbm.uy, *cfg.motion_state(n))
+        lbm.step(fx, fy, fz)
+        one_launch.append(lbm.last_step_kind == "fused")
+    return lbm, ibm, one_launch
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("storage", ["f32", "f16c", "f64"])
+@pytest.mark.parametrize("where", ["centre", "outlet"])
+def test_fused_split_step_equals_the_two_launch_split_step(pkg, gpu, storage, where):
+    """The split step as one launch whose region waits on the device for the
+    published forces (clbm_step_fused) equals the two launches around the
+    join (clbm_step_split) bit for bit, and no wait runs out."""
+    kw = dict(le_x=20.0) if where == "centre" else dict(le_x=64 - 10.0)
+    cfg = pkg.config.naca_case(64, 32, 4, 12.0, name=f"fused-{where}", **kw)
+    a_lbm, a_ibm, a_one = _fused_run(pkg, cfg, 12, True, storage)
+    b_lbm, b_ibm, b_one = _fused_run(pkg, cfg, 12, False, storage)
+    assert all(a_one) and not any(b_one)
+    np.testing.assert_array_equal(a_lbm.f.to_numpy(), b_lbm.f.to_numpy())
+    np.testing.assert_array_equal(a_ibm.FX_k.to_numpy(), b_ibm.FX_k.to_numpy())
+    np.testing.assert_array_equal(a_ibm.fx.to_numpy(), b_ibm.fx.to_numpy())
+    assert a_ibm.wait_expirations() == 0 and a_ibm.region_violations() == 0
+    a_lbm.health()
+
+
+@pytest.mark.gpu
+def test_fused_split_step_equals_the_two_launch_split_step_at_size(pkg, gpu):
+    """At 1.6 M cells, where the sweep is still running while the marker
+    kernels finish beside it."""
+    cfg = pkg.config.naca_case(320, 160, 32, 40.0, le_x=80.0, name="fused-size")
+    a_lbm, a_ibm, a_one = _fused_run(pkg, cfg, 16, True)
+    b_lbm, b_ibm, _ = _fused_run(pkg, cfg, 16, False)
+    assert all(a_one)
+    np.testing.assert_array_equal(a_lbm.f.to_numpy(), b_lbm.f.to_numpy())
+    np.testing.assert_array_equal(a_ibm.FX_k.to_numpy(), b_ibm.FX_k.to_numpy())
+    assert a_ibm.wait_expirations() == 0
+
+
+@pytest.mark.gpu
+def test_fused_split_step_needs_a_concurrent_marker_stream_or_a_join(pkg, gpu):
+    """The one-launch split step is taken only while the marker stream is
+    known to run beside the solver's (otherwise the two launches around the
+    join), or, in the "joined" mode, after a join; all equal the two-launch
+    run bit for bit."""
+    cfg = pkg.config.naca_case(64, 32, 4, 12.0, le_x=20.0, name="fused-gate")
+    lbm = pkg.CLBMSolver(cfg)
+    ibm = pkg.DroneMovingIBM(cfg, overlap=True)
+    kinds = []
+    for n in range(1, 10):
+        fx, fy, fz = ibm.compute_force(lbm.ux, lbm.uy, *cfg.motion_state(n))
+        if n == 4:
+            ibm.side_concurrent = False     # as where kernels run one at a time
+        if n == 7:
+            lbm.fused_split = "joined"
+        lbm.step(fx, fy, fz)
+        kinds.append(lbm.last_step_kind)
+    assert kinds == ["fused"] * 3 + ["split"] * 3 + ["fused"] * 3
+    b_lbm, b_ibm, _ = _fused_run(pkg, cfg, 9, False)
+    np.testing.assert_array_equal(lbm.f.to_numpy(), b_lbm.f.to_numpy())
+    np.testing.assert_array_equal(ibm.FX_k.to_numpy(), b_ibm.FX_k.to_numpy())
+    assert ibm.wait_expirations() == 0
+
+
+@pytest.mark.gpu
+def test_fused_split_step_marks_a_wait_that_runs_out(pkg, gpu):
+    """A one-launch split step asked for a generation that is never published
+    (one ahead of the last, with 0.05 ms to wait) gives up its wait, marks the
+    generation, and the health check raises; every kernel ends."""
+    lib = load_sub("_lib")
+    cfg = pkg.config.naca_case(64, 32, 4, 12.0, le_x=20.0, name="fused-late")
+    lbm, ibm, one = _fused_run(pkg, cfg, 3, True)
+    assert all(one)
+    fx, fy, fz = ibm.compute_force(lbm.ux, lbm.uy, *cfg.motion_state(4))
+    ibm._join()
+    fxp, fyp, _ = ibm._force_ptrs
+    late = ibm.gen + 1
+    rc = lib.load().clbm_step_fused(
+        C.byref(lbm.grid), C.byref(lbm.params), C.c_void_p(lbm.state.data_ptr()),
+        lbm.state_word, ibm._region_buf, fxp, fyp, ibm._box_ptr, late, 0.05,
+        C.c_void_p(lib.stream_of(lbm.device)), None, None, None)
+    assert rc == 0, lib.load().clbm_last_error()
+    assert ibm.wait_expirations() == late
+    with pytest.raises(lib.ClbmError, match="wait ran out"):
+        ibm.check_region()
+    assert lib.load().clbm_step_fused(
+        C.byref(lbm.grid), C.byref(lbm.params), C.c_void_p(lbm.state.data_ptr()),
+        lbm.state_word, ibm._region_buf, fxp, fyp, ibm._box_ptr, late, 0.0, None, None, None,
+        None) < 0
 
 
 @pytest.mark.gpu
@@ -334,7 +431,7 @@ def test_region_violation_stops_the_run_at_the_health_check(pkg, gpu, monkeypatc
     for n in range(1, 4):
         fx, fy, fz = ibm.compute_force(lbm.ux, lbm.uy, *cfg.motion_state(n))
         lbm.step(fx, fy, fz)
-        assert lbm.last_kernel_cells == lbm.ncell          # one full step, one stream
+        assert lbm.last_step_kind == "full"          # one full step, one stream
     with pytest.raises(load_sub("_lib").ClbmError, match="generation 3"):
         lbm.health()
     assert ibm.region_violations() == 3
@@ -461,7 +558,7 @@ def test_timed_outside_part_changes_nothing_and_times_the_kernel(pkg, gpu):
                 evs.append(e)
             lbm.step(fx, fy, fz)
             ibm.record_force()
-            assert lbm.last_kernel_cells < lbm.ncell      # the split step ran
+            assert lbm.last_step_kind in ("split", "fused")      # the split step ran
         torch.cuda.synchronize()
         runs.append((lbm, ibm, evs))
     (a, ai, _), (b, bi, evs) = runs
