@@ -181,6 +181,10 @@ class DroneMovingIBM:
         self._side_h = None
         self.side_concurrent = None
         self._ev = []
+        # whether the last compute_force ran its marker kernels on the marker
+        # stream, and whether force sums were queued there since the last join
+        self._last_side = False
+        self._side_sums = False
         # the generation whose clamped forces the last split compute_force
         # published for a one-launch split step (clbm_step_fused)
         self._published = 0
@@ -208,6 +212,7 @@ class DroneMovingIBM:
         (airfoil_moving.py:230-255).  ``ux``/``uy`` are the solver's velocity
         fields (read straight from its populations) or any [Nz,Ny,Nx] float32
         device Field/tensor.  Returns (fx, fy, fz), this object's own fields."""
+        self._last_side = False
         if self._fields_stale:
             # the last coupled step left fx, fy unwritten: clear them so that the
             # clamp of this call (which clears the last two boxes) leaves them exact
@@ -242,6 +247,7 @@ class DroneMovingIBM:
         _lib.check(_lib.fn("ibm_clip_forces")(*clip, self.gen, st), "ibm_clip_forces")
         if region is not None:
             self._pending = True
+            self._last_side = True
             solver._add_reader(self, region)
         return self.fx, self.fy, self.fz
 
@@ -286,6 +292,16 @@ class DroneMovingIBM:
             plan = self._build_plan(solver)
         if self._pending:
             self._join()
+        if self._side_sums:
+            # force sums queued on the marker stream still read FX_k: marker
+            # kernels queued there follow them, but ones that run on the caller's
+            # stream (no split this time) must wait for them
+            probe = (C.c_int32 * 6)()
+            rc = _lib.fn("ibm_step_region")(C.byref(plan.grid), C.byref(plan.params),
+                                            *self._hull_args, float(xc), float(yc), float(theta),
+                                            probe)
+            if rc != 1:
+                self._join()
         main_h = _lib.stream_of(self.device)
         # the fork event already completes with the solver's last step when
         # nothing else went onto this stream since (no new record needed)
@@ -302,6 +318,7 @@ class DroneMovingIBM:
             _lib.check(rc, "ibm_compute_force_split")
         if rc == 1:
             self._pending = True
+            self._last_side = True
             self._published = self.gen
             solver._add_reader(self, region)
         return self.fx, self.fy, self.fz
@@ -405,11 +422,12 @@ class DroneMovingIBM:
         """Order the current stream after this object's pending kernels
         (before the caller's stream touches its buffers)."""
         self._fork_void = True
-        if self._pending:
+        if self._pending or self._side_sums:
             main_h = _lib.stream_of(self.device)
             _lib.check(_lib.fn("clbm_stream_after")(main_h, self._side_h, self._ev[1]),
                        "clbm_stream_after")
             self._pending = False
+            self._side_sums = False
 
     def _refresh_fields(self) -> None:
         """Before fx, fy, fz are read: order after pending marker kernels and,
@@ -506,15 +524,33 @@ class DroneMovingIBM:
         return out
 
     def record_force(self) -> None:
-        """Append this step's FP64 force sums to the device history."""
+        """Append this step's FP64 force sums to the device history.  After a
+        compute_force whose marker kernels ran on the marker stream the sums
+        are queued there too, after them and before the next step's: the
+        caller's stream gets no work, so the solver's next step keeps its
+        fork bound (force_history and any other reader join first)."""
         import torch
+        side = self._last_side and self._side_h is not None
         if self._hist is None or self._hist_n == self._hist.shape[0]:
+            self._join()
             cap = 1024 if self._hist is None else 2 * self._hist.shape[0]
             h = torch.zeros((cap, 2), dtype=torch.float64, device=self.device)
             if self._hist is not None:
                 h[: self._hist_n] = self._hist[: self._hist_n]
             self._hist = h
-        self.force_sums(self._hist[self._hist_n])
+            if side:
+                # the marker stream writes into the new buffer after it is filled
+                _lib.check(_lib.fn("clbm_stream_after")(self._side_h,
+                                                        _lib.stream_of(self.device), self._ev[0]),
+                           "clbm_stream_after")
+        if side:
+            t = lambda a: C.c_void_p(a._t.data_ptr())
+            _lib.call("ibm_force_sum_f64", self.N, t(self.FX_k), t(self.FY_k),
+                      t(self.dV) if self.dV is not None else C.c_void_p(0),
+                      C.c_void_p(self._hist[self._hist_n].data_ptr()), self._side_h)
+            self._side_sums = True
+        else:
+            self.force_sums(self._hist[self._hist_n])
         self._hist_n += 1
 
     def force_history(self) -> np.ndarray:
@@ -522,6 +558,7 @@ class DroneMovingIBM:
         (the force the markers put on the fluid)."""
         if self._hist is None:
             return np.zeros((0, 2))
+        self._join()
         return self._hist[: self._hist_n].cpu().numpy().copy()
 
     def restore_history(self, sums) -> None:
@@ -530,6 +567,7 @@ class DroneMovingIBM:
         record_force appends after it."""
         import torch
         a = np.asarray(sums, dtype=np.float64).reshape(-1, 2)
+        self._join()
         cap = max(1024, 1 << max(0, int(len(a) - 1).bit_length()))
         h = torch.zeros((cap, 2), dtype=torch.float64, device=self.device)
         if len(a):

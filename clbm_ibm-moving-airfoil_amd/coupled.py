@@ -41,6 +41,7 @@ def step_coupled(lbm, ibm, xc, yc, theta, vx, vy, omega, interleave: int = 2,
     if (lbm.Nx, lbm.Ny, lbm.Nz) != (ibm.Nx, ibm.Ny, ibm.Nz):
         raise ValueError("solver and IBM lattices differ")
     ibm._join()
+    ibm._last_side = False      # this step's marker work runs in the launch below
     lbm._wait_readers()
     region = (C.c_int32 * 6)()
     rc = _lib.fn("ibm_step_region")(C.byref(lbm.grid), C.byref(lbm.params), *ibm._hull_args,

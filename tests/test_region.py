@@ -506,6 +506,48 @@ def test_bound_fork_run_equals_single_stream_run(pkg, gpu, storage):
 
 
 @pytest.mark.gpu
+def test_recorded_force_sums_keep_the_fork_bound(pkg, gpu):
+    """The driver's loop records the force sums after every step: they are
+    queued on the marker stream, so the caller's stream gets no work between
+    two steps and every compute_force after the first keeps the fork bound.
+    Histories and states equal the one-stream run bit for bit, also when a
+    step in between does not split (the body off the lattice), whose marker
+    kernels on the caller's stream must wait for the sums still queued."""
+    cfg = pkg.config.naca_case(64, 32, 4, 12.0, le_x=20.0, name="sums-side")
+
+    def kin(n):
+        xc, yc, th, vx, vy, om = cfg.motion_state(n)
+        return (xc - 500.0 if n == 6 else xc), yc, th, vx, vy, om
+
+    def run(overlap):
+        lbm = pkg.CLBMSolver(cfg)
+        ibm = pkg.DroneMovingIBM(cfg, overlap=overlap)
+        bound = []
+        for n in range(1, 11):
+            fx, fy, fz = ibm.compute_force(lbm.ux, lbm.uy, *kin(n))
+            if n == 1:
+                orig = ibm._split_fn
+
+                def spy(*a, orig=orig):
+                    bound.append(a[10])
+                    return orig(*a)
+                ibm._split_fn = spy
+            lbm.step(fx, fy, fz)
+            ibm.record_force()
+        return lbm, ibm, bound
+
+    a_lbm, a_ibm, bound = run(True)
+    b_lbm, b_ibm, _ = run(False)
+    # the first record allocates the history (a join, on the caller's stream); step 6
+    # does not split (its markers run on the caller's stream, after a join); the call
+    # after each records the fork again
+    assert bound == [0, 1, 1, 1, 0, 0, 1, 1, 1]
+    np.testing.assert_array_equal(a_ibm.force_history(), b_ibm.force_history())
+    np.testing.assert_array_equal(a_lbm.f.to_numpy(), b_lbm.f.to_numpy())
+    assert a_ibm.region_violations() == 0 and a_ibm.wait_expirations() == 0
+
+
+@pytest.mark.gpu
 def test_work_between_steps_records_the_fork_again(pkg, gpu):
     """Reading or writing the body's fields, writing the populations, or a
     step without the body between a step and the next compute_force puts
