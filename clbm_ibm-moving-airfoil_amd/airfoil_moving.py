@@ -71,6 +71,11 @@ def stencil_region(hx, hy, zr, shape, xc, yc, theta):
     return (x0, y0, int(zr[0]), x1, y1, int(zr[1]))
 
 
+def _capturing() -> bool:
+    import torch
+    return torch.cuda.is_current_stream_capturing()
+
+
 def stencil_z_range(Z0, nz):
     """The z range of the stencils of markers at heights Z0 (z never moves):
     floor(z) - 1 .. floor(z) + 2, or the whole axis when a stencil wraps."""
@@ -307,7 +312,7 @@ class DroneMovingIBM:
         # nothing else went onto this stream since (no new record needed)
         fork = solver._fork
         bound = int(fork is not None and fork[0] is self and fork[1] == main_h
-                    and not self._fork_void)
+                    and not self._fork_void and not _capturing())
         solver._fork = None
         self.gen += 1
         region = self._region_buf

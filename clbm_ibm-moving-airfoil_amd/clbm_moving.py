@@ -218,9 +218,14 @@ class CLBMSolver:
             e0 = ev[0].cuda_event if ev is not None else None
             e1 = ev[1].cuda_event if ev is not None else None
             # the body's fork event completes with the step's last kernel (the
-            # next marker kernels read the populations it writes)
+            # next marker kernels read the populations it writes); not while a graph
+            # is being captured, where launches carry no events and the marker stream
+            # must rejoin this one before the capture ends (the two launches' join)
+            capturing = torch.cuda.is_current_stream_capturing()
+            done = None if capturing else ibm._ev[0]
             joined = self.fused_split == "joined"
-            if ibm._published == ibm.gen and (joined or (self.fused_split and ibm.side_concurrent)):
+            if (ibm._published == ibm.gen and (joined or (self.fused_split and ibm.side_concurrent))
+                    and not capturing):
                 # one launch; its region waits on the device for the published forces
                 # (only where the marker stream was seen running beside this one: a
                 # profiler that serialises kernels fails that check at its creation),
@@ -229,17 +234,18 @@ class CLBMSolver:
                     ibm._join()
                 _lib.check(_lib.fn("clbm_step_fused")(
                     self._gref, self._pref, state, self.state_word, region, fxp, fyp,
-                    ibm._box_ptr, ibm.gen, self.FUSED_WAIT_MS, st, e0, e1, ibm._ev[0]),
+                    ibm._box_ptr, ibm.gen, self.FUSED_WAIT_MS, st, e0, e1, done),
                     "clbm_step_fused")
                 fused = True
             else:
                 _lib.check(_lib.fn("clbm_step_split")(
                     self._gref, self._pref, state, self.state_word, region, fxp, fyp, boxp, st,
-                    ibm._side_h, ibm._ev[1], e0, e1, ibm._ev[0]), "clbm_step_split")
+                    ibm._side_h, ibm._ev[1], e0, e1, done), "clbm_step_split")
                 fused = False
             ibm._pending = False
-            self._fork = (ibm, st_h)
-            ibm._fork_void = False
+            if done is not None:
+                self._fork = (ibm, st_h)
+                ibm._fork_void = False
             self._readers.clear()
             r = list(region)
             self.last_kernel_cells = self.ncell if fused else self.ncell - (

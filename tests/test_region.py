@@ -338,6 +338,45 @@ def test_fused_split_step_needs_a_concurrent_marker_stream_or_a_join(pkg, gpu):
 
 
 @pytest.mark.gpu
+def test_captured_split_steps_replay_as_the_eager_steps(pkg, gpu):
+    """Two coupled split steps captured in a HIP graph (after an eager first
+    step, which creates the marker stream) take the two launches around the
+    join, with the fork recorded in the graph and no event bound to a launch,
+    and one replay equals the same two steps run eagerly, bit for bit."""
+    import torch
+    cfg = pkg.config.naca_case(64, 32, 4, 12.0, le_x=20.0, name="captured")
+
+    def first_step():
+        lbm = pkg.CLBMSolver(cfg)
+        ibm = pkg.DroneMovingIBM(cfg, overlap=True)
+        fx, fy, fz = ibm.compute_force(lbm.ux, lbm.uy, *cfg.motion_state(1))
+        lbm.step(fx, fy, fz)
+        return lbm, ibm
+
+    a_lbm, a_ibm = first_step()
+    for n in (2, 3):
+        fx, fy, fz = a_ibm.compute_force(a_lbm.ux, a_lbm.uy, *cfg.motion_state(n))
+        a_lbm.step(fx, fy, fz)
+    b_lbm, b_ibm = first_step()
+    torch.cuda.synchronize()
+    graph, cap = torch.cuda.CUDAGraph(), torch.cuda.Stream()
+    cap.wait_stream(torch.cuda.current_stream())
+    kinds = []
+    with torch.cuda.stream(cap):
+        with torch.cuda.graph(graph, stream=cap):
+            for n in (2, 3):
+                fx, fy, fz = b_ibm.compute_force(b_lbm.ux, b_lbm.uy, *cfg.motion_state(n))
+                b_lbm.step(fx, fy, fz)
+                kinds.append(b_lbm.last_step_kind)
+    torch.cuda.current_stream().wait_stream(cap)
+    graph.replay()
+    torch.cuda.synchronize()
+    assert kinds == ["split", "split"]
+    np.testing.assert_array_equal(a_lbm.f.to_numpy(), b_lbm.f.to_numpy())
+    np.testing.assert_array_equal(a_ibm.FX_k.to_numpy(), b_ibm.FX_k.to_numpy())
+
+
+@pytest.mark.gpu
 def test_fused_split_step_marks_a_wait_that_runs_out(pkg, gpu):
     """A one-launch split step asked for a generation that is never published
     (one ahead of the last, with 0.05 ms to wait) gives up its wait, marks the
